@@ -1,0 +1,358 @@
+// Device-side physics of one pair / one wall / one agent, shared by the
+// kernels.  Every expression keeps the operation order of the reference
+// function it cites (paths relative to the reference repository root); the
+// library is built with -ffp-contract=off so a*b+c is not fused, matching the
+// reference's numba build.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cd {
+
+constexpr double kSightSoc = 3.0;  // core/interactions.py:45
+constexpr double kFSocMax = 2e3;   // core/motion/power_law.py:52
+constexpr double kTauMax = 30.0;   // core/motion/power_law.py:243
+constexpr double kPi = 3.14159265358979323846;
+
+template <typename T>
+struct __attribute__((aligned(2 * sizeof(T)))) V2 {
+  T x, y;
+};
+
+__device__ __forceinline__ double dhypot(double a, double b) { return hypot(a, b); }
+__device__ __forceinline__ float dhypot(float a, float b) { return hypotf(a, b); }
+__device__ __forceinline__ double dsqrt(double a) { return sqrt(a); }
+__device__ __forceinline__ float dsqrt(float a) { return sqrtf(a); }
+__device__ __forceinline__ double dexp(double a) { return exp(a); }
+__device__ __forceinline__ float dexp(float a) { return expf(a); }
+
+// Ordered-integer image of a double: unsigned comparison of the images
+// equals floating-point comparison (for non-NaN values).
+__device__ __forceinline__ unsigned long long ord_bits(double v) {
+  unsigned long long b = __double_as_longlong(v);
+  return (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double from_ord_bits(unsigned long long b) {
+  b = (b & 0x8000000000000000ull) ? (b & 0x7fffffffffffffffull) : ~b;
+  return __longlong_as_double(b);
+}
+
+// Python float remainder a % b for b > 0 (numba semantics), then
+// wrap_to_pi, core/vector2D.py:8-36.
+__device__ __forceinline__ double wrap_to_pi(double rad) {
+  const double two_pi = 2.0 * kPi;
+  double r = fmod(rad, two_pi);
+  if (r != 0.0) {
+    if (r < 0.0) r += two_pi;
+  } else {
+    r = 0.0;
+  }
+  if (rad < 0.0 && r == kPi) return -kPi;
+  if (r > kPi) return r - two_pi;
+  return r;
+}
+
+// force_contact, core/motion/contact.py:12-48
+template <typename T>
+__device__ __forceinline__ void force_contact(T h, T nx, T ny, T vx, T vy, T tx, T ty, T mu,
+                                              T kappa, T damping, T &ox, T &oy) {
+  T kvt = kappa * (vx * tx + vy * ty);
+  T cvn = damping * (vx * nx + vy * ny);
+  ox = (-h) * (mu * nx - kvt * tx) + cvn * nx;
+  oy = (-h) * (mu * ny - kvt * ty) + cvn * ny;
+}
+
+// magnitude, core/motion/power_law.py:83-102
+template <typename T>
+__device__ __forceinline__ T magnitude(T tau, T tau_0) {
+  return ((T(2) / tau + T(1) / tau_0) * dexp(-tau / tau_0)) / (tau * tau);
+}
+
+// truncate, core/vector2D.py:167-187
+template <typename T>
+__device__ __forceinline__ void truncate(T &x, T &y, T l) {
+  T vlen = dhypot(x, y);
+  if (vlen > l) {
+    T s = l / vlen;
+    x *= s;
+    y *= s;
+  }
+}
+
+// Per-agent constants the owner side of a pair needs.
+template <typename T>
+struct Owner {
+  T neg_mk;  // (-mass) * k_soc, power_law.py:250
+  T tau_0, mu, kappa, damping;
+};
+
+// interaction_agent_agent_circular (core/interactions.py:51-70) seen from
+// agent i: distance_circles (core/distance.py:16-47), force_social_circular
+// (core/motion/power_law.py:213-259) and force_contact, with x = x_i - x_j,
+// v = v_i - v_j and i's constants.  Returns false when h >= SIGHT_SOC (the
+// pair adds nothing).  The j-side of the reference pair equals this function
+// evaluated with i and j swapped bit for bit (every term is odd in (x, v)
+// and IEEE negation is exact), so owner-computes reproduces both sides.
+template <typename T>
+__device__ __forceinline__ bool pair_circular(T x, T y, T vx, T vy, T r_tot, const Owner<T> &o,
+                                              T &fx, T &fy) {
+  T d = dhypot(x, y);
+  T h = d - r_tot;
+  if (!(h < T(kSightSoc))) return false;
+  T sx = T(0), sy = T(0);
+  T a = vx * vx + vy * vy;
+  T b = -(x * vx + y * vy);
+  T c = (x * x + y * y) - r_tot * r_tot;
+  T ds = dsqrt(b * b - a * c);
+  if (!(__builtin_isnan(ds) || ds == T(0) || a == T(0))) {
+    T tau = (b - ds) / a;
+    if (!(tau <= T(0) || tau > T(kTauMax))) {
+      T gx = (vx - (vx * b + x * a) / ds) / a;
+      T gy = (vy - (vy * b + y * a) / ds) / a;
+      T m = magnitude(tau, o.tau_0);
+      sx = T(0) + (o.neg_mk * gx) * m;
+      sy = T(0) + (o.neg_mk * gy) * m;
+      truncate(sx, sy, T(kFSocMax));
+    }
+  }
+  if (h < T(0)) {
+    T nx = T(0), ny = T(0);
+    if (d != T(0)) {
+      nx = x / d;
+      ny = y / d;
+    }
+    T cx, cy;
+    force_contact(h, nx, ny, vx, vy, ny, -nx, o.mu, o.kappa, o.damping, cx, cy);
+    sx += cx;
+    sy += cy;
+  }
+  fx = sx;
+  fy = sy;
+  return true;
+}
+
+// np.sign
+__device__ __forceinline__ double np_sign(double v) {
+  if (v > 0.0) return 1.0;
+  if (v < 0.0) return -1.0;
+  if (v == 0.0) return 0.0;
+  return v;
+}
+
+// distance_circle_line, core/distance.py:108-146.  Absolute coordinates in
+// fp64 (positions are fp64 in both precision modes).
+__device__ __forceinline__ void distance_circle_line(double x, double y, double r, double p0x,
+                                                     double p0y, double p1x, double p1y,
+                                                     double &h, double &nx, double &ny) {
+  double dx = p1x - p0x, dy = p1y - p0y;
+  double l_w = hypot(dx, dy);
+  double tx = dx / l_w, ty = dy / l_w;
+  double nwx = -ty, nwy = tx;
+  double q0x = x - p0x, q0y = y - p0y;
+  double q1x = x - p1x, q1y = y - p1y;
+  double l_t = -(tx * q1x + ty * q1y) - (tx * q0x + ty * q0y);
+  double d_iw;
+  if (l_t > l_w) {
+    d_iw = hypot(q0x, q0y);
+    nx = q0x / d_iw;
+    ny = q0y / d_iw;
+  } else if (l_t < -l_w) {
+    d_iw = hypot(q1x, q1y);
+    nx = q1x / d_iw;
+    ny = q1y / d_iw;
+  } else {
+    double l_n = nwx * q0x + nwy * q0y;
+    d_iw = fabs(l_n);
+    double s = np_sign(l_n);
+    nx = s * nwx;
+    ny = s * nwy;
+  }
+  h = d_iw - r;
+}
+
+// One agent of a three-circle pair: part centres (torso, left, right),
+// velocity, radii and orientation.
+struct TCView {
+  double px[3], py[3];
+  double vx, vy;
+  double r_t, r_s, r_ts, phi;
+};
+
+// interaction_agent_agent_three_circle (core/interactions.py:73-104) in the
+// reference's argument order (A = agent i, B = agent j), returning only the
+// side `side_b ? j : i`: its force and torque.  Uses distance_three_circles
+// (core/distance.py:50-105, including r_moment1 built from x0[j_min] at :103)
+// and force_social_three_circle (core/motion/power_law.py:262-363, including
+// the first-valid-tau selection at :324).  `o` holds the constants of the
+// returned side.
+__device__ __forceinline__ bool pair_three_circle(const TCView &A, const TCView &B, bool side_b,
+                                                  const Owner<double> &o, double &fx, double &fy,
+                                                  double &torque) {
+  const double rA[3] = {A.r_t, A.r_s, A.r_s};
+  const double rB[3] = {B.r_t, B.r_s, B.r_s};
+  // distance_three_circles; the parts chosen by the minimum are recorded as
+  // they are selected (no dynamic indexing of the part arrays).
+  double h_min = __builtin_nan("");
+  double nx = 0.0, ny = 0.0;
+  double selx = 0.0, sely = 0.0, selr = 0.0;  // A.part[i_min], rA[i_min]
+  double bugx = 0.0, bugy = 0.0, bugr = 0.0;  // A.part[j_min], rB[j_min]
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      double x = A.px[i] - B.px[j], y = A.py[i] - B.py[j];
+      double d = hypot(x, y);
+      double h = d - (rA[i] + rB[j]);
+      double ex = 0.0, ey = 0.0;
+      if (d != 0.0) {
+        ex = x / d;
+        ey = y / d;
+      }
+      if (h < h_min || __builtin_isnan(h_min)) {
+        h_min = h;
+        nx = ex;
+        ny = ey;
+        selx = A.px[i];
+        sely = A.py[i];
+        selr = rA[i];
+        bugx = A.px[j];
+        bugy = A.py[j];
+        bugr = rB[j];
+      }
+    }
+  }
+  if (!(h_min < kSightSoc)) return false;
+  double rmx, rmy;
+  if (!side_b) {
+    rmx = (selx + selr * nx) - A.px[0];
+    rmy = (sely + selr * ny) - A.py[0];
+  } else {
+    rmx = (bugx - bugr * nx) - B.px[0];
+    rmy = (bugy - bugr * ny) - B.py[0];
+  }
+  // force_social_three_circle
+  double sx = 0.0, sy = 0.0;
+  double vx = A.vx - B.vx, vy = A.vy - B.vy;
+  double a = vx * vx + vy * vy;
+  if (a != 0.0) {
+    int ci = 0, cj = 0;
+    double tau = __builtin_nan(""), b_min = __builtin_nan(""), d_min = __builtin_nan("");
+#pragma unroll
+    for (int pi = 0; pi < 3; ++pi) {
+#pragma unroll
+      for (int pj = 0; pj < 3; ++pj) {
+        double x = A.px[pi] - B.px[pj], y = A.py[pi] - B.py[pj];
+        double r_tot = rA[pi] + rB[pj];
+        double b = -(x * vx + y * vy);
+        double c = (x * x + y * y) - r_tot * r_tot;
+        double d = sqrt(b * b - a * c);
+        if (__builtin_isnan(d) || d == 0.0) continue;
+        double tau_new = (b - d) / a;
+        if (__builtin_isnan(tau) || (0.0 < tau_new && tau_new < tau)) {
+          ci = pi;
+          cj = pj;
+          tau = tau_new;
+          b_min = b;
+          d_min = d;
+        }
+      }
+    }
+    if (!(__builtin_isnan(tau) || tau <= 0.0)) {
+      double roix = 0.0, roiy = 0.0, rojx = 0.0, rojy = 0.0;
+      if (ci == 1 || ci == 2) {
+        double ex = A.r_ts * sin(A.phi), ey = A.r_ts * (-cos(A.phi));
+        if (ci == 1) {
+          roix += ex;
+          roiy += ey;
+        } else {
+          roix -= ex;
+          roiy -= ey;
+        }
+      }
+      if (cj == 1 || cj == 2) {
+        double ex = B.r_ts * sin(B.phi), ey = B.r_ts * (-cos(B.phi));
+        if (cj == 1) {
+          rojx += ex;
+          rojy += ey;
+        } else {
+          rojx -= ex;
+          rojy -= ey;
+        }
+      }
+      double x = A.px[0] - B.px[0], y = A.py[0] - B.py[0];
+      double rox = roix - rojx, roy = roiy - rojy;
+      double gx = (vx - (a * (x + 2.0 * rox) + b_min * vx) / d_min) / a;
+      double gy = (vy - (a * (y + 2.0 * roy) + b_min * vy) / d_min) / a;
+      double m = magnitude(tau, o.tau_0);
+      if (!side_b) {
+        sx = 0.0 + (o.neg_mk * gx) * m;
+        sy = 0.0 + (o.neg_mk * gy) * m;
+      } else {
+        sx = 0.0 - (o.neg_mk * gx) * m;
+        sy = 0.0 - (o.neg_mk * gy) * m;
+      }
+      truncate(sx, sy, kFSocMax);
+    }
+  }
+  if (h_min < 0.0) {
+    double cx, cy;
+    force_contact(h_min, nx, ny, vx, vy, ny, -nx, o.mu, o.kappa, o.damping, cx, cy);
+    if (!side_b) {
+      sx += cx;
+      sy += cy;
+    } else {
+      sx -= cx;
+      sy -= cy;
+    }
+  }
+  fx = sx;
+  fy = sy;
+  torque = rmx * sy - rmy * sx;  // cross(r_moment, f), core/vector2D.py:149
+  return true;
+}
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
+__device__ __forceinline__ void philox4x32(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    uint32_t lo0 = 0xD2511F53u * c[0], hi0 = __umulhi(0xD2511F53u, c[0]);
+    uint32_t lo1 = 0xCD9E8D57u * c[2], hi1 = __umulhi(0xCD9E8D57u, c[2]);
+    uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = lo1;
+    c[2] = n2;
+    c[3] = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+__device__ __forceinline__ double u01(uint32_t hi, uint32_t lo) {
+  unsigned long long x = ((((unsigned long long)hi) << 32) | lo) >> 11;
+  return ((double)x + 0.5) * 0x1.0p-53;
+}
+
+// Two uniforms of (agent i, step s, purpose, attempt) keyed by seed.
+__device__ __forceinline__ void draw2(uint64_t seed, int64_t i, uint64_t s, uint32_t purpose,
+                                      uint32_t attempt, double &u0, double &u1) {
+  uint32_t c[4] = {(uint32_t)i, (uint32_t)s, (uint32_t)(s >> 32), purpose | (attempt << 8)};
+  philox4x32(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  u0 = u01(c[0], c[1]);
+  u1 = u01(c[2], c[3]);
+}
+
+// Standard normal truncated to [-3, 3]: Box-Muller with rejection.
+__device__ __forceinline__ double truncnorm3(uint64_t seed, int64_t i, uint64_t s,
+                                             uint32_t purpose) {
+  double z = 0.0;
+  for (uint32_t a = 0; a < 16; ++a) {
+    double u0, u1;
+    draw2(seed, i, s, purpose, a, u0, u1);
+    z = sqrt(-2.0 * log(u0)) * cos(2.0 * kPi * u1);
+    if (fabs(z) <= 3.0) return z;
+  }
+  return copysign(3.0, z);
+}
+
+}  // namespace cd

@@ -1,0 +1,148 @@
+"""Fused multi-step driver over cd_step (six kernels per step) with HIP
+graph capture.
+
+``Stepper`` runs one MultiAgentSimulation.update() of the standard logic
+tree (crowddynamics/examples/simulations.py:123-136:
+Reset << (Integrator << (Fluctuation, Adjusting << (Navigation, Orientation),
+AgentAgentInteractions, AgentObstacleInteractions)), in the reference's
+post-order, crowddynamics/simulation/multiagent.py:51-55) per ``step()``.
+All step state (dt, time_tot, the fluctuation counter, the reductions that
+feed the next step's binning and dt) stays on the device, so a captured
+graph of K steps replays without host involvement.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from .core import segments_to_device
+from .device import DeviceAgents, Workspace, cell_capacity_for, stream_handle
+from .exceptions import CrowdDynamicsException, InvalidValue
+
+ALL_NODES = (_lib.CD_USE_FLUCT | _lib.CD_USE_NAV | _lib.CD_USE_ADJUST |
+             _lib.CD_USE_AGENT_AGENT | _lib.CD_USE_OBSTACLE | _lib.CD_USE_RESET)
+
+
+class GridOverflow(CrowdDynamicsException):
+    """The binning grid outgrew the workspace's cell capacity."""
+
+
+class Stepper:
+    def __init__(self, array, segments=None, nav=None, cell_size=3.6, dt_min=0.01,
+                 dt_max=0.01, seed=1, flags=ALL_NODES, precision='f64', cell_capacity=None,
+                 bounds=None, first_step=0, device=None):
+        self.agents = DeviceAgents(array, precision=precision, device=device)
+        dev = self.agents.device
+        self.n = self.agents.n
+        if segments is not None and len(segments):
+            self.segments = segments_to_device(segments, dev)
+        else:
+            self.segments = torch.zeros((0, 4), dtype=torch.float64, device=dev)
+            flags &= ~_lib.CD_USE_OBSTACLE
+        self.nav = nav
+        if nav is not None:
+            self.maps = torch.as_tensor(np.ascontiguousarray(nav['maps']), dtype=torch.float64,
+                                        device=dev)
+        else:
+            self.maps = None
+            flags &= ~_lib.CD_USE_NAV
+        if cell_capacity is None:
+            if bounds is not None:
+                x0, y0, x1, y1 = bounds
+            else:
+                p = array['position']
+                x0, y0 = p.min(0) if len(p) else (0.0, 0.0)
+                x1, y1 = p.max(0) if len(p) else (1.0, 1.0)
+            cell_capacity = cell_capacity_for(x1 - x0, y1 - y0, cell_size)
+        self.ws = Workspace(self.n, cell_capacity, self.agents.model, device=dev)
+        p = _lib.CDStepParams()
+        p.cell_size = float(cell_size)
+        p.dt_min = float(dt_min)
+        p.dt_max = float(dt_max)
+        p.flags = int(flags)
+        p.seed = int(seed)
+        p.segments = C.c_void_p(self.segments.data_ptr() if len(self.segments) else 0)
+        p.n_segments = len(self.segments)
+        if self.maps is not None:
+            p.nav_mode = int(nav.get('mode', 0))
+            p.nav_maps = C.c_void_p(self.maps.data_ptr())
+            p.nav_targets, p.nav_rows, p.nav_cols = self.maps.shape[:3]
+            p.nav_minx = float(nav['minx'])
+            p.nav_miny = float(nav['miny'])
+            p.nav_step = float(nav['step'])
+        p.cell_capacity = self.ws.cell_capacity
+        self.params = p
+        self.flags = flags
+        self.prepare(first_step)
+
+    def prepare(self, first_step=0):
+        """Seed the device reductions from the current state (also after any
+        host-side change of positions or velocities)."""
+        _lib.check(_lib.lib().cd_step_prepare(self.agents.ref(), C.byref(self.params),
+                                              self.ws.ptr, self.ws.nbytes, int(first_step),
+                                              stream_handle()), 'step_prepare')
+
+    def step(self, k=1):
+        L = _lib.lib()
+        s = stream_handle()
+        for _ in range(k):
+            _lib.check(L.cd_step(self.agents.ref(), C.byref(self.params), self.ws.ptr,
+                                 self.ws.nbytes, s), 'step')
+
+    def capture(self, k):
+        """HIP graph of k consecutive steps (launched on the current stream)."""
+        if torch.cuda.current_stream() == torch.cuda.default_stream():
+            raise InvalidValue('capture needs a non-default stream (use torch.cuda.stream)')
+        L = _lib.lib()
+        s = stream_handle()
+        _lib.check(L.cd_graph_begin(s), 'graph_begin')
+        try:
+            self.step(k)
+        except Exception:
+            ex = C.c_void_p()
+            L.cd_graph_end(s, C.byref(ex))
+            if ex.value:
+                L.cd_graph_destroy(ex)
+            raise
+        ex = C.c_void_p()
+        _lib.check(L.cd_graph_end(s, C.byref(ex)), 'graph_end')
+        return Graph(ex)
+
+    def state(self):
+        return self.ws.state()
+
+    def check(self):
+        """Raise if the device recorded a grid overflow or non-finite positions."""
+        st = self.state()
+        if st.status & _lib.CD_STATUS_NONFINITE:
+            raise CrowdDynamicsException('non-finite agent position')
+        if st.status & _lib.CD_STATUS_GRID_OVERFLOW:
+            raise GridOverflow('agents spread beyond the cell capacity (%d cells)'
+                               % self.ws.cell_capacity)
+        return st
+
+    def download(self, array):
+        torch.cuda.current_stream().synchronize()
+        self.agents.download(array)
+        return array
+
+
+class Graph:
+    """An instantiated HIP graph (cd_graph_* wrapper)."""
+
+    def __init__(self, exec_ptr):
+        self.ptr = exec_ptr
+
+    def launch(self):
+        _lib.check(_lib.lib().cd_graph_launch(self.ptr, stream_handle()), 'graph_launch')
+
+    def __del__(self):
+        if getattr(self, 'ptr', None) is not None and self.ptr.value:
+            try:
+                _lib.lib().cd_graph_destroy(self.ptr)
+            except Exception:
+                pass
+            self.ptr = None

@@ -1,0 +1,222 @@
+/*
+ * cdamd.h -- C ABI of the MI355X (gfx950) per-timestep agent update.
+ *
+ * All pointers in cd_agents_t and every `ws` workspace are DEVICE pointers
+ * owned by the caller.  Every call is stream-ordered on `stream` (a
+ * hipStream_t), performs no host synchronisation and no allocation, and may
+ * be captured into a HIP graph, except the calls documented as synchronous
+ * (cd_read_state, cd_graph_*).  Calls return 0 on success or a negative
+ * CD_ERR_* code; kernels record data-dependent failures (grid overflow,
+ * non-finite positions) as sticky bits in the workspace state, read back by
+ * cd_read_state.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the reference repository root).
+ */
+#ifndef CDAMD_H
+#define CDAMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void *cd_stream_t; /* hipStream_t */
+
+enum {
+  CD_OK = 0,
+  CD_ERR_ARG = -1,         /* null pointer, bad size or bad parameter */
+  CD_ERR_TYPE = -2,        /* not a circular / three-circle agent array (InvalidType) */
+  CD_ERR_WORKSPACE = -3,   /* workspace smaller than cd_workspace_bytes() */
+  CD_ERR_UNSUPPORTED = -4, /* combination not built (e.g. three-circle in fp32) */
+  CD_ERR_HIP = -5          /* a HIP runtime call or launch failed */
+};
+
+enum { CD_CIRCULAR = 0, CD_THREE_CIRCLE = 1 };
+enum { CD_F64 = 0, CD_F32 = 1 };
+
+/* Sticky status bits kept in the workspace state. */
+enum {
+  CD_STATUS_GRID_OVERFLOW = 1, /* binning grid larger than cell_capacity: pairs skipped */
+  CD_STATUS_NONFINITE = 2      /* a non-finite position reached the binning */
+};
+
+/* Device structure-of-arrays view of agent_type_circular /
+ * agent_type_three_circle records (crowddynamics/simulation/agents.py:299-307,
+ * 447-448).  Vectors are interleaved [n][2].  `position`, `position_ls`,
+ * `position_rs` and every scalar field are fp64; `velocity`,
+ * `target_direction`, `force` and `force_prev` have the type named by `real`
+ * (fp64, or fp32 in the fp32 mode).  Three-circle-only fields may be NULL for
+ * circular agents. */
+typedef struct cd_agents {
+  int64_t n;
+  int32_t model; /* CD_CIRCULAR / CD_THREE_CIRCLE */
+  int32_t real;  /* CD_F64 / CD_F32 */
+  double *position;
+  void *velocity, *target_direction, *force, *force_prev;
+  double *position_ls, *position_rs;
+  const double *radius, *r_t, *r_s, *r_ts, *mass, *inertia_rot;
+  const double *target_velocity, *target_angular_velocity;
+  const double *tau_adj, *k_soc, *tau_0, *mu, *kappa, *damping, *std_rand_force;
+  double *orientation, *angular_velocity, *target_orientation, *torque, *torque_prev;
+  const double *tau_rot, *std_rand_torque;
+  const int64_t *target;
+} cd_agents_t;
+
+/* Host copy of the workspace state scalars. */
+typedef struct cd_state {
+  double dt;          /* dt of the last integration (simulation.data['dt']) */
+  double time_tot;    /* sum of dt (simulation.data['time_tot']) */
+  double origin[2];   /* binning origin of the last binning */
+  int64_t shape[2];   /* binning grid shape (nx, ny) of the last binning */
+  int64_t status;     /* CD_STATUS_* bits */
+  int64_t step;       /* steps taken by cd_step */
+  double v_max;       /* max |v| the last integration used */
+  double v0_max;      /* max target_velocity the last integration used */
+} cd_state_t;
+
+/* Offsets (bytes from the workspace base) of the arrays a caller may read.
+ * All int32. */
+typedef struct cd_layout {
+  int64_t points_indices; /* [n]: agents sorted stably by flat cell */
+  int64_t cells_count;    /* [cell_capacity] */
+  int64_t cells_offset;   /* [cell_capacity + 1]: exclusive prefix sum */
+  int64_t cell_of;        /* [n]: flat cell of each agent */
+  int64_t total_bytes;
+} cd_layout_t;
+
+/* ---- workspace --------------------------------------------------------- */
+
+/* Bytes of workspace for n agents, a grid of at most cell_capacity cells and
+ * the given model.  Fills *layout when not NULL. */
+int64_t cd_workspace_bytes(int64_t n, int64_t cell_capacity, int32_t model,
+                           cd_layout_t *layout);
+/* Zeroes the workspace state and counters (once, before first use). */
+int cd_workspace_init(void *ws, int64_t ws_bytes, int64_t n, int64_t cell_capacity,
+                      int32_t model, cd_stream_t stream);
+/* Synchronous: copies the state scalars to *out. */
+int cd_read_state(const void *ws, cd_state_t *out, cd_stream_t stream);
+/* Clears the sticky status bits (stream-ordered). */
+int cd_clear_status(void *ws, cd_stream_t stream);
+
+/* ---- block-list binning -------------------------------------------------- */
+
+/* cell_lists.add_to_cells(points, cell_size) (called at
+ * crowddynamics/core/interactions.py:192-193): counting sort of the agents
+ * into square cells of side cell_size anchored at the positions' minimum.
+ * Results in the workspace arrays named by cd_layout_t and the state's
+ * origin/shape.  Deterministic: points_indices is the stable order. */
+int cd_add_to_cells(const cd_agents_t *agents, double cell_size, void *ws,
+                    int64_t ws_bytes, int64_t cell_capacity, cd_stream_t stream);
+
+/* ---- LogicNode operators (crowddynamics/simulation/logic.py) ------------- */
+
+/* agent_agent_block_list(agents, cell_size) (crowddynamics/core/interactions.py:191-205):
+ * bins, then adds every adjacent pair's social + contact force (and torque)
+ * to force (torque). */
+int cd_agent_agent_block_list(const cd_agents_t *agents, double cell_size, void *ws,
+                              int64_t ws_bytes, int64_t cell_capacity, cd_stream_t stream);
+
+/* agent_obstacle(agents, obstacles) (crowddynamics/core/interactions.py:208-214):
+ * segments is obstacle_type_linear[W] = {p0: f8[2], p1: f8[2]}, [W][4] fp64. */
+int cd_agent_obstacle(const cd_agents_t *agents, const double *segments, int64_t n_segments,
+                      cd_stream_t stream);
+
+/* force_adjust_agents(agents) (crowddynamics/core/motion/adjusting.py:98-108) */
+int cd_force_adjust_agents(const cd_agents_t *agents, cd_stream_t stream);
+/* torque_adjust_agents(agents) (crowddynamics/core/motion/adjusting.py:111-121) */
+int cd_torque_adjust_agents(const cd_agents_t *agents, cd_stream_t stream);
+/* orient_towards_target_direction(agents) (crowddynamics/core/steering/orientation.py:17-21) */
+int cd_orient_towards_target_direction(const cd_agents_t *agents, cd_stream_t stream);
+
+/* Navigation.update (crowddynamics/simulation/logic.py:149-165) with
+ * meshgrid().indicer (core/steering/quickest_path.py:41-44) and getdefault
+ * (core/steering/navigation.py:67-78).  mode 0: `maps` = n_targets direction
+ * maps [rows][cols][2] read at the truncated index (reference).  mode 1:
+ * `maps` = n_targets potentials [rows][cols], e0 = -grad/|grad| of the
+ * bilinear interpolant.  Agents whose target is outside [0, n_targets) keep
+ * their direction. */
+int cd_navigation(const cd_agents_t *agents, const double *maps, int64_t n_targets,
+                  int64_t rows, int64_t cols, double minx, double miny, double step,
+                  int32_t mode, cd_stream_t stream);
+
+/* Fluctuation.update (crowddynamics/simulation/logic.py:78-86) with
+ * force_fluctuation / torque_fluctuation (core/motion/fluctuation.py:14-62):
+ * adds the step's noise drawn from a Philox4x32-10 stream keyed by seed and
+ * counted by (agent, step). */
+int cd_force_fluctuation(const cd_agents_t *agents, uint64_t seed, uint64_t step,
+                         cd_stream_t stream);
+
+/* velocity_verlet_integrator(agents, dt_min, dt_max)
+ * (crowddynamics/core/integrator.py:209-256): adaptive dt (:29-97) on the
+ * device, then the reference's translational (and rotational + shoulders)
+ * update.  dt and time_tot are kept in the workspace state. */
+int cd_velocity_verlet_integrator(const cd_agents_t *agents, double dt_min, double dt_max,
+                                  void *ws, int64_t ws_bytes, cd_stream_t stream);
+
+/* Reset.update (crowddynamics/simulation/logic.py:59-64) */
+int cd_reset(const cd_agents_t *agents, cd_stream_t stream);
+
+/* ---- fused step ---------------------------------------------------------- */
+
+enum {
+  CD_USE_FLUCT = 1,
+  CD_USE_NAV = 2,
+  CD_USE_ADJUST = 4,        /* Adjusting (+ Orientation for three-circle) */
+  CD_USE_AGENT_AGENT = 8,
+  CD_USE_OBSTACLE = 16,
+  CD_USE_RESET = 32
+};
+
+typedef struct cd_step_params {
+  double cell_size, dt_min, dt_max;
+  int32_t flags;    /* CD_USE_* */
+  int32_t nav_mode; /* as cd_navigation */
+  uint64_t seed;
+  const double *segments;
+  int64_t n_segments;
+  const double *nav_maps;
+  int64_t nav_targets, nav_rows, nav_cols;
+  double nav_minx, nav_miny, nav_step;
+  int64_t cell_capacity;
+} cd_step_params_t;
+
+/* One MultiAgentSimulation.update() (crowddynamics/simulation/multiagent.py:51-55)
+ * of the tree Reset << (Integrator << (Fluctuation, Adjusting << (Navigation,
+ * Orientation), AgentAgentInteractions, AgentObstacleInteractions))
+ * (crowddynamics/examples/simulations.py:123-136) in six kernels.  The
+ * fluctuation step counter lives on the device.  cd_step_prepare must run
+ * once before the first cd_step (and after any host-side change of
+ * positions or velocities). */
+int cd_step_prepare(const cd_agents_t *agents, const cd_step_params_t *params, void *ws,
+                    int64_t ws_bytes, uint64_t first_step, cd_stream_t stream);
+int cd_step(const cd_agents_t *agents, const cd_step_params_t *params, void *ws,
+            int64_t ws_bytes, cd_stream_t stream);
+
+/* Synchronous: runs `steps` cd_step calls and returns in kernel_ms[6] the
+ * summed GPU time of each of the six step kernels (count, scan, scatter,
+ * sort+gather, forces, integrate), measured with HIP events recorded on
+ * `stream` around each launch.  No reference counterpart (measurement). */
+int cd_step_timed(const cd_agents_t *agents, const cd_step_params_t *params, void *ws,
+                  int64_t ws_bytes, int64_t steps, double *kernel_ms, cd_stream_t stream);
+
+/* ---- HIP graphs (synchronous helpers) ------------------------------------- */
+
+int cd_graph_begin(cd_stream_t stream);
+int cd_graph_end(cd_stream_t stream, void **graph_exec);
+int cd_graph_launch(void *graph_exec, cd_stream_t stream);
+int cd_graph_destroy(void *graph_exec);
+
+/* ---- misc ----------------------------------------------------------------- */
+
+/* Philox4x32-10 on the device for one counter (testing the noise stream). */
+int cd_philox4x32(const uint32_t *ctr_key_dev, uint32_t *out_dev, cd_stream_t stream);
+const char *cd_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CDAMD_H */

@@ -1,0 +1,377 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Bars: cell indices, block lists and pair sets bit-exact; fp64 forces and
+torques within 1e-12 of the per-agent force scale (sum of the magnitudes of
+the summed terms: the two sides sum the same terms in different orders);
+per-agent operators that keep the reference's order bit-for-bit up to libm
+ulps (1e-14 relative).
+"""
+import numpy as np
+import pytest
+import torch
+
+from crowddynamics_amd import core
+from crowddynamics_amd import examples as E
+from crowddynamics_amd.agents import empty_agents, inertia_rot, shoulders_host
+from crowddynamics_amd.device import DeviceAgents, Workspace, cell_capacity_for
+from crowddynamics_amd.stepper import Stepper
+from oracle import oracle as O
+from tests.helpers import assert_vec_close, max_rel
+
+pytestmark = pytest.mark.gpu
+
+F64_FORCE_RTOL = 1e-12
+OP_RTOL = 1e-14
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope='module')
+def hallway():
+    return E.hallway()
+
+
+@pytest.fixture(scope='module')
+def room():
+    return E.room()
+
+
+@pytest.fixture(scope='module')
+def corridor_small():
+    return E.corridor(n=3000, width=80.0, height=40.0, n_obstacles=40)
+
+
+def _ws_for(arr, cell):
+    p = arr['position']
+    span = p.max(0) - p.min(0)
+    return Workspace(len(arr), cell_capacity_for(span[0], span[1], cell), 0 if
+                     'position_ls' not in arr.dtype.names else 1)
+
+
+def _perturb(arr, seed, scale=0.3):
+    """Random velocities so the social force is active in every direction."""
+    rng = np.random.default_rng(seed)
+    arr = arr.copy()
+    arr['velocity'] = rng.normal(0, scale, arr['velocity'].shape) + arr['velocity']
+    return arr
+
+
+# ---------------------------------------------------------------- binning --
+
+def _check_binning(arr, cell):
+    say('binning n=%d cell=%g' % (len(arr), cell))
+    dev = DeviceAgents(arr)
+    ws = _ws_for(arr, cell)
+    core.add_to_cells(dev, cell, ws)
+    got = ws.block_list()
+    ref = O.add_to_cells(arr['position'], cell)
+    names = ('points_indices', 'cells_count', 'cells_offset', 'shape', 'origin', 'cell_of')
+    for name, g, r in zip(names, got, ref):
+        if name in ('shape',):
+            assert tuple(g) == tuple(r), name
+        elif name == 'origin':
+            assert np.array_equal(np.asarray(g), np.asarray(r)), name
+        else:
+            assert np.array_equal(np.asarray(g), np.asarray(r)), name
+    assert ws.state().status == 0
+
+
+def test_binning_hallway(hallway):
+    _check_binning(hallway.agents, 3.6)
+    _check_binning(hallway.agents, 0.6)
+
+
+def test_binning_room(room):
+    _check_binning(room.agents, 0.6)
+
+
+def test_binning_edge_cases():
+    rng = np.random.default_rng(5)
+    # single agent
+    a = empty_agents(1)
+    a['position'] = (3.0, -2.0)
+    _check_binning(a, 0.6)
+    # points exactly on cell boundaries, negative coordinates, repeated points
+    a = empty_agents(64)
+    k = np.arange(64)
+    a['position'][:, 0] = -5.0 + (k % 8) * 0.5
+    a['position'][:, 1] = -1.0 + (k // 8) * 0.25
+    a['position'][7] = a['position'][3]
+    _check_binning(a, 0.5)
+    # all agents in one cell
+    a = empty_agents(40)
+    a['position'] = rng.uniform(10.0, 10.3, (40, 2))
+    _check_binning(a, 3.6)
+    # a line of agents (ny = 1)
+    a = empty_agents(100)
+    a['position'][:, 0] = rng.uniform(0, 50, 100)
+    a['position'][:, 1] = 7.0
+    _check_binning(a, 0.6)
+
+
+# ------------------------------------------------------------ pair forces --
+
+def _check_pairs(arr, cell, precision='f64', rtol=F64_FORCE_RTOL):
+    say('pairs n=%d cell=%g %s model=%s' % (len(arr), cell, precision,
+                                            'tc' if 'position_ls' in arr.dtype.names else 'c'))
+    arr = arr.copy()
+    arr['force'] = 0.0
+    tc = 'position_ls' in arr.dtype.names
+    if tc:
+        arr['torque'] = 0.0
+    ref = arr.copy()
+    fs, ts = O.agent_agent_scale(ref, cell)
+    dev = DeviceAgents(arr, precision=precision)
+    ws = _ws_for(arr, cell)
+    core.agent_agent_block_list(dev, cell, ws)
+    out = arr.copy()
+    dev.download(out)
+    assert ws.state().status == 0
+    assert_vec_close(out['force'], ref['force'], fs, rtol, 'force')
+    if tc:
+        assert_vec_close(out['torque'], ref['torque'], ts, rtol, 'torque')
+    return max_rel(out['force'], ref['force'], fs)
+
+
+def test_pairs_hallway(hallway):
+    _check_pairs(_perturb(hallway.agents, 1), 3.6)
+    _check_pairs(_perturb(hallway.agents, 2), 0.6)
+
+
+def test_pairs_room(room):
+    _check_pairs(_perturb(room.agents, 3), 0.6)
+
+
+def test_pairs_contact_heavy():
+    """Overlapping agents (contact + friction + damping terms active)."""
+    rng = np.random.default_rng(11)
+    n = 400
+    a = empty_agents(n)
+    a['radius'] = rng.uniform(0.22, 0.28, n)
+    a['mass'] = rng.normal(80, 8, n)
+    a['position'] = rng.uniform(0, 8, (n, 2))
+    a['velocity'] = rng.normal(0, 1, (n, 2))
+    a['target_velocity'] = 1.2
+    _check_pairs(a, 0.6)
+    _check_pairs(a, 3.6)
+
+
+def test_pairs_three_circle(corridor_small):
+    _check_pairs(_perturb(corridor_small.agents, 4), 0.6)
+
+
+def test_pairs_three_circle_contact():
+    rng = np.random.default_rng(12)
+    n = 300
+    a = empty_agents(n, 'three_circle')
+    a['radius'] = 0.25
+    a['r_t'] = 0.15
+    a['r_s'] = 0.1
+    a['r_ts'] = 0.15
+    a['mass'] = rng.normal(80, 8, n)
+    a['position'] = rng.uniform(0, 7, (n, 2))
+    a['velocity'] = rng.normal(0, 1, (n, 2))
+    a['orientation'] = rng.uniform(-np.pi, np.pi, n)
+    shoulders_host(a)
+    _check_pairs(a, 0.6)
+
+
+def test_pairs_fp32_error_bound(room):
+    """fp32 mode: forces within 2e-4 of the fp64 oracle's force scale."""
+    r = _check_pairs(_perturb(room.agents, 3), 0.6, precision='f32', rtol=2e-4)
+    say('fp32 max scaled error %.3e' % r)
+
+
+# ------------------------------------------------------ per-agent operators --
+
+def _roundtrip(arr, fn_dev, fn_ref, fields, rtol=OP_RTOL, precision='f64'):
+    ref = arr.copy()
+    fn_ref(ref)
+    dev = DeviceAgents(arr, precision=precision)
+    fn_dev(dev)
+    out = arr.copy()
+    dev.download(out)
+    for f in fields:
+        g, r = out[f], ref[f]
+        scale = np.abs(r) if r.ndim == 1 else np.hypot(*r.T)
+        assert_vec_close(g, r, scale, rtol, f)
+
+
+def test_walls(hallway, room, corridor_small):
+    for scn in (hallway, room, corridor_small):
+        say('walls', scn.name)
+        arr = scn.agents.copy()
+        # push a few agents into the walls
+        arr['position'][:20, 1] = scn.bounds[1] + 0.1
+        if 'position_ls' in arr.dtype.names:
+            shoulders_host(arr)
+        seg_dev = core.segments_to_device(scn.segments)
+        _roundtrip(arr, lambda d: core.agent_obstacle(d, seg_dev),
+                   lambda r: O.agent_obstacle(r, scn.segments),
+                   ['force'] + (['torque'] if 'torque' in arr.dtype.names else []))
+
+
+def test_adjust_orient(corridor_small, room):
+    say('adjust circular')
+    _roundtrip(_perturb(room.agents, 7), core.force_adjust_agents, O.force_adjust_agents,
+               ['force'])
+    say('adjust three-circle')
+    arr = _perturb(corridor_small.agents, 8)
+    arr['orientation'] = np.random.default_rng(3).uniform(-np.pi, np.pi, len(arr))
+    _roundtrip(arr, lambda d: (core.orient_towards_target_direction(d),
+                               core.force_adjust_agents(d), core.torque_adjust_agents(d)),
+               lambda r: (O.orient_towards_target_direction(r), O.force_adjust_agents(r),
+                          O.torque_adjust_agents(r)),
+               ['force', 'torque', 'target_orientation'])
+
+
+def test_navigation_modes(room):
+    for mode in (0, 1):
+        say('navigation mode', mode)
+        arr = room.agents.copy()
+        maps_np = room.nav['directions'][None] if mode == 0 else room.nav['potential'][None]
+        maps = torch.as_tensor(np.ascontiguousarray(maps_np), device='cuda')
+        # some agents outside the grid and some without a target
+        arr['position'][:10] = (-3.0, 50.0)
+        arr['target'][10:20] = -1
+        _roundtrip(arr, lambda d: core.navigation(d, maps, 0.0, 0.0, 0.1, mode),
+                   lambda r: O.navigation(r, maps_np, 0.0, 0.0, 0.1, mode),
+                   ['target_direction'], rtol=0.0 if mode == 0 else OP_RTOL)
+
+
+def test_fluctuation(corridor_small, hallway):
+    for scn in (hallway, corridor_small):
+        say('fluctuation', scn.name)
+        arr = scn.agents.copy()
+        _roundtrip(arr, lambda d: core.force_fluctuation(d, 17, 42),
+                   lambda r: O.fluctuation(r, 17, 42),
+                   ['force'] + (['torque'] if 'torque' in arr.dtype.names else []),
+                   rtol=1e-13)
+
+
+def test_integrator(hallway, corridor_small):
+    for scn in (hallway, corridor_small):
+        say('integrator', scn.name)
+        arr = _perturb(scn.agents, 9)
+        rng = np.random.default_rng(4)
+        arr['force'] = rng.normal(0, 50, arr['force'].shape)
+        arr['force_prev'] = rng.normal(0, 50, arr['force'].shape)
+        if 'torque' in arr.dtype.names:
+            arr['torque'] = rng.normal(0, 5, len(arr))
+            arr['torque_prev'] = rng.normal(0, 5, len(arr))
+            arr['angular_velocity'] = rng.normal(0, 1, len(arr))
+        ref = arr.copy()
+        dt_ref = O.velocity_verlet_integrator(ref, 0.001, 0.01)
+        dev = DeviceAgents(arr)
+        ws = Workspace(len(arr), 16, dev.model)
+        core.velocity_verlet_integrator(dev, 0.001, 0.01, ws)
+        st = ws.state()
+        assert st.dt == dt_ref
+        out = arr.copy()
+        dev.download(out)
+        for f in ('position', 'velocity', 'force_prev') + (
+                ('orientation', 'angular_velocity', 'position_ls', 'position_rs')
+                if 'torque' in arr.dtype.names else ()):
+            r = ref[f]
+            scale = np.abs(r) if r.ndim == 1 else np.hypot(*r.T)
+            assert_vec_close(out[f], r, scale, 1e-15, f)
+
+
+# -------------------------------------------------------------- full step --
+
+def _oracle_scenario(scn, fluct=True, nav=True):
+    nv = scn.nav
+    return O.Scenario(cell_size=scn.cell_size, dt_min=scn.dt_min, dt_max=scn.dt_max,
+                      seed=scn.seed, segments=scn.segments,
+                      nav_maps=nv['maps'] if (nv and nav) else None,
+                      nav_mode=nv['mode'] if nv else 0,
+                      nav_origin=(nv['minx'], nv['miny']) if nv else (0.0, 0.0),
+                      nav_step=nv['step'] if nv else 0.1, fluct=fluct)
+
+
+def _stepper(scn, precision='f64', fluct=True):
+    from crowddynamics_amd import _lib
+    from crowddynamics_amd.stepper import ALL_NODES
+    flags = ALL_NODES if fluct else ALL_NODES & ~_lib.CD_USE_FLUCT
+    return Stepper(scn.agents.copy(), segments=scn.segments, nav=scn.nav,
+                   cell_size=scn.cell_size, dt_min=scn.dt_min, dt_max=scn.dt_max,
+                   seed=scn.seed, flags=flags, precision=precision, bounds=scn.bounds)
+
+
+def test_step_one(hallway, room):
+    for scn in (hallway, room):
+        say('fused one step', scn.name)
+        ref = scn.agents.copy()
+        dts = O.run(ref, _oracle_scenario(scn), 1)
+        st = _stepper(scn)
+        st.step(1)
+        out = st.download(scn.agents.copy())
+        s = st.check()
+        assert s.dt == dts[0]
+        assert s.step == 1
+        # one step moves x by v*dt: compare against the step's displacement scale
+        for f in ('position', 'velocity', 'force_prev', 'target_direction'):
+            r = ref[f]
+            scale = np.hypot(*r.T) + 1.0
+            assert_vec_close(out[f], r, scale, 1e-13, f)
+        assert np.all(out['force'] == 0.0)
+
+
+def test_step_many_hallway(hallway):
+    say('fused 1000 steps hallway')
+    ref = hallway.agents.copy()
+    dts = O.run(ref, _oracle_scenario(hallway), 1000)
+    st = _stepper(hallway)
+    torch.cuda.synchronize()
+    st.step(1000)
+    out = st.download(hallway.agents.copy())
+    s = st.check()
+    err = np.abs(out['position'] - ref['position']).max()
+    say('hallway 1000-step max |dx| = %.3e m, time %.6f vs %.6f' % (err, s.time_tot, dts.sum()))
+    assert abs(s.time_tot - dts.sum()) < 1e-9
+    assert err < 1e-6
+
+
+def test_step_graph_replay_matches_eager(hallway):
+    say('graph replay vs eager')
+    a = _stepper(hallway)
+    a.step(20)
+    out_a = a.download(hallway.agents.copy())
+    b = _stepper(hallway)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        g = b.capture(5)
+        for _ in range(4):
+            g.launch()
+        s.synchronize()
+    out_b = b.download(hallway.agents.copy())
+    assert np.array_equal(out_a['position'], out_b['position'])
+    assert np.array_equal(out_a['velocity'], out_b['velocity'])
+    assert b.state().step == 20
+
+
+def test_step_deterministic(room):
+    say('determinism room 20 steps')
+    outs = []
+    for _ in range(2):
+        st = _stepper(room)
+        st.step(20)
+        outs.append(st.download(room.agents.copy()))
+    assert np.array_equal(outs[0]['position'], outs[1]['position'])
+    assert np.array_equal(outs[0]['velocity'], outs[1]['velocity'])
+
+
+def test_step_room_fp64_and_fp32(room):
+    say('room 20 steps fp64 / fp32 vs oracle')
+    ref = room.agents.copy()
+    O.run(ref, _oracle_scenario(room), 20)
+    for prec, tol in (('f64', 1e-9), ('f32', 1e-3)):
+        st = _stepper(room, precision=prec)
+        st.step(20)
+        out = st.download(room.agents.copy())
+        st.check()
+        err = np.abs(out['position'] - ref['position']).max()
+        say('  %s max |dx| after 20 steps: %.3e m' % (prec, err))
+        assert err < tol
