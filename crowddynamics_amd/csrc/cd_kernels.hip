@@ -46,6 +46,9 @@ struct DevState {
   int C;      // cells of the last binning
   int C_clear;  // cells whose counts are non-zero (to clear before next count)
   int oob;      // an agent fell outside the grid (stale bounding box): binning invalid
+  unsigned int scan_epoch;   // tags the scan's tile descriptors of this binning
+  unsigned int scan_ticket;  // dynamic tile index of the look-back scan
+  int pad1;
   long long status;
   unsigned long long step;
   double v_max, v0_max;
@@ -54,8 +57,12 @@ struct DevState {
 constexpr int64_t kAlign = 256;
 inline int64_t align_up(int64_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
 struct Layout {
-  int64_t state, cell_of, rank, pidx, scell, count, offset, spos, svel, sr;
+  int64_t state, tiles, cell_of, rank, pidx, scell, count, offset, spos, svel, sr;
   int64_t sls, srs, sphi, srt, srs_r, srts;
   int64_t total;
 };
@@ -69,6 +76,7 @@ Layout make_layout(int64_t n, int64_t cap, int32_t model) {
     return r;
   };
   L.state = take(4096);
+  L.tiles = take(8 * ((cap + kScanTile - 1) / kScanTile + 1));
   L.cell_of = take(4 * n);
   L.rank = take(4 * n);
   L.pidx = take(4 * n);
@@ -144,6 +152,8 @@ Agents<T> typed(const cd_agents_t *a) {
 
 struct Work {
   DevState *st;
+  unsigned long long *tiles;
+  int ntiles;
   int *cell_of, *rank, *pidx, *scell, *count, *offset;
   double2 *spos;
   void *svel;
@@ -157,6 +167,8 @@ Work carve(void *ws, const Layout &L, int64_t cap) {
   char *b = (char *)ws;
   Work W;
   W.st = (DevState *)(b + L.state);
+  W.tiles = (unsigned long long *)(b + L.tiles);
+  W.ntiles = (int)((cap + kScanTile - 1) / kScanTile);
   W.cell_of = (int *)(b + L.cell_of);
   W.rank = (int *)(b + L.rank);
   W.pidx = (int *)(b + L.pidx);
@@ -190,8 +202,10 @@ __device__ __forceinline__ void reset_acc(DevState *st) {
   st->red[1][7] = 0ull;
 }
 
-__global__ void __launch_bounds__(256) k_ws_init(DevState *st, int *count, int cap) {
+__global__ void __launch_bounds__(256) k_ws_init(DevState *st, int *count, int cap,
+                                                 unsigned long long *tiles, int ntiles) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int k = t; k < ntiles; k += gridDim.x * blockDim.x) tiles[k] = 0ull;
   if (t == 0) {
     memset(st, 0, sizeof(DevState));
     for (int k = 0; k < 8; ++k) st->red[0][k] = 0ull;
@@ -349,6 +363,9 @@ __global__ void __launch_bounds__(256) k_bin_count(const double2 *pos, int n, do
     st->valid = g.ok ? 1 : 0;
     st->C = g.ok ? g.nx * g.ny : 0;
     st->C_clear = g.ok ? g.nx * g.ny : 0;
+    st->scan_epoch = (st->scan_epoch + 1u) & 0x3fffffffu;
+    if (st->scan_epoch == 0u) st->scan_epoch = 1u;
+    st->scan_ticket = 0u;
     if (!g.ok) st->status |= CD_STATUS_GRID_OVERFLOW;
   }
   if (!g.ok || i >= n) return;
@@ -366,35 +383,105 @@ __global__ void __launch_bounds__(256) k_bin_count(const double2 *pos, int n, do
   rank[i] = atomicAdd(&count[c], 1);
 }
 
-// K1b: exclusive scan of the counts (one workgroup of 1024 threads).
-__global__ void __launch_bounds__(1024) k_bin_scan(DevState *st, const int *count,
-                                                   int *offset) {
-  __shared__ int sums[1024];
-  if (st->oob && threadIdx.x == 0) {
-    st->valid = 0;
-    st->status |= CD_STATUS_GRID_OVERFLOW;
+// K1b: exclusive scan of the counts, single pass with decoupled look-back.
+// Each workgroup takes the next tile of kScanTile cells (dynamic ticket, so
+// every tile's predecessors are already resident), publishes its aggregate,
+// then walks back over predecessors' descriptors until an inclusive prefix.
+// A descriptor packs (epoch:30 | flag:2 | value:32); the epoch of this
+// binning makes stale descriptors of earlier steps invisible.  The wait is
+// bounded: past kSpinLimit polls the binning is marked invalid instead.
+constexpr long long kSpinLimit = 1ll << 24;
+
+__device__ __forceinline__ unsigned long long scan_pack(unsigned epoch, unsigned flag, int v) {
+  return ((unsigned long long)epoch << 34) | ((unsigned long long)flag << 32) | (unsigned)v;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_bin_scan(DevState *st, const int *count,
+                                                           int *offset,
+                                                           unsigned long long *tiles) {
+  __shared__ int s_tile, s_prefix;
+  __shared__ int s_wave[kScanThreads / 64];
+  if (st->oob) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->valid = 0;
+      st->status |= CD_STATUS_GRID_OVERFLOW;
+    }
+    return;
   }
-  if (!st->valid || st->oob) return;
+  if (!st->valid) return;
   const int C = st->C;
-  const int t = threadIdx.x;
-  const int per = (C + 1023) / 1024;
-  const int b = t * per, e = min(b + per, C);
-  int s = 0;
-  for (int c = b; c < e; ++c) s += count[c];
-  sums[t] = s;
+  const unsigned epoch = st->scan_epoch;
+  const int ntiles = (C + kScanTile - 1) / kScanTile;
+  if (threadIdx.x == 0) s_tile = (int)atomicAdd(&st->scan_ticket, 1u);
   __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {
-    int v = (t >= o) ? sums[t - o] : 0;
-    __syncthreads();
-    sums[t] += v;
-    __syncthreads();
+  const int tile = s_tile;
+  if (tile >= ntiles) return;
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  const int base = tile * kScanTile + t * kScanItems;
+  int v[kScanItems];
+  int sum = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    int idx = base + k;
+    v[k] = idx < C ? count[idx] : 0;
+    sum += v[k];
   }
-  int run = sums[t] - s;  // exclusive prefix of this chunk
-  for (int c = b; c < e; ++c) {
-    offset[c] = run;
-    run += count[c];
+  // inclusive scan of the thread sums within each wave, then across waves
+  int inc = sum;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int u = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += u;
   }
-  if (t == 1023) offset[C] = sums[1023];
+  if (lane == 63) s_wave[wid] = inc;
+  __syncthreads();
+  int wave_off = 0, agg = 0;
+#pragma unroll
+  for (int w = 0; w < kScanThreads / 64; ++w) {
+    if (w < wid) wave_off += s_wave[w];
+    agg += s_wave[w];
+  }
+  const int excl = wave_off + inc - sum;
+  if (t == 0) {
+    int prefix = 0;
+    if (tile == 0) {
+      __hip_atomic_store(&tiles[0], scan_pack(epoch, 2u, agg), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      __hip_atomic_store(&tiles[tile], scan_pack(epoch, 1u, agg), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+      int p = tile - 1;
+      long long spins = 0;
+      while (p >= 0) {
+        unsigned long long d =
+            __hip_atomic_load(&tiles[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned flag = (unsigned)(d >> 32) & 3u;
+        if ((unsigned)(d >> 34) != epoch || flag == 0u) {
+          if (++spins > kSpinLimit) {
+            st->valid = 0;
+            st->status |= CD_STATUS_GRID_OVERFLOW;
+            break;
+          }
+          continue;
+        }
+        prefix += (int)(unsigned)d;
+        if (flag == 2u) break;
+        --p;
+      }
+      __hip_atomic_store(&tiles[tile], scan_pack(epoch, 2u, prefix + agg), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    }
+    s_prefix = prefix;
+  }
+  __syncthreads();
+  int run = s_prefix + excl;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    int idx = base + k;
+    if (idx < C) offset[idx] = run;
+    run += v[k];
+  }
+  if (tile == ntiles - 1 && t == kScanThreads - 1) offset[C] = s_prefix + agg;
 }
 
 // K1c: scatter agents to their cell's segment.
@@ -1068,8 +1155,8 @@ int launch_binning(const Agents<T> &A, const Work &W, double cell, hipStream_t s
   hipLaunchKernelGGL(k_bin_count, dim3(grid_for(n, kBS)), dim3(kBS), 0, s,
                      (const double2 *)A.pos, n, cell, W.st, W.count, W.cell_of, W.rank, W.cap);
   mark(ev, 1, s);
-  hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, W.st, (const int *)W.count,
-                     W.offset);
+  hipLaunchKernelGGL(k_bin_scan, dim3(W.ntiles), dim3(kScanThreads), 0, s, W.st,
+                     (const int *)W.count, W.offset, W.tiles);
   mark(ev, 2, s);
   hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(n, kBS)), dim3(kBS), 0, s, n,
                      (const DevState *)W.st, (const int *)W.cell_of, (const int *)W.rank,
@@ -1132,7 +1219,7 @@ int cd_workspace_init(void *ws, int64_t ws_bytes, int64_t n, int64_t cap, int32_
   if (ws_bytes < L.total) return CD_ERR_WORKSPACE;
   Work W = carve(ws, L, cap);
   hipLaunchKernelGGL(k_ws_init, dim3(grid_for(cap + 1, kBS)), dim3(kBS), 0, (hipStream_t)stream,
-                     W.st, W.count, (int)cap);
+                     W.st, W.count, (int)cap, W.tiles, W.ntiles);
   return check_launch();
 }
 

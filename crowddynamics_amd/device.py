@@ -79,7 +79,7 @@ class DeviceAgents:
         for f, t in self.fields.items():
             if fields is not None and f not in fields:
                 continue
-            src = np.ascontiguousarray(array[f])
+            src = np.array(array[f], order='C', copy=True)
             t.copy_(torch.from_numpy(src).to(t.dtype))
 
     def download(self, array, fields=MUTABLE_FIELDS):

@@ -213,6 +213,14 @@ static void distance_three_circle_line(const double *x[3], const double r[3],
   *h_out = h_min;
 }
 
+/* Test access to distance_three_circles: x0, x1 are [3][2] part centres. */
+void orc_distance_three_circles(const double *x0, const double *r0, const double *x1,
+                                const double *r1, double *h, double *n, double *rm0,
+                                double *rm1) {
+  const double *a[3] = {x0, x0 + 2, x0 + 4}, *b[3] = {x1, x1 + 2, x1 + 4};
+  distance_three_circles(a, r0, b, r1, h, n, rm0, rm1);
+}
+
 /* ------------------------------------------------------------------ */
 /* motion (core/motion/)                                                */
 /* ------------------------------------------------------------------ */

@@ -165,6 +165,19 @@ def distance_circle_line(x, r, p0, p1):
     return h.value, n
 
 
+def distance_three_circles(x0, r0, x1, r1):
+    """(h, normal, r_moment0, r_moment1) for part centres x0, x1 [3, 2]."""
+    x0 = np.ascontiguousarray(x0, dtype=np.float64)
+    x1 = np.ascontiguousarray(x1, dtype=np.float64)
+    r0 = np.ascontiguousarray(r0, dtype=np.float64)
+    r1 = np.ascontiguousarray(r1, dtype=np.float64)
+    h = C.c_double()
+    n, m0, m1 = np.zeros(2), np.zeros(2), np.zeros(2)
+    lib().orc_distance_three_circles(_ptr(x0), _ptr(r0), _ptr(x1), _ptr(r1), C.byref(h),
+                                     _ptr(n), _ptr(m0), _ptr(m1))
+    return h.value, n, m0, m1
+
+
 def pair(arr, i, j):
     """Pair contributions (fi, fj[, mi, mj]) of interaction(i, j) exactly as
     the reference computes them for one pair, or None when h >= 3."""
