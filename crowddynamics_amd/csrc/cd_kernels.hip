@@ -62,10 +62,14 @@ constexpr int kScanItems = 16;
 constexpr int kScanTile = kScanThreads * kScanItems;
 
 struct Layout {
-  int64_t state, tiles, cell_of, rank, pidx, scell, count, offset, spos, svel, sr;
+  int64_t state, tiles, partials, cell_of, rank, pidx, scell, count, offset, spos, svel, sr;
   int64_t sls, srs, sphi, srt, srs_r, srts;
   int64_t total;
 };
+
+inline int64_t partials_bytes(int64_t n) {
+  return 8 * 8 * (std::max<int64_t>(512, (n + 255) / 256) + 1);
+}
 
 Layout make_layout(int64_t n, int64_t cap, int32_t model) {
   Layout L;
@@ -75,7 +79,9 @@ Layout make_layout(int64_t n, int64_t cap, int32_t model) {
     o = align_up(o + bytes);
     return r;
   };
+  // state and partials first: their offsets depend on n only
   L.state = take(4096);
+  L.partials = take(partials_bytes(n));
   L.tiles = take(8 * ((cap + kScanTile - 1) / kScanTile + 1));
   L.cell_of = take(4 * n);
   L.rank = take(4 * n);
@@ -153,6 +159,7 @@ Agents<T> typed(const cd_agents_t *a) {
 struct Work {
   DevState *st;
   unsigned long long *tiles;
+  unsigned long long *partials;
   int ntiles;
   int *cell_of, *rank, *pidx, *scell, *count, *offset;
   double2 *spos;
@@ -168,6 +175,7 @@ Work carve(void *ws, const Layout &L, int64_t cap) {
   Work W;
   W.st = (DevState *)(b + L.state);
   W.tiles = (unsigned long long *)(b + L.tiles);
+  W.partials = (unsigned long long *)(b + L.partials);
   W.ntiles = (int)((cap + kScanTile - 1) / kScanTile);
   W.cell_of = (int *)(b + L.cell_of);
   W.rank = (int *)(b + L.rank);
@@ -236,12 +244,19 @@ struct MaxOp {
   }
 };
 
-// Accumulates this block's local reduction values (in ord-bit form) into the
-// state accumulator; the last block to finish moves the accumulator to the
-// current slot and re-arms it.  Returns true in the last block's threads.
-__device__ bool block_reduce_commit(DevState *st, unsigned long long v[R_COUNT], int counter) {
+__device__ __forceinline__ void combine(unsigned long long *v, const unsigned long long *u) {
+  v[R_MINX] = min(v[R_MINX], u[R_MINX]);
+  v[R_MINY] = min(v[R_MINY], u[R_MINY]);
+  v[R_MAXX] = max(v[R_MAXX], u[R_MAXX]);
+  v[R_MAXY] = max(v[R_MAXY], u[R_MAXY]);
+  v[R_VMAX] = max(v[R_VMAX], u[R_VMAX]);
+  v[R_V0MAX] = max(v[R_V0MAX], u[R_V0MAX]);
+  v[R_NONFINITE] = max(v[R_NONFINITE], u[R_NONFINITE]);
+}
+
+// Block reduction of the per-thread values into thread 0's v.
+__device__ void block_reduce(unsigned long long v[R_COUNT]) {
   __shared__ unsigned long long sh[R_COUNT][16];
-  __shared__ bool last;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
   v[R_MINX] = wave_reduce(v[R_MINX], MinOp());
   v[R_MINY] = wave_reduce(v[R_MINY], MinOp());
@@ -250,43 +265,53 @@ __device__ bool block_reduce_commit(DevState *st, unsigned long long v[R_COUNT],
   v[R_VMAX] = wave_reduce(v[R_VMAX], MaxOp());
   v[R_V0MAX] = wave_reduce(v[R_V0MAX], MaxOp());
   v[R_NONFINITE] = wave_reduce(v[R_NONFINITE], MaxOp());
+  __syncthreads();
   if (lane == 0)
     for (int k = 0; k < R_COUNT; ++k) sh[k][wid] = v[k];
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int w = 1; w < nw; ++w) {
-      v[R_MINX] = min(v[R_MINX], sh[R_MINX][w]);
-      v[R_MINY] = min(v[R_MINY], sh[R_MINY][w]);
-      v[R_MAXX] = max(v[R_MAXX], sh[R_MAXX][w]);
-      v[R_MAXY] = max(v[R_MAXY], sh[R_MAXY][w]);
-      v[R_VMAX] = max(v[R_VMAX], sh[R_VMAX][w]);
-      v[R_V0MAX] = max(v[R_V0MAX], sh[R_V0MAX][w]);
-      v[R_NONFINITE] = max(v[R_NONFINITE], sh[R_NONFINITE][w]);
+      unsigned long long u[R_COUNT];
+      for (int k = 0; k < R_COUNT; ++k) u[k] = sh[k][w];
+      combine(v, u);
     }
-    atomicMin(&st->red[1][R_MINX], v[R_MINX]);
-    atomicMin(&st->red[1][R_MINY], v[R_MINY]);
-    atomicMax(&st->red[1][R_MAXX], v[R_MAXX]);
-    atomicMax(&st->red[1][R_MAXY], v[R_MAXY]);
-    atomicMax(&st->red[1][R_VMAX], v[R_VMAX]);
-    atomicMax(&st->red[1][R_V0MAX], v[R_V0MAX]);
-    if (v[R_NONFINITE]) atomicMax(&st->red[1][R_NONFINITE], v[R_NONFINITE]);
+  }
+}
+
+// Reduces this block's values, stores them as the block's partial, and lets
+// the last block to finish reduce all partials into the current slot
+// (st->red[0]).  One atomic per block (the completion counter); the
+// partials are written and read with agent-scope atomics so every XCD sees
+// them.  Returns true in the last block.
+__device__ bool block_reduce_commit(DevState *st, unsigned long long v[R_COUNT], int counter,
+                                    unsigned long long *partials) {
+  __shared__ bool last;
+  block_reduce(v);
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < R_COUNT; ++k)
+      __hip_atomic_store(&partials[blockIdx.x * 8 + k], v[k], __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
     __threadfence();
     unsigned int prev = atomicAdd(&st->done[counter], 1u);
     last = (prev == gridDim.x - 1);
   }
   __syncthreads();
   if (!last) return false;
+  __threadfence();
+  unsigned long long w[R_COUNT] = {kOrdPosInf, kOrdPosInf, kOrdNegInf, kOrdNegInf, 0ull,
+                                   kOrdNegInf, 0ull};
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
+    unsigned long long u[R_COUNT];
+    for (int k = 0; k < R_COUNT; ++k)
+      u[k] = __hip_atomic_load(&partials[b * 8 + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    combine(w, u);
+  }
+  block_reduce(w);
   if (threadIdx.x == 0) {
-    __threadfence();
-    for (int k = 0; k < R_COUNT; ++k) {
-      unsigned long long x = __hip_atomic_load(&st->red[1][k], __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT);
-      st->red[0][k] = x;
-    }
-    reset_acc(st);
+    for (int k = 0; k < R_COUNT; ++k) st->red[0][k] = w[k];
     st->done[counter] = 0u;
     st->oob = 0;
-    if (st->red[0][R_NONFINITE]) st->status |= CD_STATUS_NONFINITE;
+    if (w[R_NONFINITE]) st->status |= CD_STATUS_NONFINITE;
   }
   return true;
 }
@@ -295,7 +320,8 @@ __device__ bool block_reduce_commit(DevState *st, unsigned long long v[R_COUNT],
 // ignored as in core/integrator.py:81-85) and max target_velocity; also
 // clears the counts of the previous binning.
 template <typename T>
-__global__ void __launch_bounds__(256) k_reduce(Agents<T> A, DevState *st, int *count) {
+__global__ void __launch_bounds__(256) k_reduce(Agents<T> A, DevState *st, int *count,
+                                                unsigned long long *partials) {
   const int n = A.n;
   if (count) {
     const int clear = st->C_clear;
@@ -319,7 +345,7 @@ __global__ void __launch_bounds__(256) k_reduce(Agents<T> A, DevState *st, int *
     v[R_V0MAX] = max(v[R_V0MAX], ord_bits(A.v0[i]));
   }
   v[R_VMAX] = (unsigned long long)__double_as_longlong(vmax);
-  if (block_reduce_commit(st, v, 0) && threadIdx.x == 0 && count) st->C_clear = 0;
+  if (block_reduce_commit(st, v, 0, partials) && threadIdx.x == 0 && count) st->C_clear = 0;
 }
 
 // Grid of the binning from the current bounding box (contract in the file
@@ -442,36 +468,56 @@ __global__ void __launch_bounds__(kScanThreads) k_bin_scan(DevState *st, const i
     agg += s_wave[w];
   }
   const int excl = wave_off + inc - sum;
-  if (t == 0) {
+  if (wid == 0) {
+    // publish this tile's aggregate, then look back over up to 64 predecessor
+    // descriptors at once (one per lane) until an inclusive prefix is found
+    if (lane == 0)
+      __hip_atomic_store(&tiles[tile], scan_pack(epoch, tile == 0 ? 2u : 1u, agg),
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     int prefix = 0;
-    if (tile == 0) {
-      __hip_atomic_store(&tiles[0], scan_pack(epoch, 2u, agg), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      __hip_atomic_store(&tiles[tile], scan_pack(epoch, 1u, agg), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-      int p = tile - 1;
-      long long spins = 0;
-      while (p >= 0) {
+    int p = tile - 1;
+    long long spins = 0;
+    bool failed = false;
+    while (p >= 0) {
+      const int idx = p - lane;
+      unsigned flag = 2u;
+      int val = 0;
+      if (idx >= 0) {
         unsigned long long d =
-            __hip_atomic_load(&tiles[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        unsigned flag = (unsigned)(d >> 32) & 3u;
-        if ((unsigned)(d >> 34) != epoch || flag == 0u) {
-          if (++spins > kSpinLimit) {
-            st->valid = 0;
-            st->status |= CD_STATUS_GRID_OVERFLOW;
-            break;
-          }
-          continue;
-        }
-        prefix += (int)(unsigned)d;
-        if (flag == 2u) break;
-        --p;
+            __hip_atomic_load(&tiles[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        flag = ((unsigned)(d >> 34) == epoch) ? ((unsigned)(d >> 32) & 3u) : 0u;
+        val = (int)(unsigned)d;
       }
-      __hip_atomic_store(&tiles[tile], scan_pack(epoch, 2u, prefix + agg), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned long long incl = __ballot(flag == 2u);
+      const unsigned long long bad = __ballot(flag == 0u);
+      // lanes up to (and including) the nearest inclusive descriptor count
+      const int stop = incl ? (__ffsll((long long)incl) - 1) : 63;
+      const unsigned long long need =
+          (stop >= 63) ? ~0ull : ((2ull << stop) - 1ull);
+      if (bad & need) {
+        if (++spins > kSpinLimit) {
+          failed = true;
+          break;
+        }
+        continue;
+      }
+      int part = lane <= stop ? val : 0;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o, 64);
+      prefix += part;
+      if (incl) break;
+      p -= 64;
     }
-    s_prefix = prefix;
+    if (lane == 0) {
+      if (failed) {
+        st->valid = 0;
+        st->status |= CD_STATUS_GRID_OVERFLOW;
+      }
+      if (tile != 0)
+        __hip_atomic_store(&tiles[tile], scan_pack(epoch, 2u, prefix + agg), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      s_prefix = prefix;
+    }
   }
   __syncthreads();
   int run = s_prefix + excl;
@@ -672,7 +718,7 @@ __global__ void __launch_bounds__(256) k_reset(Agents<T> A, int model) {
 template <typename T>
 __device__ __forceinline__ void walls_agent(const Agents<T> &A, int model, int i, double2 p,
                                             V2<T> v, const double4 *seg, int W, T &fx, T &fy,
-                                            double &tq) {
+                                            double &tq, int w_begin = 0, int w_stride = 1) {
   const double r = A.radius[i];
   double2 ls, rs;
   double rt = 0.0, rsr = 0.0;
@@ -683,7 +729,7 @@ __device__ __forceinline__ void walls_agent(const Agents<T> &A, int model, int i
     rsr = A.r_s[i];
   }
   const T mu = (T)A.mu[i], kappa = (T)A.kappa[i], damping = (T)A.damping[i];
-  for (int w = 0; w < W; ++w) {
+  for (int w = w_begin; w < W; w += w_stride) {
     double4 s = seg[w];
     double h, nx, ny, rmx = 0.0, rmy = 0.0;
     if (!model) {
@@ -756,145 +802,128 @@ __global__ void __launch_bounds__(256) k_walls(Agents<T> A, int model, const dou
 // ---------------------------------------------------------------------------
 // Pair forces (owner-computes over the 3x3 cell stencil)
 // ---------------------------------------------------------------------------
+//
+// Four lanes serve one agent (the agent of sorted slot s = thread / 4).
+// Lanes 0..2 sum the agent's side of every pair in stencil row dx = -1, 0, +1
+// (the three cells of a row are contiguous in the cell-sorted order, so a
+// row is one slot range); lane 3 handles the agent's own terms
+// (fluctuation, navigation, orientation, adjusting) and the wall contacts
+// are split over the four lanes.  The partial sums meet on lane 3 through
+// wavefront shuffles in a fixed order, so results are deterministic.
 
-// Sum of agent i's side of every adjacent pair (circular).  The three cells
-// of one stencil row are contiguous in the cell-sorted order, so each row is
-// one slot range.
+struct Slots {
+  const int *offset, *pidx, *scell;
+  const double2 *spos;
+  const void *svel;
+  const double *sr;
+  const double2 *sls, *srs;
+  const double *sphi, *srt, *srs_r, *srts;
+};
+
+Slots slots_of(const Work &W) {
+  Slots S;
+  S.offset = W.offset;
+  S.pidx = W.pidx;
+  S.scell = W.scell;
+  S.spos = W.spos;
+  S.svel = W.svel;
+  S.sr = W.sr;
+  S.sls = W.sls;
+  S.srs = W.srs;
+  S.sphi = W.sphi;
+  S.srt = W.srt;
+  S.srs_r = W.srs_r;
+  S.srts = W.srts;
+  return S;
+}
+
 template <typename T>
-__device__ __forceinline__ void pairs_circular(const Agents<T> &A, const DevState *st,
-                                               const int *offset, const double2 *spos,
-                                               const V2<T> *svel, const double *sr, int s, int i,
-                                               int c, T &fx, T &fy) {
-  const int ny = st->ny, nx = st->nx;
-  const int lx = c / ny, ly = c - lx * ny;
-  const double2 xi = spos[s];
-  const V2<T> vi = svel[s];
-  const double ri = sr[s];
+__device__ __forceinline__ Owner<T> owner_of(const Agents<T> &A, int i) {
   Owner<T> o;
   o.neg_mk = (T)((-A.mass[i]) * A.k_soc[i]);
   o.tau_0 = (T)A.tau_0[i];
   o.mu = (T)A.mu[i];
   o.kappa = (T)A.kappa[i];
   o.damping = (T)A.damping[i];
+  return o;
+}
+
+// Slot range of stencil row ox (cells ylo..yhi of column ox).
+__device__ __forceinline__ void row_range(const int *offset, int ox, int ny, int ly, int &k0,
+                                          int &k1) {
   const int ylo = max(ly - 1, 0), yhi = min(ly + 1, ny - 1);
-#pragma unroll
-  for (int dx = -1; dx <= 1; ++dx) {
-    const int ox = lx + dx;
-    if (ox < 0 || ox >= nx) continue;
-    const int k0 = offset[ox * ny + ylo], k1 = offset[ox * ny + yhi + 1];
-    for (int k = k0; k < k1; ++k) {
-      if (k == s) continue;
-      const double2 xj = spos[k];
-      const V2<T> vj = svel[k];
-      T px, py;
-      if (pair_circular<T>((T)(xi.x - xj.x), (T)(xi.y - xj.y), vi.x - vj.x, vi.y - vj.y,
-                           (T)(ri + sr[k]), o, px, py)) {
-        fx += px;
-        fy += py;
-      }
+  k0 = offset[ox * ny + ylo];
+  k1 = offset[ox * ny + yhi + 1];
+}
+
+// Circular pairs of one stencil row, neighbour loads pipelined one ahead.
+template <typename T>
+__device__ __forceinline__ void row_circular(const Slots &S, int k0, int k1, int s, double2 xi,
+                                             V2<T> vi, double ri, const Owner<T> &o, T &fx,
+                                             T &fy) {
+  const V2<T> *svel = (const V2<T> *)S.svel;
+  if (k0 >= k1) return;
+  double2 xn = S.spos[k0];
+  V2<T> vn = svel[k0];
+  double rn = S.sr[k0];
+  for (int k = k0; k < k1; ++k) {
+    const double2 xj = xn;
+    const V2<T> vj = vn;
+    const double rj = rn;
+    if (k + 1 < k1) {
+      xn = S.spos[k + 1];
+      vn = svel[k + 1];
+      rn = S.sr[k + 1];
+    }
+    if (k == s) continue;
+    T px, py;
+    if (pair_circular<T>((T)(xi.x - xj.x), (T)(xi.y - xj.y), vi.x - vj.x, vi.y - vj.y,
+                         (T)(ri + rj), o, px, py)) {
+      fx += px;
+      fy += py;
     }
   }
 }
 
-// Three-circle: evaluates each pair in the reference's (min index, max index)
-// argument order and keeps the owner's side.
-__device__ __forceinline__ void pairs_three_circle(const Agents<double> &A, const DevState *st,
-                                                   const int *offset, const int *pidx,
-                                                   const double2 *spos, const V2<double> *svel,
-                                                   const double2 *sls, const double2 *srs,
-                                                   const double *sphi, const double *srt,
-                                                   const double *srs_r, const double *srts,
-                                                   int s, int i, int c, double &fx, double &fy,
-                                                   double &tq) {
-  const int ny = st->ny, nx = st->nx;
-  const int lx = c / ny, ly = c - lx * ny;
-  TCView me;
-  {
-    double2 p = spos[s], l = sls[s], r = srs[s];
-    me.px[0] = p.x;
-    me.py[0] = p.y;
-    me.px[1] = l.x;
-    me.py[1] = l.y;
-    me.px[2] = r.x;
-    me.py[2] = r.y;
-    V2<double> v = svel[s];
-    me.vx = v.x;
-    me.vy = v.y;
-    me.r_t = srt[s];
-    me.r_s = srs_r[s];
-    me.r_ts = srts[s];
-    me.phi = sphi[s];
-  }
-  Owner<double> o;
-  o.neg_mk = (-A.mass[i]) * A.k_soc[i];
-  o.tau_0 = A.tau_0[i];
-  o.mu = A.mu[i];
-  o.kappa = A.kappa[i];
-  o.damping = A.damping[i];
-  const int ylo = max(ly - 1, 0), yhi = min(ly + 1, ny - 1);
-  for (int dx = -1; dx <= 1; ++dx) {
-    const int ox = lx + dx;
-    if (ox < 0 || ox >= nx) continue;
-    const int k0 = offset[ox * ny + ylo], k1 = offset[ox * ny + yhi + 1];
-    for (int k = k0; k < k1; ++k) {
-      if (k == s) continue;
-      const int j = pidx[k];
-      TCView other;
-      double2 p = spos[k], l = sls[k], r = srs[k];
-      other.px[0] = p.x;
-      other.py[0] = p.y;
-      other.px[1] = l.x;
-      other.py[1] = l.y;
-      other.px[2] = r.x;
-      other.py[2] = r.y;
-      V2<double> v = svel[k];
-      other.vx = v.x;
-      other.vy = v.y;
-      other.r_t = srt[k];
-      other.r_s = srs_r[k];
-      other.r_ts = srts[k];
-      other.phi = sphi[k];
-      double px, py, m;
-      bool hit = (i < j) ? pair_three_circle(me, other, false, o, px, py, m)
-                         : pair_three_circle(other, me, true, o, px, py, m);
-      if (hit) {
-        fx += px;
-        fy += py;
-        tq += m;
-      }
+__device__ __forceinline__ TCView tc_view(const Slots &S, int k) {
+  TCView t;
+  double2 p = S.spos[k], l = S.sls[k], r = S.srs[k];
+  t.px[0] = p.x;
+  t.py[0] = p.y;
+  t.px[1] = l.x;
+  t.py[1] = l.y;
+  t.px[2] = r.x;
+  t.py[2] = r.y;
+  V2<double> v = ((const V2<double> *)S.svel)[k];
+  t.vx = v.x;
+  t.vy = v.y;
+  t.r_t = S.srt[k];
+  t.r_s = S.srs_r[k];
+  t.r_ts = S.srts[k];
+  t.phi = S.sphi[k];
+  return t;
+}
+
+// Three-circle pairs of one stencil row: each pair is evaluated in the
+// reference's (min index, max index) argument order, keeping the owner's side.
+__device__ __forceinline__ void row_three_circle(const Slots &S, int k0, int k1, int s, int i,
+                                                 const TCView &me, const Owner<double> &o,
+                                                 double &fx, double &fy, double &tq) {
+  for (int k = k0; k < k1; ++k) {
+    if (k == s) continue;
+    const int j = S.pidx[k];
+    const TCView other = tc_view(S, k);
+    double px, py, m;
+    bool hit = (i < j) ? pair_three_circle(me, other, false, o, px, py, m)
+                       : pair_three_circle(other, me, true, o, px, py, m);
+    if (hit) {
+      fx += px;
+      fy += py;
+      tq += m;
     }
   }
 }
 
-// AgentAgentInteractions node: force (torque) += pair sums.
-template <typename T, int model>
-__global__ void __launch_bounds__(256) k_pairs(Agents<T> A, const DevState *st, const int *offset,
-                        const int *pidx, const int *scell, const double2 *spos, const V2<T> *svel,
-                        const double *sr, const double2 *sls, const double2 *srs,
-                        const double *sphi, const double *srt, const double *srs_r,
-                        const double *srts) {
-  if (!st->valid) return;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= A.n) return;
-  const int i = pidx[s];
-  const int c = scell[s];
-  V2<T> f = A.force[i];
-  if constexpr (model == 0) {
-    pairs_circular<T>(A, st, offset, spos, svel, sr, s, i, c, f.x, f.y);
-  } else {
-    if constexpr (sizeof(T) == 8) {
-      double tq = A.torque[i];
-      pairs_three_circle(A, st, offset, pidx, spos, svel, sls, srs, sphi, srt, srs_r, srts, s,
-                         i, c, f.x, f.y, tq);
-      A.torque[i] = tq;
-    }
-  }
-  A.force[i] = f;
-}
-
-// Fused forces of one step for the agent in sorted slot s: fluctuation,
-// navigation, orientation, adjusting, pairs and walls in the post-order of
-// the standard logic tree.
 struct FusedArgs {
   int flags, nav_mode;
   uint64_t seed;
@@ -905,48 +934,108 @@ struct FusedArgs {
   int W;
 };
 
-template <typename T, int model>
-__global__ void __launch_bounds__(256) k_forces_fused(Agents<T> A, DevState *st, FusedArgs fa,
-                               const int *offset, const int *pidx, const int *scell,
-                               const double2 *spos, const V2<T> *svel, const double *sr,
-                               const double2 *sls, const double2 *srs, const double *sphi,
-                               const double *srt, const double *srs_r, const double *srts) {
+// FUSED = false: the AgentAgentInteractions node, force (torque) += pairs.
+// FUSED = true: all per-agent forces of one step in the post-order of the
+// standard logic tree (fluctuation, navigation, orientation, adjusting,
+// pairs, walls), written to force (torque).
+template <typename T, int MODEL, bool FUSED>
+__global__ void __launch_bounds__(256) k_forces(Agents<T> A, DevState *st, FusedArgs fa,
+                                                Slots S) {
   __shared__ double4 sseg[kSegTile];
-  const bool valid = (fa.flags & CD_USE_AGENT_AGENT) && st->valid;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool live = valid && s < A.n;
-  // Agents are visited in cell order when the binning is valid; otherwise
-  // in index order (pairs skipped, overflow flagged).
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = gt >> 2;
+  const int q = threadIdx.x & 3;
+  const int lane0 = (threadIdx.x & 63) & ~3;
+  const bool pairs_on = FUSED ? ((fa.flags & CD_USE_AGENT_AGENT) != 0) : true;
+  const bool sorted = pairs_on && st->valid;
+  if (!FUSED && !sorted) return;  // node mode: binning invalid, nothing to add
+  const bool have = s < A.n;
   int i = -1, c = 0;
-  if (s < A.n) {
-    if (valid) {
-      i = pidx[s];
-      c = scell[s];
+  if (have) {
+    if (sorted) {
+      i = S.pidx[s];
+      c = S.scell[s];
     } else {
       i = s;
     }
   }
-  V2<T> f;
-  double tq = 0.0;
-  double2 p;
-  V2<T> v;
-  if (i >= 0) {
-    f = A.force[i];
-    if (model) tq = A.torque[i];
-    p = A.pos[i];
-    v = A.vel[i];
+  T px = T(0), py = T(0);  // this lane's pair partial sum
+  T wx = T(0), wy = T(0);  // this lane's wall partial sum
+  double ptq = 0.0, wtq = 0.0;
+  double2 pos;
+  V2<T> vel;
+  if (have) {
+    pos = sorted ? S.spos[s] : A.pos[i];
+    vel = sorted ? ((const V2<T> *)S.svel)[s] : A.vel[i];
+  }
+  if (have && sorted && q < 3) {
+    const int ny = st->ny, nx = st->nx;
+    const int lx = c / ny, ly = c - lx * ny;
+    const int ox = lx + q - 1;
+    if (ox >= 0 && ox < nx) {
+      int k0, k1;
+      row_range(S.offset, ox, ny, ly, k0, k1);
+      if constexpr (MODEL == 0) {
+        row_circular<T>(S, k0, k1, s, pos, vel, S.sr[s], owner_of(A, i), px, py);
+      } else {
+        if constexpr (sizeof(T) == 8) {
+          double fx = 0.0, fy = 0.0;
+          row_three_circle(S, k0, k1, s, i, tc_view(S, s), owner_of(A, i), fx, fy, ptq);
+          px = fx;
+          py = fy;
+        }
+      }
+    }
+  }
+  if (FUSED && (fa.flags & CD_USE_OBSTACLE)) {
+    for (int w0 = 0; w0 < fa.W; w0 += kSegTile) {
+      const int cnt = min(kSegTile, fa.W - w0);
+      __syncthreads();
+      for (int k = threadIdx.x; k < cnt; k += blockDim.x) sseg[k] = fa.segs[w0 + k];
+      __syncthreads();
+      if (have) walls_agent(A, MODEL, i, pos, vel, sseg, cnt, wx, wy, wtq, q, 4);
+    }
+  }
+  // partial sums of lanes 0..3 to lane 3 (all lanes take part in the shuffles)
+  const T p0x = __shfl(px, lane0 + 0, 64), p0y = __shfl(py, lane0 + 0, 64);
+  const T p1x = __shfl(px, lane0 + 1, 64), p1y = __shfl(py, lane0 + 1, 64);
+  const T p2x = __shfl(px, lane0 + 2, 64), p2y = __shfl(py, lane0 + 2, 64);
+  const double t0 = __shfl(ptq, lane0 + 0, 64), t1 = __shfl(ptq, lane0 + 1, 64),
+               t2 = __shfl(ptq, lane0 + 2, 64);
+  T w0x = T(0), w0y = T(0), w1x = T(0), w1y = T(0), w2x = T(0), w2y = T(0);
+  double u0 = 0.0, u1 = 0.0, u2 = 0.0;
+  if (FUSED) {
+    w0x = __shfl(wx, lane0 + 0, 64);
+    w0y = __shfl(wy, lane0 + 0, 64);
+    w1x = __shfl(wx, lane0 + 1, 64);
+    w1y = __shfl(wy, lane0 + 1, 64);
+    w2x = __shfl(wx, lane0 + 2, 64);
+    w2y = __shfl(wy, lane0 + 2, 64);
+    u0 = __shfl(wtq, lane0 + 0, 64);
+    u1 = __shfl(wtq, lane0 + 1, 64);
+    u2 = __shfl(wtq, lane0 + 2, 64);
+  }
+  if (!have || q != 3) return;
+  V2<T> f = A.force[i];
+  double tq = MODEL ? A.torque[i] : 0.0;
+  if (FUSED) {
     if (fa.flags & CD_USE_FLUCT) {
-      double nfx, nfy, nm;
-      fluct_noise(fa.seed, st->step, i, A.mass[i], A.sig_f[i], model ? A.inertia[i] : 0.0,
-                  model ? A.sig_t[i] : 0.0, nfx, nfy, nm);
+      double nfx, nfy, nm = 0.0;
+      double ua, ub;
+      draw2(fa.seed, i, st->step, 0, 0, ua, ub);
+      const double phi = 2.0 * kPi * ua;
+      const double mm = A.mass[i] * (A.sig_f[i] * fabs(truncnorm3(fa.seed, i, st->step, 1)));
+      nfx = mm * cos(phi);
+      nfy = mm * sin(phi);
+      if (MODEL) nm = A.inertia[i] * (A.sig_t[i] * truncnorm3(fa.seed, i, st->step, 2));
       f.x += (T)nfx;
       f.y += (T)nfy;
-      if (model) tq += nm;
+      if (MODEL) tq += nm;
     }
     V2<T> e = A.e0[i];
     if (fa.flags & CD_USE_NAV) {
       double ex, ey;
-      if (nav_lookup(p.x, p.y, A.target[i], fa.maps, fa.nav_T, fa.nav_rows, fa.nav_cols,
+      if (nav_lookup(pos.x, pos.y, A.target[i], fa.maps, fa.nav_T, fa.nav_rows, fa.nav_cols,
                      fa.nav_minx, fa.nav_miny, fa.nav_step, fa.nav_mode, ex, ey)) {
         e.x = (T)ex;
         e.y = (T)ey;
@@ -954,36 +1043,24 @@ __global__ void __launch_bounds__(256) k_forces_fused(Agents<T> A, DevState *st,
       }
     }
     if (fa.flags & CD_USE_ADJUST) {
-      if (model) A.phi0[i] = atan2((double)e.y, (double)e.x);
+      if (MODEL) A.phi0[i] = atan2((double)e.y, (double)e.x);
       T ax, ay;
-      adjust_force(A, i, v, e, ax, ay);
+      adjust_force(A, i, vel, e, ax, ay);
       f.x += ax;
       f.y += ay;
-      if (model) tq += adjust_torque(A, i);
-    }
-    if ((fa.flags & CD_USE_AGENT_AGENT) && live) {
-      if constexpr (model == 0) {
-        pairs_circular<T>(A, st, offset, spos, svel, sr, s, i, c, f.x, f.y);
-      } else {
-        if constexpr (sizeof(T) == 8)
-          pairs_three_circle(A, st, offset, pidx, spos, svel, sls, srs, sphi, srt, srs_r, srts,
-                             s, i, c, f.x, f.y, tq);
-      }
+      if (MODEL) tq += adjust_torque(A, i);
     }
   }
-  if (fa.flags & CD_USE_OBSTACLE) {
-    for (int w0 = 0; w0 < fa.W; w0 += kSegTile) {
-      int cnt = min(kSegTile, fa.W - w0);
-      __syncthreads();
-      for (int k = threadIdx.x; k < cnt; k += blockDim.x) sseg[k] = fa.segs[w0 + k];
-      __syncthreads();
-      if (i >= 0) walls_agent(A, model, i, p, v, sseg, cnt, f.x, f.y, tq);
-    }
+  f.x += (p0x + p1x) + p2x;
+  f.y += (p0y + p1y) + p2y;
+  if (MODEL) tq += (t0 + t1) + t2;
+  if (FUSED) {
+    f.x += (w0x + w1x) + (w2x + wx);
+    f.y += (w0y + w1y) + (w2y + wy);
+    if (MODEL) tq += (u0 + u1) + (u2 + wtq);
   }
-  if (i >= 0) {
-    A.force[i] = f;
-    if (model) A.torque[i] = tq;
-  }
+  A.force[i] = f;
+  if (MODEL) A.torque[i] = tq;
 }
 
 // ---------------------------------------------------------------------------
@@ -1008,7 +1085,8 @@ __device__ __forceinline__ double adaptive_dt(const DevState *st, double dt_min,
 // clearing of this step's cell counts.
 template <typename T>
 __global__ void __launch_bounds__(256) k_integrate(Agents<T> A, int model, DevState *st, double dt_min, double dt_max,
-                            int fused, int do_reset, int *count) {
+                            int fused, int do_reset, int *count,
+                            unsigned long long *partials) {
   const double dt = adaptive_dt(st, dt_min, dt_max);
   const int n = A.n;
   unsigned long long v[R_COUNT] = {kOrdPosInf, kOrdPosInf, kOrdNegInf, kOrdNegInf, 0ull,
@@ -1081,7 +1159,7 @@ __global__ void __launch_bounds__(256) k_integrate(Agents<T> A, int model, DevSt
   }
   if (fused) {
     v[R_VMAX] = (unsigned long long)__double_as_longlong(vmax);
-    if (block_reduce_commit(st, v, 1) && threadIdx.x == 0) {
+    if (block_reduce_commit(st, v, 1, partials) && threadIdx.x == 0) {
       st->dt = dt;
       st->time_tot += dt;
       st->step += 1;
@@ -1137,7 +1215,8 @@ int launch_reduce(const Agents<T> &A, const Work &W, hipStream_t s) {
   int blocks = grid_for(A.n, kBS);
   if (blocks > g_reduce_blocks) blocks = g_reduce_blocks;
   if (blocks < 1) blocks = 1;
-  hipLaunchKernelGGL(k_reduce<T>, dim3(blocks), dim3(kBS), 0, s, A, W.st, W.count);
+  hipLaunchKernelGGL(k_reduce<T>, dim3(blocks), dim3(kBS), 0, s, A, W.st, W.count,
+                     W.partials);
   return check_launch();
 }
 
@@ -1173,6 +1252,27 @@ int launch_binning(const Agents<T> &A, const Work &W, double cell, hipStream_t s
 }
 
 template <typename T>
+int launch_forces(const cd_agents_t *a, const Work &W, DevState *st, const FusedArgs &fa,
+                  bool fused, hipStream_t s) {
+  Agents<T> A = typed<T>(a);
+  const Slots S = slots_of(W);
+  const int threads = 4 * (int)a->n;
+  const dim3 grid(grid_for(threads, g_pair_block)), block(g_pair_block);
+  if (a->model == CD_THREE_CIRCLE) {
+    if (fused)
+      hipLaunchKernelGGL((k_forces<T, 1, true>), grid, block, 0, s, A, st, fa, S);
+    else
+      hipLaunchKernelGGL((k_forces<T, 1, false>), grid, block, 0, s, A, st, fa, S);
+  } else {
+    if (fused)
+      hipLaunchKernelGGL((k_forces<T, 0, true>), grid, block, 0, s, A, st, fa, S);
+    else
+      hipLaunchKernelGGL((k_forces<T, 0, false>), grid, block, 0, s, A, st, fa, S);
+  }
+  return check_launch();
+}
+
+template <typename T>
 int do_agent_agent(const cd_agents_t *a, double cell, void *ws, int64_t cap, hipStream_t s) {
   Agents<T> A = typed<T>(a);
   Layout L = make_layout(a->n, cap, a->model);
@@ -1181,14 +1281,9 @@ int do_agent_agent(const cd_agents_t *a, double cell, void *ws, int64_t cap, hip
   if (rc) return rc;
   rc = launch_binning(A, W, cell, s);
   if (rc) return rc;
-  auto kern = a->model == CD_THREE_CIRCLE ? k_pairs<T, 1> : k_pairs<T, 0>;
-  hipLaunchKernelGGL(kern, dim3(grid_for(A.n, g_pair_block)), dim3(g_pair_block), 0, s,
-                     A, (const DevState *)W.st, (const int *)W.offset,
-                     (const int *)W.pidx, (const int *)W.scell, (const double2 *)W.spos,
-                     (const V2<T> *)W.svel, (const double *)W.sr, (const double2 *)W.sls,
-                     (const double2 *)W.srs, (const double *)W.sphi, (const double *)W.srt,
-                     (const double *)W.srs_r, (const double *)W.srts);
-  return check_launch();
+  FusedArgs fa;
+  memset(&fa, 0, sizeof(fa));
+  return launch_forces<T>(a, W, W.st, fa, false, s);
 }
 
 }  // namespace
@@ -1382,26 +1477,27 @@ int cd_velocity_verlet_integrator(const cd_agents_t *a, double dt_min, double dt
                                   int64_t ws_bytes, cd_stream_t stream) {
   int rc = check_agents(a);
   if (rc) return rc;
-  if (!ws || ws_bytes < 4096) return CD_ERR_WORKSPACE;
+  const int64_t need = align_up(4096) + align_up(partials_bytes(a->n));
+  if (!ws || ws_bytes < need) return CD_ERR_WORKSPACE;
   if (!(dt_min >= 0.0) || !(dt_max >= 0.0)) return CD_ERR_ARG;
   if (a->n == 0) return CD_OK;
   hipStream_t s = (hipStream_t)stream;
-  Work W;
-  W.st = (DevState *)ws;
-  W.count = nullptr;
+  DevState *st = (DevState *)ws;
+  unsigned long long *partials = (unsigned long long *)((char *)ws + align_up(4096));
+  const int rblocks = std::min(grid_for(a->n, kBS), g_reduce_blocks);
   if (a->real == CD_F64) {
     Agents<double> A = typed<double>(a);
-    // the reduction kernel's count clearing is a no-op without a binning
-    hipLaunchKernelGGL(k_reduce<double>, dim3(std::min(grid_for(A.n, kBS), g_reduce_blocks)),
-                       dim3(kBS), 0, s, A, W.st, (int *)nullptr);
+    // count clearing is skipped (null counts): the integrator owns no binning
+    hipLaunchKernelGGL(k_reduce<double>, dim3(rblocks), dim3(kBS), 0, s, A, st, (int *)nullptr,
+                       partials);
     hipLaunchKernelGGL(k_integrate<double>, dim3(grid_for(A.n, kBS)), dim3(kBS), 0, s, A,
-                       (int)a->model, W.st, dt_min, dt_max, 0, 0, (int *)nullptr);
+                       (int)a->model, st, dt_min, dt_max, 0, 0, (int *)nullptr, partials);
   } else {
     Agents<float> A = typed<float>(a);
-    hipLaunchKernelGGL(k_reduce<float>, dim3(std::min(grid_for(A.n, kBS), g_reduce_blocks)),
-                       dim3(kBS), 0, s, A, W.st, (int *)nullptr);
+    hipLaunchKernelGGL(k_reduce<float>, dim3(rblocks), dim3(kBS), 0, s, A, st, (int *)nullptr,
+                       partials);
     hipLaunchKernelGGL(k_integrate<float>, dim3(grid_for(A.n, kBS)), dim3(kBS), 0, s, A,
-                       (int)a->model, W.st, dt_min, dt_max, 0, 0, (int *)nullptr);
+                       (int)a->model, st, dt_min, dt_max, 0, 0, (int *)nullptr, partials);
   }
   return check_launch();
 }
@@ -1464,18 +1560,12 @@ int step_impl(const cd_agents_t *a, const cd_step_params_t *p, void *ws, hipStre
   fa.nav_step = p->nav_step;
   fa.segs = (const double4 *)p->segments;
   fa.W = (p->flags & CD_USE_OBSTACLE) ? (int)p->n_segments : 0;
-  auto kern = a->model == CD_THREE_CIRCLE ? k_forces_fused<T, 1> : k_forces_fused<T, 0>;
-  hipLaunchKernelGGL(kern, dim3(grid_for(A.n, g_pair_block)), dim3(g_pair_block),
-                     0, s, A, W.st, fa, (const int *)W.offset,
-                     (const int *)W.pidx, (const int *)W.scell, (const double2 *)W.spos,
-                     (const V2<T> *)W.svel, (const double *)W.sr, (const double2 *)W.sls,
-                     (const double2 *)W.srs, (const double *)W.sphi, (const double *)W.srt,
-                     (const double *)W.srs_r, (const double *)W.srts);
+  if ((rc = launch_forces<T>(a, W, W.st, fa, true, s))) return rc;
   mark(ev, 5, s);
   if ((rc = check_launch())) return rc;
   hipLaunchKernelGGL(k_integrate<T>, dim3(grid_for(A.n, kBS)), dim3(kBS), 0, s, A,
                      (int)a->model, W.st, p->dt_min, p->dt_max, 1,
-                     (p->flags & CD_USE_RESET) ? 1 : 0, W.count);
+                     (p->flags & CD_USE_RESET) ? 1 : 0, W.count, W.partials);
   mark(ev, 6, s);
   return check_launch();
 }

@@ -353,3 +353,66 @@ def rollout(n=4000, seed=0, steps=500, dt=0.005, density=1.0):
     arr['target'] = 0
     return Scenario('rollout', arr, segments_array([]), 3.6, dt, dt, steps, seed=1,
                     bounds=(0.0, 0.0, side, side))
+
+
+class Hallway:
+    """Factory of the reference's Hallway simulation
+    (crowddynamics/examples/simulations.py:74-163) on the drop-in classes:
+    two line walls, targets at both ends, two groups walking in opposite
+    directions (target 1 toward +x, target 0 toward -x), and the same logic
+    tree.  Navigation reads nearest-node direction maps of the distance to
+    each end (the reference derives them by FMM; for a straight hallway the
+    distance maps are the closed forms used here)."""
+
+    def __new__(cls, size=50, agent_type='circular', width=40.0, height=5.0, ratio=1.0 / 3,
+                 seed=0, fused='auto', precision='f64'):
+        from .logic import (Adjusting, AgentAgentInteractions, AgentObstacleInteractions,
+                            Fluctuation, InsideDomain, Integrator, Navigation, Orientation,
+                            Reset)
+        from .simulation import Agents, Field, MultiAgentSimulation
+        rng = np.random.default_rng(seed)
+        n = size // 2 * 2
+        arr = _attributes(rng, n, agent_type)
+        if agent_type == 'three_circle':
+            r = arr['radius']
+            arr['r_t'], arr['r_s'], arr['r_ts'] = 0.5882 * r, 0.3725 * r, 0.6275 * r
+        half = n // 2
+        segs = segments_array([((0.0, 0.0), (width, 0.0)), ((0.0, height), (width, height))])
+        r = arr['radius'].copy()
+        p0, k0 = place_rejection(rng, half, r[:half].copy(),
+                                 (0.0, 0.0, ratio * width, height), segs)
+        p1, k1 = place_rejection(rng, n - half, r[half:].copy(),
+                                 ((1 - ratio) * width, 0.0, width, height), segs)
+        if k0 < half or k1 < n - half:
+            raise ValueError('hallway spawn areas too small for %d agents' % n)
+        arr['position'][:half] = p0
+        arr['position'][half:] = p1
+        for sl, phi, target in ((slice(0, half), 0.0, 1), (slice(half, n), np.pi, 0)):
+            e = np.array([np.cos(phi), np.sin(phi)])
+            arr['target_direction'][sl] = e
+            arr['velocity'][sl] = e
+            arr['target'][sl] = target
+            if agent_type == 'three_circle':
+                arr['orientation'][sl] = phi
+                arr['target_orientation'][sl] = phi
+        if agent_type == 'three_circle':
+            shoulders_host(arr)
+        step = 0.1
+        cols = int(np.floor(width / step + 1e-9)) + 1
+        rows = int(np.floor(height / step + 1e-9)) + 1
+        maps = np.zeros((2, rows, cols, 2))
+        maps[0, ..., 0] = -1.0   # target 0: the x = 0 end
+        maps[1, ..., 0] = 1.0    # target 1: the x = width end
+        field = Field(obstacles=segs, targets=2, navigation=maps, nav_origin=(0.0, 0.0),
+                      nav_step=step, nav_mode=0,
+                      domain=[(0, 0), (width, 0), (width, height), (0, height)])
+        sim = MultiAgentSimulation(field=field, agents=Agents(agent_type, arr, precision),
+                                   name='Hallway', fused=fused,
+                                   cell_capacity=int((width * 2 / 3.6 + 4) * (height * 4 / 3.6 + 4)))
+        sim.logic = Reset(sim) << InsideDomain(sim) << (
+            Integrator(sim) << (
+                Fluctuation(sim),
+                Adjusting(sim) << (Navigation(sim), Orientation(sim)),
+                AgentAgentInteractions(sim),
+                AgentObstacleInteractions(sim)))
+        return sim

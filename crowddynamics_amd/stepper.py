@@ -33,8 +33,11 @@ class GridOverflow(CrowdDynamicsException):
 class Stepper:
     def __init__(self, array, segments=None, nav=None, cell_size=3.6, dt_min=0.01,
                  dt_max=0.01, seed=1, flags=ALL_NODES, precision='f64', cell_capacity=None,
-                 bounds=None, first_step=0, device=None):
-        self.agents = DeviceAgents(array, precision=precision, device=device)
+                 bounds=None, first_step=0, device=None, device_agents=None):
+        if device_agents is not None:
+            self.agents = device_agents
+        else:
+            self.agents = DeviceAgents(array, precision=precision, device=device)
         dev = self.agents.device
         self.n = self.agents.n
         if segments is not None and len(segments):

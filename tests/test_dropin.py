@@ -1,0 +1,77 @@
+"""Host-side semantics of the drop-in simulation layer."""
+import numpy as np
+import pytest
+
+from crowddynamics_amd import examples as E
+from crowddynamics_amd import logic as L
+from crowddynamics_amd.agents import empty_agents
+from crowddynamics_amd.exceptions import InvalidType
+from crowddynamics_amd.simulation import (Agents, LogicNodeBase, MultiAgentSimulation,
+                                          post_order)
+
+
+class N(LogicNodeBase):
+    def update(self):
+        pass
+
+
+def test_lshift_returns_left_operand_and_builds_children():
+    # crowddynamics/simulation/tests/test_base.py:15-22
+    a, b, c, d = N('a'), N('b'), N('c'), N('d')
+    tree = a << (b << c, d)
+    assert tree is a
+    assert a.children == [b, d] and b.children == [c]
+    assert [n.name for n in post_order(a)] == ['c', 'b', 'd', 'a']
+    assert a['c'] is c and c['d'] is d
+    with pytest.raises(KeyError):
+        a['zz']
+
+
+def test_inject_before_and_after():
+    a, b, c = N('a'), N('b'), N('c')
+    a << b
+    b.inject_before(c)          # a -> c -> b
+    assert a.children == [c] and c.children == [b]
+    x = N('x')
+    a.inject_after(x)           # a -> x -> c -> b
+    assert a.children == [x] and x.children == [c]
+
+
+def test_hallway_post_order_and_fused_plan():
+    sim = E.Hallway(size=20)
+    names = [n.name for n in post_order(sim.logic.root)]
+    assert names == ['InsideDomain', 'Fluctuation', 'Navigation', 'Orientation', 'Adjusting',
+                     'AgentAgentInteractions', 'AgentObstacleInteractions', 'Integrator',
+                     'Reset']
+    plan = sim._make_plan()
+    assert plan['fused_flags'] == 63
+
+
+def test_unfusable_order_falls_back_to_nodes():
+    arr = empty_agents(4)
+    sim = MultiAgentSimulation(agents=Agents('circular', arr))
+    # Reset before the forces (not the reference order) cannot be fused
+    sim.logic = L.Integrator(sim) << (L.Reset(sim), L.Adjusting(sim))
+    assert sim._make_plan()['fused_flags'] is None
+
+
+def test_agents_type_check():
+    with pytest.raises(InvalidType):
+        Agents('capsule')
+    with pytest.raises(InvalidType):
+        Agents(np.zeros(2, dtype=[('x', 'f8')]).dtype)
+
+
+def test_inside_domain_flag():
+    sim = E.Hallway(size=20)
+    node = sim.logic['InsideDomain']
+    arr = sim.agents._array.copy()
+    arr['position'][0] = (-1.0, 2.0)
+    node.evaluate(arr)
+    assert not arr['active'][0] and arr['active'][1:].all()
+
+
+def test_agent_agent_cell_size_default():
+    sim = MultiAgentSimulation(agents=Agents('circular', empty_agents(2)))
+    assert L.AgentAgentInteractions(sim).cell_size == 3.6     # logic.py:114-116
+    assert L.AgentAgentInteractions(sim, cell_size=0.6).cell_size == 0.6
