@@ -26,7 +26,7 @@ sys.path.insert(0, ROOT)
 METRIC = ('agent-steps/s @20k Circular (GPU fp64/fp32 vs host numba); adjoint step time')
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
-KERNELS = ('bin_count', 'bin_scan', 'bin_scatter', 'bin_sort_gather', 'forces', 'integrate')
+KERNELS = ('bin_insert', 'cell_gather', 'forces', 'integrate')
 
 
 def log(*a):
@@ -139,11 +139,8 @@ def main():
             state = st.check()
             out = st.download(scn.agents.copy())
             # per-kernel GPU time, HIP events on the launch stream
-            ms = (C.c_double * 6)()
             nt = 50
-            _lib.check(_lib.lib().cd_step_timed(st.agents.ref(), C.byref(st.params), st.ws.ptr,
-                                                st.ws.nbytes, nt, ms, C.c_void_p(stream.cuda_stream)),
-                       'step_timed')
+            ms = st.timed(nt)
             kernel_ms[prec] = {k: ms[i] / nt for i, k in enumerate(KERNELS)}
             results[prec] = dict(value=scn.n * args.steps / el, ms_per_step=1e3 * el / args.steps,
                                  time_tot=state.time_tot, cells=int(state.shape[0] * state.shape[1]),
@@ -162,7 +159,7 @@ def main():
     cand_step = 0.5 * (t0_tally['candidate_pairs'] + t1_tally['candidate_pairs'])
     rb = 8 if main_p == 'f64' else 4
     per_kernel = {}
-    bin_ms = sum(km[k] for k in KERNELS[:4])
+    bin_ms = km['bin_insert'] + km['cell_gather']
     per_kernel['binning'] = dict(bound='hbm', unit='GB/s', ms=bin_ms,
                                  achieved=binning_bytes(scn.n, cells) / (bin_ms * 1e-3) / 1e9,
                                  peak=HBM_PEAK_GBS)

@@ -23,6 +23,7 @@ CD_ERR_HIP = -5
 
 CD_STATUS_GRID_OVERFLOW = 1
 CD_STATUS_NONFINITE = 2
+CD_STATUS_SLOTS_FULL = 4
 
 CD_USE_FLUCT = 1
 CD_USE_NAV = 2
@@ -48,6 +49,12 @@ class CDAgents(C.Structure):
                 ('tau_rot', _vp), ('std_rand_torque', _vp), ('target', _vp)]
 
 
+class CDWorkspace(C.Structure):
+    _fields_ = [('ptr', _vp), ('bytes', C.c_int64), ('n', C.c_int64),
+                ('cell_capacity', C.c_int64), ('cell_slots', C.c_int64), ('model', C.c_int32),
+                ('_pad', C.c_int32)]
+
+
 class CDState(C.Structure):
     _fields_ = [('dt', C.c_double), ('time_tot', C.c_double),
                 ('origin', C.c_double * 2), ('shape', C.c_int64 * 2),
@@ -66,20 +73,19 @@ class CDStepParams(C.Structure):
                 ('flags', C.c_int32), ('nav_mode', C.c_int32), ('seed', C.c_uint64),
                 ('segments', _vp), ('n_segments', C.c_int64), ('nav_maps', _vp),
                 ('nav_targets', C.c_int64), ('nav_rows', C.c_int64), ('nav_cols', C.c_int64),
-                ('nav_minx', C.c_double), ('nav_miny', C.c_double), ('nav_step', C.c_double),
-                ('cell_capacity', C.c_int64)]
+                ('nav_minx', C.c_double), ('nav_miny', C.c_double), ('nav_step', C.c_double)]
 
 
 #: every symbol include/cdamd.h declares, with (restype, argtypes)
 _P = C.POINTER
+_W = _P(CDWorkspace)
 SIGNATURES = {
-    'cd_workspace_bytes': (C.c_int64, [C.c_int64, C.c_int64, C.c_int32, _P(CDLayout)]),
-    'cd_workspace_init': (C.c_int, [_vp, C.c_int64, C.c_int64, C.c_int64, C.c_int32, _vp]),
-    'cd_read_state': (C.c_int, [_vp, _P(CDState), _vp]),
-    'cd_clear_status': (C.c_int, [_vp, _vp]),
-    'cd_add_to_cells': (C.c_int, [_P(CDAgents), C.c_double, _vp, C.c_int64, C.c_int64, _vp]),
-    'cd_agent_agent_block_list': (C.c_int, [_P(CDAgents), C.c_double, _vp, C.c_int64,
-                                            C.c_int64, _vp]),
+    'cd_workspace_bytes': (C.c_int64, [_W, _P(CDLayout)]),
+    'cd_workspace_init': (C.c_int, [_W, _vp]),
+    'cd_read_state': (C.c_int, [_W, _P(CDState), _vp]),
+    'cd_clear_status': (C.c_int, [_W, _vp]),
+    'cd_add_to_cells': (C.c_int, [_P(CDAgents), C.c_double, _W, _vp]),
+    'cd_agent_agent_block_list': (C.c_int, [_P(CDAgents), C.c_double, _W, _vp]),
     'cd_agent_obstacle': (C.c_int, [_P(CDAgents), _vp, C.c_int64, _vp]),
     'cd_force_adjust_agents': (C.c_int, [_P(CDAgents), _vp]),
     'cd_torque_adjust_agents': (C.c_int, [_P(CDAgents), _vp]),
@@ -87,13 +93,11 @@ SIGNATURES = {
     'cd_navigation': (C.c_int, [_P(CDAgents), _vp, C.c_int64, C.c_int64, C.c_int64,
                                 C.c_double, C.c_double, C.c_double, C.c_int32, _vp]),
     'cd_force_fluctuation': (C.c_int, [_P(CDAgents), C.c_uint64, C.c_uint64, _vp]),
-    'cd_velocity_verlet_integrator': (C.c_int, [_P(CDAgents), C.c_double, C.c_double, _vp,
-                                                C.c_int64, _vp]),
+    'cd_velocity_verlet_integrator': (C.c_int, [_P(CDAgents), C.c_double, C.c_double, _W, _vp]),
     'cd_reset': (C.c_int, [_P(CDAgents), _vp]),
-    'cd_step_prepare': (C.c_int, [_P(CDAgents), _P(CDStepParams), _vp, C.c_int64,
-                                  C.c_uint64, _vp]),
-    'cd_step': (C.c_int, [_P(CDAgents), _P(CDStepParams), _vp, C.c_int64, _vp]),
-    'cd_step_timed': (C.c_int, [_P(CDAgents), _P(CDStepParams), _vp, C.c_int64, C.c_int64,
+    'cd_step_prepare': (C.c_int, [_P(CDAgents), _P(CDStepParams), _W, C.c_uint64, _vp]),
+    'cd_step': (C.c_int, [_P(CDAgents), _P(CDStepParams), _W, _vp]),
+    'cd_step_timed': (C.c_int, [_P(CDAgents), _P(CDStepParams), _W, C.c_int64,
                                 _P(C.c_double), _vp]),
     'cd_graph_begin': (C.c_int, [_vp]),
     'cd_graph_end': (C.c_int, [_vp, _P(_vp)]),

@@ -44,8 +44,7 @@ def add_to_cells(agents, cell_size, workspace):
     """cell_lists.add_to_cells (called at crowddynamics/core/interactions.py:192-193).
     Results stay in `workspace` (see Workspace.block_list)."""
     _check_agents(agents)
-    _lib.check(_lib.lib().cd_add_to_cells(agents.ref(), float(cell_size), workspace.ptr,
-                                          workspace.nbytes, workspace.cell_capacity,
+    _lib.check(_lib.lib().cd_add_to_cells(agents.ref(), float(cell_size), workspace.ref(),
                                           stream_handle()), 'add_to_cells')
 
 
@@ -53,8 +52,8 @@ def agent_agent_block_list(agents, cell_size, workspace):
     """agent_agent_block_list (crowddynamics/core/interactions.py:191-205)."""
     _check_agents(agents)
     _lib.check(_lib.lib().cd_agent_agent_block_list(
-        agents.ref(), float(cell_size), workspace.ptr, workspace.nbytes,
-        workspace.cell_capacity, stream_handle()), 'agent_agent_block_list')
+        agents.ref(), float(cell_size), workspace.ref(), stream_handle()),
+        'agent_agent_block_list')
 
 
 def agent_obstacle(agents, obstacles):
@@ -122,8 +121,8 @@ def velocity_verlet_integrator(agents, dt_min, dt_max, workspace):
     dt stays on the device (Workspace.state().dt)."""
     _check_agents(agents)
     _lib.check(_lib.lib().cd_velocity_verlet_integrator(
-        agents.ref(), float(dt_min), float(dt_max), workspace.ptr, workspace.nbytes,
-        stream_handle()), 'velocity_verlet_integrator')
+        agents.ref(), float(dt_min), float(dt_max), workspace.ref(), stream_handle()),
+        'velocity_verlet_integrator')
 
 
 def reset(agents):

@@ -2,20 +2,29 @@
 // the C ABI declared in include/cdamd.h.
 //
 // Data layout in HBM: one contiguous array per agent field (vectors
-// interleaved [n][2]), plus a caller-owned workspace that holds the device
-// state scalars, the block-list arrays and cell-sorted copies of the fields
-// the pair kernel gathers from neighbours.
+// interleaved [n][2]), plus a caller-owned workspace holding the device state
+// scalars, per-block reduction partials, per-cell agent slot lists with
+// gathered neighbour data, and the block-list arrays of cd_add_to_cells.
 //
 // Binning contract (the reference binds cell_lists, an un-vendored
 // dependency, at crowddynamics/core/interactions.py:30,192-195):
 //   origin o = per-axis minimum of the positions, l = trunc((p - o) / c),
 //   shape = max l + 1, flat cell = l_x * ny + l_y, agents ordered stably by
-//   flat cell, and every unordered pair of agents in the same or adjacent
-//   cells (|dl_x| <= 1 and |dl_y| <= 1) interacts once.
+//   flat cell (ties in agent index order), and every unordered pair of
+//   agents in the same or adjacent cells (|dl_x| <= 1 and |dl_y| <= 1)
+//   interacts once.
 //
-// Pair forces are owner-computes: each agent sums its own side of every
-// pair, so there are no atomics on forces and results are run-to-run
-// deterministic.
+// One fused step is four kernels:
+//   K_a  k_bin_insert     reduce the previous step's partials (bounding box,
+//                         max |v|, max v0); bin every agent into its cell's
+//                         slot list
+//   K_b  k_cell_gather    per cell: order the slot list by agent index and
+//                         gather the neighbour data the force kernel reads
+//   K_e  k_forces         four lanes per agent: stencil rows, walls, own terms
+//   K_f  k_integrate      Verlet update, Reset, per-block partials for the
+//                         next step, clear the cell counts
+// Pair forces are owner-computes: each agent sums its own side of every pair,
+// so forces need no atomics and results are run-to-run deterministic.
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -37,41 +46,36 @@ namespace {
 enum RedField { R_MINX = 0, R_MINY, R_MAXX, R_MAXY, R_VMAX, R_V0MAX, R_NONFINITE, R_COUNT };
 
 struct DevState {
-  unsigned long long red[2][8];  // [0] = current values, [1] = accumulator
-  unsigned int done[4];          // last-block counters
+  unsigned long long red[8];  // last reduced values (R_*), ord-bit form except R_VMAX
   double dt, time_tot;
   double origin[2];
   int nx, ny;
-  int valid;  // grid of the last binning fits the capacity
-  int C;      // cells of the last binning
-  int C_clear;  // cells whose counts are non-zero (to clear before next count)
-  int oob;      // an agent fell outside the grid (stale bounding box): binning invalid
-  unsigned int scan_epoch;   // tags the scan's tile descriptors of this binning
-  unsigned int scan_ticket;  // dynamic tile index of the look-back scan
-  int pad1;
+  int valid;       // grid of the last binning fits the capacity
+  int C;           // cells of the last binning
+  int C_clear;     // cells whose counts the next clearing pass must zero
+  int oob;         // an agent fell outside the grid (stale bounding box)
+  int nparts;      // partials written by the last reduction producer
+  int pad0;
+  unsigned int scan_epoch, scan_ticket;
   long long status;
   unsigned long long step;
-  double v_max, v0_max;
 };
 
-constexpr int64_t kAlign = 256;
-inline int64_t align_up(int64_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
-
+constexpr int kMaxParts = 512;      // blocks of any partial-producing kernel
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 16;
 constexpr int kScanTile = kScanThreads * kScanItems;
 
+constexpr int64_t kAlign = 256;
+inline int64_t align_up(int64_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
+
 struct Layout {
-  int64_t state, tiles, partials, cell_of, rank, pidx, scell, count, offset, spos, svel, sr;
-  int64_t sls, srs, sphi, srt, srs_r, srts;
+  int64_t state, partials, tiles, cell_of, rank, pidx, count, offset;
+  int64_t ell_ids, e_pos, e_vel, e_r, e_ls, e_rs, e_phi, e_rt, e_rs_r, e_rts;
   int64_t total;
 };
 
-inline int64_t partials_bytes(int64_t n) {
-  return 8 * 8 * (std::max<int64_t>(512, (n + 255) / 256) + 1);
-}
-
-Layout make_layout(int64_t n, int64_t cap, int32_t model) {
+Layout make_layout(int64_t n, int64_t cap, int64_t K, int32_t model) {
   Layout L;
   int64_t o = 0;
   auto take = [&](int64_t bytes) {
@@ -79,28 +83,28 @@ Layout make_layout(int64_t n, int64_t cap, int32_t model) {
     o = align_up(o + bytes);
     return r;
   };
-  // state and partials first: their offsets depend on n only
   L.state = take(4096);
-  L.partials = take(partials_bytes(n));
+  L.partials = take(8 * 8 * (kMaxParts + 1));
   L.tiles = take(8 * ((cap + kScanTile - 1) / kScanTile + 1));
   L.cell_of = take(4 * n);
   L.rank = take(4 * n);
   L.pidx = take(4 * n);
-  L.scell = take(4 * n);
   L.count = take(4 * cap);
   L.offset = take(4 * (cap + 1));
-  L.spos = take(16 * n);
-  L.svel = take(16 * n);
-  L.sr = take(8 * n);
+  const int64_t slots = cap * K;
+  L.ell_ids = take(4 * slots);
+  L.e_pos = take(16 * slots);
+  L.e_vel = take(16 * slots);
+  L.e_r = take(8 * slots);
   if (model == CD_THREE_CIRCLE) {
-    L.sls = take(16 * n);
-    L.srs = take(16 * n);
-    L.sphi = take(8 * n);
-    L.srt = take(8 * n);
-    L.srs_r = take(8 * n);
-    L.srts = take(8 * n);
+    L.e_ls = take(16 * slots);
+    L.e_rs = take(16 * slots);
+    L.e_phi = take(8 * slots);
+    L.e_rt = take(8 * slots);
+    L.e_rs_r = take(8 * slots);
+    L.e_rts = take(8 * slots);
   } else {
-    L.sls = L.srs = L.sphi = L.srt = L.srs_r = L.srts = 0;
+    L.e_ls = L.e_rs = L.e_phi = L.e_rt = L.e_rs_r = L.e_rts = -1;
   }
   L.total = o;
   return L;
@@ -156,277 +160,343 @@ Agents<T> typed(const cd_agents_t *a) {
   return A;
 }
 
+// Device views of the workspace arrays (passed by value to kernels).
 struct Work {
   DevState *st;
-  unsigned long long *tiles;
-  unsigned long long *partials;
-  int ntiles;
-  int *cell_of, *rank, *pidx, *scell, *count, *offset;
-  double2 *spos;
-  void *svel;
-  double *sr;
-  double2 *sls, *srs;
-  double *sphi, *srt, *srs_r, *srts;
-  int cap;
+  unsigned long long *partials, *tiles;
+  int *cell_of, *rank, *pidx, *count, *offset;
+  int *ell_ids;
+  double2 *e_pos;
+  void *e_vel;
+  double *e_r;
+  double2 *e_ls, *e_rs;
+  double *e_phi, *e_rt, *e_rs_r, *e_rts;
+  int cap, K, ntiles;
 };
 
-Work carve(void *ws, const Layout &L, int64_t cap) {
-  char *b = (char *)ws;
+Work carve(const cd_workspace_t *w) {
+  const Layout L = make_layout(w->n, w->cell_capacity, w->cell_slots, w->model);
+  char *b = (char *)w->ptr;
   Work W;
   W.st = (DevState *)(b + L.state);
-  W.tiles = (unsigned long long *)(b + L.tiles);
   W.partials = (unsigned long long *)(b + L.partials);
-  W.ntiles = (int)((cap + kScanTile - 1) / kScanTile);
+  W.tiles = (unsigned long long *)(b + L.tiles);
   W.cell_of = (int *)(b + L.cell_of);
   W.rank = (int *)(b + L.rank);
   W.pidx = (int *)(b + L.pidx);
-  W.scell = (int *)(b + L.scell);
   W.count = (int *)(b + L.count);
   W.offset = (int *)(b + L.offset);
-  W.spos = (double2 *)(b + L.spos);
-  W.svel = (void *)(b + L.svel);
-  W.sr = (double *)(b + L.sr);
-  W.sls = L.sls ? (double2 *)(b + L.sls) : nullptr;
-  W.srs = L.srs ? (double2 *)(b + L.srs) : nullptr;
-  W.sphi = L.sphi ? (double *)(b + L.sphi) : nullptr;
-  W.srt = L.srt ? (double *)(b + L.srt) : nullptr;
-  W.srs_r = L.srs_r ? (double *)(b + L.srs_r) : nullptr;
-  W.srts = L.srts ? (double *)(b + L.srts) : nullptr;
-  W.cap = (int)cap;
+  W.ell_ids = (int *)(b + L.ell_ids);
+  W.e_pos = (double2 *)(b + L.e_pos);
+  W.e_vel = (void *)(b + L.e_vel);
+  W.e_r = (double *)(b + L.e_r);
+  W.e_ls = L.e_ls >= 0 ? (double2 *)(b + L.e_ls) : nullptr;
+  W.e_rs = L.e_rs >= 0 ? (double2 *)(b + L.e_rs) : nullptr;
+  W.e_phi = L.e_phi >= 0 ? (double *)(b + L.e_phi) : nullptr;
+  W.e_rt = L.e_rt >= 0 ? (double *)(b + L.e_rt) : nullptr;
+  W.e_rs_r = L.e_rs_r >= 0 ? (double *)(b + L.e_rs_r) : nullptr;
+  W.e_rts = L.e_rts >= 0 ? (double *)(b + L.e_rts) : nullptr;
+  W.cap = (int)w->cell_capacity;
+  W.K = (int)w->cell_slots;
+  W.ntiles = (int)((w->cell_capacity + kScanTile - 1) / kScanTile);
   return W;
 }
 
 constexpr unsigned long long kOrdPosInf = 0xFFF0000000000000ull;  // ord_bits(+inf)
 constexpr unsigned long long kOrdNegInf = 0x000FFFFFFFFFFFFFull;  // ord_bits(-inf)
 
-__device__ __forceinline__ void reset_acc(DevState *st) {
-  st->red[1][R_MINX] = kOrdPosInf;
-  st->red[1][R_MINY] = kOrdPosInf;
-  st->red[1][R_MAXX] = kOrdNegInf;
-  st->red[1][R_MAXY] = kOrdNegInf;
-  st->red[1][R_VMAX] = 0ull;
-  st->red[1][R_V0MAX] = kOrdNegInf;
-  st->red[1][R_NONFINITE] = 0ull;
-  st->red[1][7] = 0ull;
+__global__ void __launch_bounds__(256) k_ws_init(Work W) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int stride = gridDim.x * blockDim.x;
+  if (t == 0) memset(W.st, 0, sizeof(DevState));
+  for (int c = t; c < W.cap; c += stride) W.count[c] = 0;
+  for (int k = t; k < W.ntiles + 1; k += stride) W.tiles[k] = 0ull;
+  for (int k = t; k < 8 * (kMaxParts + 1); k += stride) W.partials[k] = 0ull;
 }
 
-__global__ void __launch_bounds__(256) k_ws_init(DevState *st, int *count, int cap,
-                                                 unsigned long long *tiles, int ntiles) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  for (int k = t; k < ntiles; k += gridDim.x * blockDim.x) tiles[k] = 0ull;
-  if (t == 0) {
-    memset(st, 0, sizeof(DevState));
-    for (int k = 0; k < 8; ++k) st->red[0][k] = 0ull;
-    reset_acc(st);
+// ---------------------------------------------------------------------------
+// Reductions (bounding box, max |v|, max v0, non-finite flag)
+// ---------------------------------------------------------------------------
+
+struct Red {
+  unsigned long long v[R_COUNT];
+  __device__ void init() {
+    v[R_MINX] = kOrdPosInf;
+    v[R_MINY] = kOrdPosInf;
+    v[R_MAXX] = kOrdNegInf;
+    v[R_MAXY] = kOrdNegInf;
+    v[R_VMAX] = 0ull;  // bits of a non-negative double
+    v[R_V0MAX] = kOrdNegInf;
+    v[R_NONFINITE] = 0ull;
   }
-  for (int c = t; c < cap; c += gridDim.x * blockDim.x) count[c] = 0;
+  __device__ void add_agent(double2 p, double speed, double v0) {
+    if (!(isfinite(p.x) && isfinite(p.y))) v[R_NONFINITE] = 1ull;
+    const unsigned long long bx = ord_bits(p.x), by = ord_bits(p.y);
+    v[R_MINX] = min(v[R_MINX], bx);
+    v[R_MINY] = min(v[R_MINY], by);
+    v[R_MAXX] = max(v[R_MAXX], bx);
+    v[R_MAXY] = max(v[R_MAXY], by);
+    // strict >, NaN never selected (core/integrator.py:81-85)
+    if (speed > __longlong_as_double((long long)v[R_VMAX]))
+      v[R_VMAX] = (unsigned long long)__double_as_longlong(speed);
+    v[R_V0MAX] = max(v[R_V0MAX], ord_bits(v0));
+  }
+  __device__ void combine(const unsigned long long *u) {
+    v[R_MINX] = min(v[R_MINX], u[R_MINX]);
+    v[R_MINY] = min(v[R_MINY], u[R_MINY]);
+    v[R_MAXX] = max(v[R_MAXX], u[R_MAXX]);
+    v[R_MAXY] = max(v[R_MAXY], u[R_MAXY]);
+    v[R_VMAX] = max(v[R_VMAX], u[R_VMAX]);
+    v[R_V0MAX] = max(v[R_V0MAX], u[R_V0MAX]);
+    v[R_NONFINITE] = max(v[R_NONFINITE], u[R_NONFINITE]);
+  }
+};
+
+__device__ __forceinline__ unsigned long long shfl_xor64(unsigned long long v, int o) {
+  return (unsigned long long)__shfl_xor((long long)v, o, 64);
 }
 
-// Block-wide reduction helpers (wave64 shuffles, then LDS across waves).
-template <typename F>
-__device__ __forceinline__ unsigned long long wave_reduce(unsigned long long v, F op) {
+// Block-wide reduction; the result is returned to every thread.
+__device__ Red block_reduce(Red r) {
+  __shared__ unsigned long long sh[R_COUNT][16];
+  __shared__ unsigned long long res[R_COUNT];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
-    unsigned long long w = __shfl_xor(v, o, 64);
-    v = op(v, w);
+    unsigned long long u[R_COUNT];
+#pragma unroll
+    for (int k = 0; k < R_COUNT; ++k) u[k] = shfl_xor64(r.v[k], o);
+    r.combine(u);
   }
-  return v;
-}
-
-struct MinOp {
-  __device__ unsigned long long operator()(unsigned long long a, unsigned long long b) const {
-    return a < b ? a : b;
-  }
-};
-struct MaxOp {
-  __device__ unsigned long long operator()(unsigned long long a, unsigned long long b) const {
-    return a > b ? a : b;
-  }
-};
-
-__device__ __forceinline__ void combine(unsigned long long *v, const unsigned long long *u) {
-  v[R_MINX] = min(v[R_MINX], u[R_MINX]);
-  v[R_MINY] = min(v[R_MINY], u[R_MINY]);
-  v[R_MAXX] = max(v[R_MAXX], u[R_MAXX]);
-  v[R_MAXY] = max(v[R_MAXY], u[R_MAXY]);
-  v[R_VMAX] = max(v[R_VMAX], u[R_VMAX]);
-  v[R_V0MAX] = max(v[R_V0MAX], u[R_V0MAX]);
-  v[R_NONFINITE] = max(v[R_NONFINITE], u[R_NONFINITE]);
-}
-
-// Block reduction of the per-thread values into thread 0's v.
-__device__ void block_reduce(unsigned long long v[R_COUNT]) {
-  __shared__ unsigned long long sh[R_COUNT][16];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-  v[R_MINX] = wave_reduce(v[R_MINX], MinOp());
-  v[R_MINY] = wave_reduce(v[R_MINY], MinOp());
-  v[R_MAXX] = wave_reduce(v[R_MAXX], MaxOp());
-  v[R_MAXY] = wave_reduce(v[R_MAXY], MaxOp());
-  v[R_VMAX] = wave_reduce(v[R_VMAX], MaxOp());
-  v[R_V0MAX] = wave_reduce(v[R_V0MAX], MaxOp());
-  v[R_NONFINITE] = wave_reduce(v[R_NONFINITE], MaxOp());
   __syncthreads();
   if (lane == 0)
-    for (int k = 0; k < R_COUNT; ++k) sh[k][wid] = v[k];
+    for (int k = 0; k < R_COUNT; ++k) sh[k][wid] = r.v[k];
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int w = 1; w < nw; ++w) {
       unsigned long long u[R_COUNT];
       for (int k = 0; k < R_COUNT; ++k) u[k] = sh[k][w];
-      combine(v, u);
+      r.combine(u);
     }
-  }
-}
-
-// Reduces this block's values, stores them as the block's partial, and lets
-// the last block to finish reduce all partials into the current slot
-// (st->red[0]).  One atomic per block (the completion counter); the
-// partials are written and read with agent-scope atomics so every XCD sees
-// them.  Returns true in the last block.
-__device__ bool block_reduce_commit(DevState *st, unsigned long long v[R_COUNT], int counter,
-                                    unsigned long long *partials) {
-  __shared__ bool last;
-  block_reduce(v);
-  if (threadIdx.x == 0) {
-    for (int k = 0; k < R_COUNT; ++k)
-      __hip_atomic_store(&partials[blockIdx.x * 8 + k], v[k], __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    unsigned int prev = atomicAdd(&st->done[counter], 1u);
-    last = (prev == gridDim.x - 1);
+    for (int k = 0; k < R_COUNT; ++k) res[k] = r.v[k];
   }
   __syncthreads();
-  if (!last) return false;
-  __threadfence();
-  unsigned long long w[R_COUNT] = {kOrdPosInf, kOrdPosInf, kOrdNegInf, kOrdNegInf, 0ull,
-                                   kOrdNegInf, 0ull};
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
-    unsigned long long u[R_COUNT];
-    for (int k = 0; k < R_COUNT; ++k)
-      u[k] = __hip_atomic_load(&partials[b * 8 + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    combine(w, u);
-  }
-  block_reduce(w);
+  for (int k = 0; k < R_COUNT; ++k) r.v[k] = res[k];
+  return r;
+}
+
+// Producer side: each block stores its reduction as one partial.  The next
+// kernel on the stream combines the partials (kernel boundaries make them
+// visible), so producers need neither fences nor atomics.
+__device__ __forceinline__ void store_partial(const Work &W, const Red &r) {
   if (threadIdx.x == 0) {
-    for (int k = 0; k < R_COUNT; ++k) st->red[0][k] = w[k];
-    st->done[counter] = 0u;
-    st->oob = 0;
-    if (w[R_NONFINITE]) st->status |= CD_STATUS_NONFINITE;
+    for (int k = 0; k < R_COUNT; ++k) W.partials[blockIdx.x * 8 + k] = r.v[k];
+    if (blockIdx.x == 0) W.st->nparts = gridDim.x;
   }
-  return true;
 }
 
-// Reduction of the binning bounding box, max |v| (hypot, strict >, NaN
-// ignored as in core/integrator.py:81-85) and max target_velocity; also
-// clears the counts of the previous binning.
+// Consumer side: every block combines all partials (at most kMaxParts).
+__device__ Red reduce_partials(const Work &W) {
+  Red r;
+  r.init();
+  const int np = W.st->nparts;
+  for (int b = threadIdx.x; b < np; b += blockDim.x) r.combine(&W.partials[b * 8]);
+  return block_reduce(r);
+}
+
+__device__ __forceinline__ double red_vmax(const Red &r) {
+  return __longlong_as_double((long long)r.v[R_VMAX]);
+}
+
+// Partials of the current state (node mode and cd_step_prepare); also zeroes
+// the counts of the previous binning.
 template <typename T>
-__global__ void __launch_bounds__(256) k_reduce(Agents<T> A, DevState *st, int *count,
-                                                unsigned long long *partials) {
-  const int n = A.n;
-  if (count) {
-    const int clear = st->C_clear;
+__global__ void __launch_bounds__(256) k_reduce(Agents<T> A, Work W, int clear_counts) {
+  if (clear_counts) {
+    const int clear = W.st->C_clear;
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < clear; c += gridDim.x * blockDim.x)
-      count[c] = 0;
+      W.count[c] = 0;
   }
-  unsigned long long v[R_COUNT] = {kOrdPosInf, kOrdPosInf, kOrdNegInf, kOrdNegInf, 0ull,
-                                   kOrdNegInf, 0ull};
-  double vmax = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    double2 p = A.pos[i];
-    if (!(isfinite(p.x) && isfinite(p.y))) v[R_NONFINITE] = 1ull;
-    unsigned long long bx = ord_bits(p.x), by = ord_bits(p.y);
-    v[R_MINX] = min(v[R_MINX], bx);
-    v[R_MINY] = min(v[R_MINY], by);
-    v[R_MAXX] = max(v[R_MAXX], bx);
-    v[R_MAXY] = max(v[R_MAXY], by);
-    V2<T> w = A.vel[i];
-    double l = dhypot((double)w.x, (double)w.y);
-    if (l > vmax) vmax = l;
-    v[R_V0MAX] = max(v[R_V0MAX], ord_bits(A.v0[i]));
+  Red r;
+  r.init();
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gridDim.x * blockDim.x) {
+    const V2<T> w = A.vel[i];
+    r.add_agent(A.pos[i], dhypot((double)w.x, (double)w.y), A.v0[i]);
   }
-  v[R_VMAX] = (unsigned long long)__double_as_longlong(vmax);
-  if (block_reduce_commit(st, v, 0, partials) && threadIdx.x == 0 && count) st->C_clear = 0;
+  r = block_reduce(r);
+  store_partial(W, r);
+  if (blockIdx.x == 0 && threadIdx.x == 0) W.st->oob = 0;
 }
 
-// Grid of the binning from the current bounding box (contract in the file
-// header); all threads compute the same values.
+// Grid of the binning from a reduced bounding box (contract in the header).
 struct Grid {
   double ox, oy;
   int nx, ny;
   bool ok;
 };
 
-__device__ __forceinline__ Grid grid_from_state(const DevState *st, double cell, int cap) {
+__device__ __forceinline__ Grid grid_of(const Red &r, double cell, int cap) {
   Grid g;
-  double mnx = from_ord_bits(st->red[0][R_MINX]), mny = from_ord_bits(st->red[0][R_MINY]);
-  double mxx = from_ord_bits(st->red[0][R_MAXX]), mxy = from_ord_bits(st->red[0][R_MAXY]);
+  const double mnx = from_ord_bits(r.v[R_MINX]), mny = from_ord_bits(r.v[R_MINY]);
+  const double mxx = from_ord_bits(r.v[R_MAXX]), mxy = from_ord_bits(r.v[R_MAXY]);
   g.ox = mnx;
   g.oy = mny;
-  double qx = (mxx - mnx) / cell, qy = (mxy - mny) / cell;
-  g.ok = st->red[0][R_NONFINITE] == 0ull && qx >= 0.0 && qy >= 0.0 && qx < (double)cap &&
+  const double qx = (mxx - mnx) / cell, qy = (mxy - mny) / cell;
+  g.ok = r.v[R_NONFINITE] == 0ull && qx >= 0.0 && qy >= 0.0 && qx < (double)cap &&
          qy < (double)cap;
+  g.nx = g.ny = 0;
   if (g.ok) {
-    long long nx = (long long)qx + 1, ny = (long long)qy + 1;
+    const long long nx = (long long)qx + 1, ny = (long long)qy + 1;
     g.ok = nx * ny <= (long long)cap;
     g.nx = (int)nx;
     g.ny = (int)ny;
-  } else {
-    g.nx = g.ny = 0;
   }
   return g;
 }
 
-// K1a: cell of each agent and its (arbitrary) rank within the cell.
-__global__ void __launch_bounds__(256) k_bin_count(const double2 *pos, int n, double cell, DevState *st, int *count,
-                            int *cell_of, int *rank, int cap) {
-  Grid g = grid_from_state(st, cell, cap);
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) {
-    st->origin[0] = g.ox;
-    st->origin[1] = g.oy;
-    st->nx = g.nx;
-    st->ny = g.ny;
-    st->valid = g.ok ? 1 : 0;
-    st->C = g.ok ? g.nx * g.ny : 0;
-    st->C_clear = g.ok ? g.nx * g.ny : 0;
-    st->scan_epoch = (st->scan_epoch + 1u) & 0x3fffffffu;
-    if (st->scan_epoch == 0u) st->scan_epoch = 1u;
-    st->scan_ticket = 0u;
-    if (!g.ok) st->status |= CD_STATUS_GRID_OVERFLOW;
-  }
-  if (!g.ok || i >= n) return;
-  double2 p = pos[i];
-  double qx = (p.x - g.ox) / cell, qy = (p.y - g.oy) / cell;
-  // positions changed after the bounding box was reduced: refuse to bin
-  if (!(qx >= 0.0 && qy >= 0.0 && qx < (double)g.nx && qy < (double)g.ny)) {
-    atomicOr(&st->oob, 1);
-    return;
-  }
-  int lx = (int)qx;
-  int ly = (int)qy;
-  int c = lx * g.ny + ly;
-  cell_of[i] = c;
-  rank[i] = atomicAdd(&count[c], 1);
+// Records the reduction and the grid in the state (block 0, thread 0).
+__device__ __forceinline__ void publish_grid(const Work &W, const Red &r, const Grid &g,
+                                             bool binning) {
+  DevState *st = W.st;
+  for (int k = 0; k < R_COUNT; ++k) st->red[k] = r.v[k];
+  if (r.v[R_NONFINITE]) st->status |= CD_STATUS_NONFINITE;
+  if (!binning) return;
+  st->origin[0] = g.ox;
+  st->origin[1] = g.oy;
+  st->nx = g.nx;
+  st->ny = g.ny;
+  st->valid = g.ok ? 1 : 0;
+  st->C = g.ok ? g.nx * g.ny : 0;
+  st->C_clear = g.ok ? g.nx * g.ny : 0;
+  if (!g.ok) st->status |= CD_STATUS_GRID_OVERFLOW;
 }
 
-// K1b: exclusive scan of the counts, single pass with decoupled look-back.
-// Each workgroup takes the next tile of kScanTile cells (dynamic ticket, so
-// every tile's predecessors are already resident), publishes its aggregate,
-// then walks back over predecessors' descriptors until an inclusive prefix.
-// A descriptor packs (epoch:30 | flag:2 | value:32); the epoch of this
-// binning makes stale descriptors of earlier steps invisible.  The wait is
-// bounded: past kSpinLimit polls the binning is marked invalid instead.
+// Cell of a position, or -1 outside the grid.
+__device__ __forceinline__ int cell_of_pos(double2 p, const Grid &g, double cell) {
+  const double qx = (p.x - g.ox) / cell, qy = (p.y - g.oy) / cell;
+  if (!(qx >= 0.0 && qy >= 0.0 && qx < (double)g.nx && qy < (double)g.ny)) return -1;
+  return (int)qx * g.ny + (int)qy;
+}
+
+// ---------------------------------------------------------------------------
+// K_a / K_b: per-cell slot lists
+// ---------------------------------------------------------------------------
+
+// K_a: reduce the partials; when `binning`, bin agent i into its cell's slot
+// list (slots beyond cell_slots are not stored; the cell count tells).
+__global__ void __launch_bounds__(256) k_bin_insert(const double2 *pos, int n, double cell,
+                                                    Work W, int binning) {
+  const Red r = reduce_partials(W);
+  const Grid g = grid_of(r, cell, W.cap);
+  if (blockIdx.x == 0 && threadIdx.x == 0) publish_grid(W, r, g, binning != 0);
+  if (!binning || !g.ok) return;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int c = cell_of_pos(pos[i], g, cell);
+    if (c < 0) {  // positions changed after the bounding box was reduced
+      atomicOr(&W.st->oob, 1);
+      W.cell_of[i] = -1;
+      continue;
+    }
+    W.cell_of[i] = c;
+    const int k = atomicAdd(&W.count[c], 1);
+    if (k < W.K) W.ell_ids[c * W.K + k] = i;
+  }
+}
+
+// K_b: one thread per cell.  Orders the cell's slot list by agent index and
+// gathers the neighbour data; cells with more agents than slots are flagged
+// (the force kernel visits their members through the exact slow path).
+template <typename T>
+__global__ void __launch_bounds__(256) k_cell_gather(Agents<T> A, Work W) {
+  DevState *st = W.st;
+  if (!st->valid) return;
+  if (st->oob) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->valid = 0;
+      st->status |= CD_STATUS_GRID_OVERFLOW;
+    }
+    return;
+  }
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= st->C) return;
+  const int cnt = W.count[c];
+  if (cnt == 0) return;
+  if (cnt > W.K) {
+    atomicOr((unsigned long long *)&st->status, (unsigned long long)CD_STATUS_SLOTS_FULL);
+    return;
+  }
+  int *ids = W.ell_ids + (size_t)c * W.K;
+  for (int s = 1; s < cnt; ++s) {  // insertion sort (cells hold a few agents)
+    const int key = ids[s];
+    int t = s - 1;
+    while (t >= 0 && ids[t] > key) {
+      ids[t + 1] = ids[t];
+      --t;
+    }
+    ids[t + 1] = key;
+  }
+  V2<T> *e_vel = (V2<T> *)W.e_vel;
+  for (int s = 0; s < cnt; ++s) {
+    const int i = ids[s];
+    const size_t e = (size_t)c * W.K + s;
+    W.e_pos[e] = A.pos[i];
+    e_vel[e] = A.vel[i];
+    W.e_r[e] = A.radius[i];
+    if (W.e_ls) {
+      W.e_ls[e] = A.ls[i];
+      W.e_rs[e] = A.rs[i];
+      W.e_phi[e] = A.phi[i];
+      W.e_rt[e] = A.r_t[i];
+      W.e_rs_r[e] = A.r_s[i];
+      W.e_rts[e] = A.r_ts[i];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Block-list arrays of cd_add_to_cells (counting sort with offsets)
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(256) k_bl_count(const double2 *pos, int n, double cell,
+                                                  Work W) {
+  const Red r = reduce_partials(W);
+  const Grid g = grid_of(r, cell, W.cap);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    publish_grid(W, r, g, true);
+    W.st->scan_epoch = (W.st->scan_epoch + 1u) & 0x3fffffffu;
+    if (W.st->scan_epoch == 0u) W.st->scan_epoch = 1u;
+    W.st->scan_ticket = 0u;
+  }
+  if (!g.ok) return;
+  for (int j = i; j < n; j += gridDim.x * blockDim.x) {
+    const int c = cell_of_pos(pos[j], g, cell);
+    if (c < 0) {
+      atomicOr(&W.st->oob, 1);
+      continue;
+    }
+    W.cell_of[j] = c;
+    W.rank[j] = atomicAdd(&W.count[c], 1);
+  }
+}
+
+// Exclusive scan of the counts, single pass with decoupled look-back.  Each
+// workgroup takes the next tile of kScanTile cells (dynamic ticket, so every
+// tile's predecessors are already resident), publishes its aggregate, then
+// its first wave reads up to 64 predecessor descriptors at once until an
+// inclusive prefix.  A descriptor packs (epoch:30 | flag:2 | value:32); the
+// binning's epoch hides stale descriptors.  The wait is bounded: past
+// kSpinLimit polls the binning is marked invalid instead.
 constexpr long long kSpinLimit = 1ll << 24;
 
 __device__ __forceinline__ unsigned long long scan_pack(unsigned epoch, unsigned flag, int v) {
   return ((unsigned long long)epoch << 34) | ((unsigned long long)flag << 32) | (unsigned)v;
 }
 
-__global__ void __launch_bounds__(kScanThreads) k_bin_scan(DevState *st, const int *count,
-                                                           int *offset,
-                                                           unsigned long long *tiles) {
+__global__ void __launch_bounds__(kScanThreads) k_bl_scan(Work W) {
   __shared__ int s_tile, s_prefix;
   __shared__ int s_wave[kScanThreads / 64];
+  DevState *st = W.st;
   if (st->oob) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       st->valid = 0;
@@ -448,15 +518,14 @@ __global__ void __launch_bounds__(kScanThreads) k_bin_scan(DevState *st, const i
   int sum = 0;
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
-    int idx = base + k;
-    v[k] = idx < C ? count[idx] : 0;
+    const int idx = base + k;
+    v[k] = idx < C ? W.count[idx] : 0;
     sum += v[k];
   }
-  // inclusive scan of the thread sums within each wave, then across waves
   int inc = sum;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
-    int u = __shfl_up(inc, o, 64);
+    const int u = __shfl_up(inc, o, 64);
     if (lane >= o) inc += u;
   }
   if (lane == 63) s_wave[wid] = inc;
@@ -468,9 +537,8 @@ __global__ void __launch_bounds__(kScanThreads) k_bin_scan(DevState *st, const i
     agg += s_wave[w];
   }
   const int excl = wave_off + inc - sum;
+  unsigned long long *tiles = W.tiles;
   if (wid == 0) {
-    // publish this tile's aggregate, then look back over up to 64 predecessor
-    // descriptors at once (one per lane) until an inclusive prefix is found
     if (lane == 0)
       __hip_atomic_store(&tiles[tile], scan_pack(epoch, tile == 0 ? 2u : 1u, agg),
                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -483,17 +551,15 @@ __global__ void __launch_bounds__(kScanThreads) k_bin_scan(DevState *st, const i
       unsigned flag = 2u;
       int val = 0;
       if (idx >= 0) {
-        unsigned long long d =
+        const unsigned long long d =
             __hip_atomic_load(&tiles[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         flag = ((unsigned)(d >> 34) == epoch) ? ((unsigned)(d >> 32) & 3u) : 0u;
         val = (int)(unsigned)d;
       }
       const unsigned long long incl = __ballot(flag == 2u);
       const unsigned long long bad = __ballot(flag == 0u);
-      // lanes up to (and including) the nearest inclusive descriptor count
       const int stop = incl ? (__ffsll((long long)incl) - 1) : 63;
-      const unsigned long long need =
-          (stop >= 63) ? ~0ull : ((2ull << stop) - 1ull);
+      const unsigned long long need = (stop >= 63) ? ~0ull : ((2ull << stop) - 1ull);
       if (bad & need) {
         if (++spins > kSpinLimit) {
           failed = true;
@@ -523,59 +589,34 @@ __global__ void __launch_bounds__(kScanThreads) k_bin_scan(DevState *st, const i
   int run = s_prefix + excl;
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
-    int idx = base + k;
-    if (idx < C) offset[idx] = run;
+    const int idx = base + k;
+    if (idx < C) W.offset[idx] = run;
     run += v[k];
   }
-  if (tile == ntiles - 1 && t == kScanThreads - 1) offset[C] = s_prefix + agg;
+  if (tile == ntiles - 1 && t == kScanThreads - 1) W.offset[C] = s_prefix + agg;
 }
 
-// K1c: scatter agents to their cell's segment.
-__global__ void __launch_bounds__(256) k_bin_scatter(int n, const DevState *st, const int *cell_of, const int *rank,
-                              const int *offset, int *pidx) {
-  if (!st->valid) return;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_bl_scatter(int n, Work W) {
+  if (!W.st->valid) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  pidx[offset[cell_of[i]] + rank[i]] = i;
+  W.pidx[W.offset[W.cell_of[i]] + W.rank[i]] = i;
 }
 
-// K1d: order each cell's segment by agent index (stable counting sort), then
-// gather the cell-sorted copies the pair kernel reads.
-template <typename T>
-__global__ void __launch_bounds__(256) k_bin_sort_gather(Agents<T> A, const DevState *st, const int *offset, int *pidx,
-                                  int *scell, double2 *spos, V2<T> *svel, double *sr,
-                                  double2 *sls, double2 *srs, double *sphi, double *srt,
-                                  double *srs_r, double *srts) {
-  if (!st->valid) return;
-  const int C = st->C;
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= C) return;
-  const int s0 = offset[c], s1 = offset[c + 1];
-  if (s1 - s0 > 1) {
-    for (int s = s0 + 1; s < s1; ++s) {
-      int key = pidx[s];
-      int t = s - 1;
-      while (t >= s0 && pidx[t] > key) {
-        pidx[t + 1] = pidx[t];
-        --t;
-      }
-      pidx[t + 1] = key;
+// Orders each cell's segment by agent index (stable counting sort).
+__global__ void __launch_bounds__(256) k_bl_sort(Work W) {
+  if (!W.st->valid) return;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= W.st->C) return;
+  const int s0 = W.offset[c], s1 = W.offset[c + 1];
+  for (int s = s0 + 1; s < s1; ++s) {
+    const int key = W.pidx[s];
+    int t = s - 1;
+    while (t >= s0 && W.pidx[t] > key) {
+      W.pidx[t + 1] = W.pidx[t];
+      --t;
     }
-  }
-  for (int s = s0; s < s1; ++s) {
-    int i = pidx[s];
-    scell[s] = c;
-    spos[s] = A.pos[i];
-    svel[s] = A.vel[i];
-    sr[s] = A.radius[i];
-    if (sls) {
-      sls[s] = A.ls[i];
-      srs[s] = A.rs[i];
-      sphi[s] = A.phi[i];
-      srt[s] = A.r_t[i];
-      srs_r[s] = A.r_s[i];
-      srts[s] = A.r_ts[i];
-    }
+    W.pidx[t + 1] = key;
   }
 }
 
@@ -583,26 +624,28 @@ __global__ void __launch_bounds__(256) k_bin_sort_gather(Agents<T> A, const DevS
 // Per-agent operators
 // ---------------------------------------------------------------------------
 
-// Fluctuation noise of agent i (core/motion/fluctuation.py:14-62).
+// Fluctuation noise of agent i (core/motion/fluctuation.py:14-62): force
+// (mass * sigma * |TN(0,3)|) * (cos phi, sin phi), phi ~ U(0, 2 pi); torque
+// inertia * sigma_t * TN(-3, 3) when `torque`.
 __device__ __forceinline__ void fluct_noise(uint64_t seed, uint64_t step, int i, double mass,
-                                            double sig_f, double inertia, double sig_t, double &fx,
-                                            double &fy, double &m) {
+                                            double sig_f, bool torque, double inertia,
+                                            double sig_t, double &fx, double &fy, double &m) {
   double ua, ub;
   draw2(seed, i, step, 0, 0, ua, ub);
-  double phi = 2.0 * kPi * ua;
-  double mag = sig_f * fabs(truncnorm3(seed, i, step, 1));
-  double mm = mass * mag;
+  const double phi = 2.0 * kPi * ua;
+  const double mm = mass * (sig_f * fabs(truncnorm3(seed, i, step, 1)));
   fx = mm * cos(phi);
   fy = mm * sin(phi);
-  m = inertia * (sig_t * truncnorm3(seed, i, step, 2));
+  m = torque ? inertia * (sig_t * truncnorm3(seed, i, step, 2)) : 0.0;
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_fluct(Agents<T> A, int model, uint64_t seed, uint64_t step) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_fluct(Agents<T> A, int model, uint64_t seed,
+                                               uint64_t step) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= A.n) return;
   double fx, fy, m;
-  fluct_noise(seed, step, i, A.mass[i], A.sig_f[i], model ? A.inertia[i] : 0.0,
+  fluct_noise(seed, step, i, A.mass[i], A.sig_f[i], model != 0, model ? A.inertia[i] : 0.0,
               model ? A.sig_t[i] : 0.0, fx, fy, m);
   V2<T> f = A.force[i];
   f.x += (T)fx;
@@ -611,18 +654,18 @@ __global__ void __launch_bounds__(256) k_fluct(Agents<T> A, int model, uint64_t 
   if (model) A.torque[i] += m;
 }
 
-// Navigation lookup of agent i; returns false when e0 is kept.
+// Navigation lookup; returns false when e0 is kept.
 __device__ __forceinline__ bool nav_lookup(double px, double py, int64_t t, const double *maps,
                                            int64_t T_, int64_t rows, int64_t cols, double minx,
                                            double miny, double step, int mode, double &ex,
                                            double &ey) {
   if (t < 0 || t >= T_) return false;
-  double qx = (px - minx) / step;
-  double qy = (py - miny) / step;
+  const double qx = (px - minx) / step;
+  const double qy = (py - miny) / step;
   if (mode == 0) {
     // indicer truncation + is_inside (quickest_path.py:41-44, navigation.py:59-78)
     if (!(qx > -1.0 && qx < (double)cols && qy > -1.0 && qy < (double)rows)) return false;
-    int64_t jj = (int64_t)qx, ii = (int64_t)qy;
+    const int64_t jj = (int64_t)qx, ii = (int64_t)qy;
     const double2 m = ((const double2 *)maps)[(t * rows + ii) * cols + jj];
     ex = m.x;
     ey = m.y;
@@ -634,13 +677,13 @@ __device__ __forceinline__ bool nav_lookup(double px, double py, int64_t t, cons
   if (c0 > cols - 2) c0 = cols - 2;
   if (r0 > rows - 2) r0 = rows - 2;
   if (c0 < 0 || r0 < 0) return false;
-  double fx = qx - (double)c0, fy = qy - (double)r0;
+  const double fx = qx - (double)c0, fy = qy - (double)r0;
   const double *P = maps + t * rows * cols;
-  double p00 = P[r0 * cols + c0], p01 = P[r0 * cols + c0 + 1];
-  double p10 = P[(r0 + 1) * cols + c0], p11 = P[(r0 + 1) * cols + c0 + 1];
-  double gx = ((p01 - p00) * (1.0 - fy) + (p11 - p10) * fy) / step;
-  double gy = ((p10 - p00) * (1.0 - fx) + (p11 - p01) * fx) / step;
-  double g = hypot(gx, gy);
+  const double p00 = P[r0 * cols + c0], p01 = P[r0 * cols + c0 + 1];
+  const double p10 = P[(r0 + 1) * cols + c0], p11 = P[(r0 + 1) * cols + c0 + 1];
+  const double gx = ((p01 - p00) * (1.0 - fy) + (p11 - p10) * fy) / step;
+  const double gy = ((p10 - p00) * (1.0 - fx) + (p11 - p01) * fx) / step;
+  const double g = hypot(gx, gy);
   if (!(g > 0.0)) return false;
   ex = -gx / g;
   ey = -gy / g;
@@ -648,11 +691,12 @@ __device__ __forceinline__ bool nav_lookup(double px, double py, int64_t t, cons
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_nav(Agents<T> A, const double *maps, int64_t T_, int64_t rows, int64_t cols,
-                      double minx, double miny, double step, int mode) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_nav(Agents<T> A, const double *maps, int64_t T_,
+                                             int64_t rows, int64_t cols, double minx,
+                                             double miny, double step, int mode) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= A.n) return;
-  double2 p = A.pos[i];
+  const double2 p = A.pos[i];
   double ex, ey;
   if (nav_lookup(p.x, p.y, A.target[i], maps, T_, rows, cols, minx, miny, step, mode, ex, ey)) {
     V2<T> e;
@@ -664,18 +708,18 @@ __global__ void __launch_bounds__(256) k_nav(Agents<T> A, const double *maps, in
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_orient(Agents<T> A) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= A.n) return;
-  V2<T> e = A.e0[i];
+  const V2<T> e = A.e0[i];
   A.phi0[i] = atan2((double)e.y, (double)e.x);  // orientation.py:21
 }
 
 // force_adjust (core/motion/adjusting.py:16-51)
 template <typename T>
-__device__ __forceinline__ void adjust_force(const Agents<T> &A, int i, V2<T> v, V2<T> e,
-                                             T &fx, T &fy) {
-  T s = (T)(A.mass[i] / A.tau_adj[i]);
-  T v0 = (T)A.v0[i];
+__device__ __forceinline__ void adjust_force(const Agents<T> &A, int i, V2<T> v, V2<T> e, T &fx,
+                                             T &fy) {
+  const T s = (T)(A.mass[i] / A.tau_adj[i]);
+  const T v0 = (T)A.v0[i];
   fx = s * (v0 * e.x - v.x);
   fy = s * (v0 * e.y - v.y);
 }
@@ -683,13 +727,13 @@ __device__ __forceinline__ void adjust_force(const Agents<T> &A, int i, V2<T> v,
 // torque_adjust (core/motion/adjusting.py:54-95)
 template <typename T>
 __device__ __forceinline__ double adjust_torque(const Agents<T> &A, int i) {
-  double s = A.inertia[i] / A.tau_rot[i];
+  const double s = A.inertia[i] / A.tau_rot[i];
   return s * ((wrap_to_pi(A.phi0[i] - A.phi[i]) / kPi) * A.w0[i] - A.omega[i]);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_adjust(Agents<T> A, int do_force, int do_torque) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= A.n) return;
   if (do_force) {
     T fx, fy;
@@ -704,7 +748,7 @@ __global__ void __launch_bounds__(256) k_adjust(Agents<T> A, int do_force, int d
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_reset(Agents<T> A, int model) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= A.n) return;
   V2<T> z;
   z.x = T(0);
@@ -713,14 +757,14 @@ __global__ void __launch_bounds__(256) k_reset(Agents<T> A, int model) {
   if (model) A.torque[i] = 0.0;
 }
 
-// Wall contacts of agent i in segment order (core/interactions.py:107-141,
-// 169-186).  Segments are staged through LDS by the caller.
+// Wall contacts of agent i over segments w_begin, w_begin + w_stride, ...
+// (core/interactions.py:107-141, 169-186).
 template <typename T>
 __device__ __forceinline__ void walls_agent(const Agents<T> &A, int model, int i, double2 p,
                                             V2<T> v, const double4 *seg, int W, T &fx, T &fy,
                                             double &tq, int w_begin = 0, int w_stride = 1) {
   const double r = A.radius[i];
-  double2 ls, rs;
+  double2 ls = p, rs = p;
   double rt = 0.0, rsr = 0.0;
   if (model) {
     ls = A.ls[i];
@@ -730,7 +774,7 @@ __device__ __forceinline__ void walls_agent(const Agents<T> &A, int model, int i
   }
   const T mu = (T)A.mu[i], kappa = (T)A.kappa[i], damping = (T)A.damping[i];
   for (int w = w_begin; w < W; w += w_stride) {
-    double4 s = seg[w];
+    const double4 s = seg[w];
     double h, nx, ny, rmx = 0.0, rmy = 0.0;
     if (!model) {
       distance_circle_line(p.x, p.y, r, s.x, s.y, s.z, s.w, h, nx, ny);
@@ -761,7 +805,7 @@ __device__ __forceinline__ void walls_agent(const Agents<T> &A, int model, int i
     }
     if (h < 0.0) {
       T cx, cy;
-      T tnx = (T)nx, tny = (T)ny;
+      const T tnx = (T)nx, tny = (T)ny;
       force_contact((T)h, tnx, tny, v.x, v.y, tny, -tnx, mu, kappa, damping, cx, cy);
       fx += cx;
       fy += cy;
@@ -772,11 +816,13 @@ __device__ __forceinline__ void walls_agent(const Agents<T> &A, int model, int i
 
 constexpr int kSegTile = 512;
 
+// AgentObstacleInteractions node: walls in the reference's order.
 template <typename T>
-__global__ void __launch_bounds__(256) k_walls(Agents<T> A, int model, const double4 *segs, int W) {
+__global__ void __launch_bounds__(256) k_walls(Agents<T> A, int model, const double4 *segs,
+                                               int W) {
   __shared__ double4 sseg[kSegTile];
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  bool live = i < A.n;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = i < A.n;
   double2 p;
   V2<T> v, f;
   double tq = 0.0;
@@ -787,7 +833,7 @@ __global__ void __launch_bounds__(256) k_walls(Agents<T> A, int model, const dou
     if (model) tq = A.torque[i];
   }
   for (int w0 = 0; w0 < W; w0 += kSegTile) {
-    int cnt = min(kSegTile, W - w0);
+    const int cnt = min(kSegTile, W - w0);
     __syncthreads();
     for (int k = threadIdx.x; k < cnt; k += blockDim.x) sseg[k] = segs[w0 + k];
     __syncthreads();
@@ -800,42 +846,26 @@ __global__ void __launch_bounds__(256) k_walls(Agents<T> A, int model, const dou
 }
 
 // ---------------------------------------------------------------------------
-// Pair forces (owner-computes over the 3x3 cell stencil)
+// K_e: forces (owner-computes over the 3x3 cell stencil)
 // ---------------------------------------------------------------------------
 //
-// Four lanes serve one agent (the agent of sorted slot s = thread / 4).
-// Lanes 0..2 sum the agent's side of every pair in stencil row dx = -1, 0, +1
-// (the three cells of a row are contiguous in the cell-sorted order, so a
-// row is one slot range); lane 3 handles the agent's own terms
-// (fluctuation, navigation, orientation, adjusting) and the wall contacts
-// are split over the four lanes.  The partial sums meet on lane 3 through
-// wavefront shuffles in a fixed order, so results are deterministic.
+// Four lanes serve agent i = thread / 4 (agent index order, so the agent's
+// own data loads are coalesced).  Lanes 0..2 sum the agent's side of every
+// pair in stencil row dx = -1, 0, +1 (cells ly-1..ly+1 in slot order within
+// each cell); lane 3 evaluates the agent's own terms (fluctuation,
+// navigation, orientation, adjusting), issuing its loads before the pair
+// work; wall contacts are split over the four lanes.  The partial sums meet
+// on lane 3 through wavefront shuffles in a fixed order (deterministic).
 
-struct Slots {
-  const int *offset, *pidx, *scell;
-  const double2 *spos;
-  const void *svel;
-  const double *sr;
-  const double2 *sls, *srs;
-  const double *sphi, *srt, *srs_r, *srts;
+struct FusedArgs {
+  int flags, nav_mode;
+  uint64_t seed;
+  const double *maps;
+  int64_t nav_T, nav_rows, nav_cols;
+  double nav_minx, nav_miny, nav_step;
+  const double4 *segs;
+  int W;
 };
-
-Slots slots_of(const Work &W) {
-  Slots S;
-  S.offset = W.offset;
-  S.pidx = W.pidx;
-  S.scell = W.scell;
-  S.spos = W.spos;
-  S.svel = W.svel;
-  S.sr = W.sr;
-  S.sls = W.sls;
-  S.srs = W.srs;
-  S.sphi = W.sphi;
-  S.srt = W.srt;
-  S.srs_r = W.srs_r;
-  S.srts = W.srts;
-  return S;
-}
 
 template <typename T>
 __device__ __forceinline__ Owner<T> owner_of(const Agents<T> &A, int i) {
@@ -848,145 +878,170 @@ __device__ __forceinline__ Owner<T> owner_of(const Agents<T> &A, int i) {
   return o;
 }
 
-// Slot range of stencil row ox (cells ylo..yhi of column ox).
-__device__ __forceinline__ void row_range(const int *offset, int ox, int ny, int ly, int &k0,
-                                          int &k1) {
-  const int ylo = max(ly - 1, 0), yhi = min(ly + 1, ny - 1);
-  k0 = offset[ox * ny + ylo];
-  k1 = offset[ox * ny + yhi + 1];
-}
-
-// Circular pairs of one stencil row, neighbour loads pipelined one ahead.
-template <typename T>
-__device__ __forceinline__ void row_circular(const Slots &S, int k0, int k1, int s, double2 xi,
-                                             V2<T> vi, double ri, const Owner<T> &o, T &fx,
-                                             T &fy) {
-  const V2<T> *svel = (const V2<T> *)S.svel;
-  if (k0 >= k1) return;
-  double2 xn = S.spos[k0];
-  V2<T> vn = svel[k0];
-  double rn = S.sr[k0];
-  for (int k = k0; k < k1; ++k) {
-    const double2 xj = xn;
-    const V2<T> vj = vn;
-    const double rj = rn;
-    if (k + 1 < k1) {
-      xn = S.spos[k + 1];
-      vn = svel[k + 1];
-      rn = S.sr[k + 1];
-    }
-    if (k == s) continue;
-    T px, py;
-    if (pair_circular<T>((T)(xi.x - xj.x), (T)(xi.y - xj.y), vi.x - vj.x, vi.y - vj.y,
-                         (T)(ri + rj), o, px, py)) {
-      fx += px;
-      fy += py;
-    }
-  }
-}
-
-__device__ __forceinline__ TCView tc_view(const Slots &S, int k) {
+__device__ __forceinline__ TCView tc_slot(const Work &W, size_t e) {
   TCView t;
-  double2 p = S.spos[k], l = S.sls[k], r = S.srs[k];
+  const double2 p = W.e_pos[e], l = W.e_ls[e], r = W.e_rs[e];
   t.px[0] = p.x;
   t.py[0] = p.y;
   t.px[1] = l.x;
   t.py[1] = l.y;
   t.px[2] = r.x;
   t.py[2] = r.y;
-  V2<double> v = ((const V2<double> *)S.svel)[k];
+  const V2<double> v = ((const V2<double> *)W.e_vel)[e];
   t.vx = v.x;
   t.vy = v.y;
-  t.r_t = S.srt[k];
-  t.r_s = S.srs_r[k];
-  t.r_ts = S.srts[k];
-  t.phi = S.sphi[k];
+  t.r_t = W.e_rt[e];
+  t.r_s = W.e_rs_r[e];
+  t.r_ts = W.e_rts[e];
+  t.phi = W.e_phi[e];
   return t;
 }
 
-// Three-circle pairs of one stencil row: each pair is evaluated in the
-// reference's (min index, max index) argument order, keeping the owner's side.
-__device__ __forceinline__ void row_three_circle(const Slots &S, int k0, int k1, int s, int i,
-                                                 const TCView &me, const Owner<double> &o,
-                                                 double &fx, double &fy, double &tq) {
-  for (int k = k0; k < k1; ++k) {
-    if (k == s) continue;
-    const int j = S.pidx[k];
-    const TCView other = tc_view(S, k);
-    double px, py, m;
-    bool hit = (i < j) ? pair_three_circle(me, other, false, o, px, py, m)
-                       : pair_three_circle(other, me, true, o, px, py, m);
-    if (hit) {
-      fx += px;
-      fy += py;
-      tq += m;
-    }
+template <typename T>
+__device__ __forceinline__ TCView tc_agent(const Agents<T> &A, int j) {
+  TCView t;
+  const double2 p = A.pos[j], l = A.ls[j], r = A.rs[j];
+  t.px[0] = p.x;
+  t.py[0] = p.y;
+  t.px[1] = l.x;
+  t.py[1] = l.y;
+  t.px[2] = r.x;
+  t.py[2] = r.y;
+  const V2<T> v = A.vel[j];
+  t.vx = (double)v.x;
+  t.vy = (double)v.y;
+  t.r_t = A.r_t[j];
+  t.r_s = A.r_s[j];
+  t.r_ts = A.r_ts[j];
+  t.phi = A.phi[j];
+  return t;
+}
+
+// Circular: agent i's side of its pair with a neighbour at xj, vj, rj.
+template <typename T>
+__device__ __forceinline__ void pair_add_c(double2 xi, V2<T> vi, double ri, const Owner<T> &o,
+                                           double2 xj, V2<T> vj, double rj, T &fx, T &fy) {
+  T px, py;
+  if (pair_circular<T>((T)(xi.x - xj.x), (T)(xi.y - xj.y), vi.x - vj.x, vi.y - vj.y,
+                       (T)(ri + rj), o, px, py)) {
+    fx += px;
+    fy += py;
   }
 }
 
-struct FusedArgs {
-  int flags, nav_mode;
-  uint64_t seed;
-  const double *maps;
-  int64_t nav_T, nav_rows, nav_cols;
-  double nav_minx, nav_miny, nav_step;
-  const double4 *segs;
-  int W;
-};
+// Three-circle: the reference evaluates the pair as interaction(min, max).
+__device__ __forceinline__ void pair_add_tc(int i, int j, const TCView &me, const TCView &other,
+                                            const Owner<double> &o, double &fx, double &fy,
+                                            double &tq) {
+  double px, py, m;
+  const bool hit = (i < j) ? pair_three_circle(me, other, false, o, px, py, m)
+                           : pair_three_circle(other, me, true, o, px, py, m);
+  if (hit) {
+    fx += px;
+    fy += py;
+    tq += m;
+  }
+}
+
+// Pairs of agent i with the agents of cell c.
+template <typename T, int MODEL>
+__device__ __forceinline__ void cell_pairs(const Agents<T> &A, const Work &W, int c, int i,
+                                           double2 xi, V2<T> vi, double ri, const Owner<T> &o,
+                                           const TCView &me, T &fx, T &fy, double &tq) {
+  const int cnt = W.count[c];
+  if (cnt <= 0) return;
+  if (cnt <= W.K) {
+    const size_t e0 = (size_t)c * W.K;
+    const V2<T> *e_vel = (const V2<T> *)W.e_vel;
+    for (int s = 0; s < cnt; ++s) {
+      const size_t e = e0 + s;
+      const int j = W.ell_ids[e];
+      if (j == i) continue;
+      if constexpr (MODEL == 0) {
+        pair_add_c<T>(xi, vi, ri, o, W.e_pos[e], e_vel[e], W.e_r[e], fx, fy);
+      } else {
+        if constexpr (sizeof(T) == 8) pair_add_tc(i, j, me, tc_slot(W, e), o, fx, fy, tq);
+      }
+    }
+    return;
+  }
+  // more agents than slots: visit the cell's members in agent index order
+  // (the contract's order) by scanning the cell of every agent
+  for (int j = 0; j < A.n; ++j) {
+    if (j == i || W.cell_of[j] != c) continue;
+    if constexpr (MODEL == 0) {
+      pair_add_c<T>(xi, vi, ri, o, A.pos[j], A.vel[j], A.radius[j], fx, fy);
+    } else {
+      if constexpr (sizeof(T) == 8) pair_add_tc(i, j, me, tc_agent(A, j), o, fx, fy, tq);
+    }
+  }
+}
 
 // FUSED = false: the AgentAgentInteractions node, force (torque) += pairs.
 // FUSED = true: all per-agent forces of one step in the post-order of the
 // standard logic tree (fluctuation, navigation, orientation, adjusting,
-// pairs, walls), written to force (torque).
+// pairs, walls), added to force (torque).
 template <typename T, int MODEL, bool FUSED>
-__global__ void __launch_bounds__(256) k_forces(Agents<T> A, DevState *st, FusedArgs fa,
-                                                Slots S) {
+__global__ void __launch_bounds__(256) k_forces(Agents<T> A, Work W, FusedArgs fa) {
   __shared__ double4 sseg[kSegTile];
   const int gt = blockIdx.x * blockDim.x + threadIdx.x;
-  const int s = gt >> 2;
+  const int i = gt >> 2;
   const int q = threadIdx.x & 3;
   const int lane0 = (threadIdx.x & 63) & ~3;
+  const DevState *st = W.st;
   const bool pairs_on = FUSED ? ((fa.flags & CD_USE_AGENT_AGENT) != 0) : true;
-  const bool sorted = pairs_on && st->valid;
-  if (!FUSED && !sorted) return;  // node mode: binning invalid, nothing to add
-  const bool have = s < A.n;
-  int i = -1, c = 0;
+  const bool binned = pairs_on && st->valid && !st->oob;
+  if (!FUSED && !binned) return;  // node mode: binning invalid, nothing to add
+  const bool have = i < A.n;
+  double2 pos = make_double2(0.0, 0.0);
+  V2<T> vel;
+  vel.x = T(0);
+  vel.y = T(0);
   if (have) {
-    if (sorted) {
-      i = S.pidx[s];
-      c = S.scell[s];
-    } else {
-      i = s;
+    pos = A.pos[i];
+    vel = A.vel[i];
+  }
+  // lane 3 loads the agent's own terms first, so their latency overlaps the
+  // pair work of the other lanes
+  V2<T> f;
+  f.x = T(0);
+  f.y = T(0);
+  double tq = 0.0;
+  V2<T> e;
+  e.x = T(0);
+  e.y = T(0);
+  bool nav_hit = false;
+  double nex = 0.0, ney = 0.0;
+  if (have && q == 3) {
+    f = A.force[i];
+    if (MODEL) tq = A.torque[i];
+    if (FUSED) {
+      e = A.e0[i];
+      if (fa.flags & CD_USE_NAV)
+        nav_hit = nav_lookup(pos.x, pos.y, A.target[i], fa.maps, fa.nav_T, fa.nav_rows,
+                             fa.nav_cols, fa.nav_minx, fa.nav_miny, fa.nav_step, fa.nav_mode,
+                             nex, ney);
     }
   }
   T px = T(0), py = T(0);  // this lane's pair partial sum
-  T wx = T(0), wy = T(0);  // this lane's wall partial sum
-  double ptq = 0.0, wtq = 0.0;
-  double2 pos;
-  V2<T> vel;
-  if (have) {
-    pos = sorted ? S.spos[s] : A.pos[i];
-    vel = sorted ? ((const V2<T> *)S.svel)[s] : A.vel[i];
-  }
-  if (have && sorted && q < 3) {
+  double ptq = 0.0;
+  if (have && binned && q < 3) {
+    const int c = W.cell_of[i];
     const int ny = st->ny, nx = st->nx;
     const int lx = c / ny, ly = c - lx * ny;
     const int ox = lx + q - 1;
     if (ox >= 0 && ox < nx) {
-      int k0, k1;
-      row_range(S.offset, ox, ny, ly, k0, k1);
-      if constexpr (MODEL == 0) {
-        row_circular<T>(S, k0, k1, s, pos, vel, S.sr[s], owner_of(A, i), px, py);
-      } else {
-        if constexpr (sizeof(T) == 8) {
-          double fx = 0.0, fy = 0.0;
-          row_three_circle(S, k0, k1, s, i, tc_view(S, s), owner_of(A, i), fx, fy, ptq);
-          px = fx;
-          py = fy;
-        }
-      }
+      const Owner<T> o = owner_of(A, i);
+      TCView me;
+      if constexpr (MODEL == 1) me = tc_agent(A, i);
+      const double ri = A.radius[i];
+      const int ylo = max(ly - 1, 0), yhi = min(ly + 1, ny - 1);
+      for (int oy = ylo; oy <= yhi; ++oy)
+        cell_pairs<T, MODEL>(A, W, ox * ny + oy, i, pos, vel, ri, o, me, px, py, ptq);
     }
   }
+  T wx = T(0), wy = T(0);  // this lane's wall partial sum
+  double wtq = 0.0;
   if (FUSED && (fa.flags & CD_USE_OBSTACLE)) {
     for (int w0 = 0; w0 < fa.W; w0 += kSegTile) {
       const int cnt = min(kSegTile, fa.W - w0);
@@ -996,7 +1051,7 @@ __global__ void __launch_bounds__(256) k_forces(Agents<T> A, DevState *st, Fused
       if (have) walls_agent(A, MODEL, i, pos, vel, sseg, cnt, wx, wy, wtq, q, 4);
     }
   }
-  // partial sums of lanes 0..3 to lane 3 (all lanes take part in the shuffles)
+  // partial sums of lanes 0..3 to lane 3 (every lane takes part in the shuffles)
   const T p0x = __shfl(px, lane0 + 0, 64), p0y = __shfl(py, lane0 + 0, 64);
   const T p1x = __shfl(px, lane0 + 1, 64), p1y = __shfl(py, lane0 + 1, 64);
   const T p2x = __shfl(px, lane0 + 2, 64), p2y = __shfl(py, lane0 + 2, 64);
@@ -1016,31 +1071,19 @@ __global__ void __launch_bounds__(256) k_forces(Agents<T> A, DevState *st, Fused
     u2 = __shfl(wtq, lane0 + 2, 64);
   }
   if (!have || q != 3) return;
-  V2<T> f = A.force[i];
-  double tq = MODEL ? A.torque[i] : 0.0;
   if (FUSED) {
     if (fa.flags & CD_USE_FLUCT) {
-      double nfx, nfy, nm = 0.0;
-      double ua, ub;
-      draw2(fa.seed, i, st->step, 0, 0, ua, ub);
-      const double phi = 2.0 * kPi * ua;
-      const double mm = A.mass[i] * (A.sig_f[i] * fabs(truncnorm3(fa.seed, i, st->step, 1)));
-      nfx = mm * cos(phi);
-      nfy = mm * sin(phi);
-      if (MODEL) nm = A.inertia[i] * (A.sig_t[i] * truncnorm3(fa.seed, i, st->step, 2));
+      double nfx, nfy, nm;
+      fluct_noise(fa.seed, st->step, i, A.mass[i], A.sig_f[i], MODEL != 0,
+                  MODEL ? A.inertia[i] : 0.0, MODEL ? A.sig_t[i] : 0.0, nfx, nfy, nm);
       f.x += (T)nfx;
       f.y += (T)nfy;
       if (MODEL) tq += nm;
     }
-    V2<T> e = A.e0[i];
-    if (fa.flags & CD_USE_NAV) {
-      double ex, ey;
-      if (nav_lookup(pos.x, pos.y, A.target[i], fa.maps, fa.nav_T, fa.nav_rows, fa.nav_cols,
-                     fa.nav_minx, fa.nav_miny, fa.nav_step, fa.nav_mode, ex, ey)) {
-        e.x = (T)ex;
-        e.y = (T)ey;
-        A.e0[i] = e;
-      }
+    if (nav_hit) {
+      e.x = (T)nex;
+      e.y = (T)ney;
+      A.e0[i] = e;
     }
     if (fa.flags & CD_USE_ADJUST) {
       if (MODEL) A.phi0[i] = atan2((double)e.y, (double)e.x);
@@ -1064,110 +1107,118 @@ __global__ void __launch_bounds__(256) k_forces(Agents<T> A, DevState *st, Fused
 }
 
 // ---------------------------------------------------------------------------
-// Integrator
+// K_f: integrator
 // ---------------------------------------------------------------------------
 
-// adaptive_timestep (core/integrator.py:81-97) from the reduced values.
-__device__ __forceinline__ double adaptive_dt(const DevState *st, double dt_min, double dt_max) {
-  double v_max = __longlong_as_double((long long)st->red[0][R_VMAX]);
+// adaptive_timestep (core/integrator.py:81-97) from reduced values.
+__device__ __forceinline__ double adaptive_dt(const Red &r, double dt_min, double dt_max) {
+  const double v_max = red_vmax(r);
   if (v_max == 0.0) return dt_max;
-  double v0max = from_ord_bits(st->red[0][R_V0MAX]);
-  double dx_max = 1.1 * v0max * dt_max;
-  double dt = dx_max / v_max;
+  const double v0max = from_ord_bits(r.v[R_V0MAX]);
+  const double dx_max = 1.1 * v0max * dt_max;
+  const double dt = dx_max / v_max;
   if (dt > dt_max) return dt_max;
   if (dt < dt_min) return dt_min;
   return dt;
 }
 
 // translational_verlet (core/integrator.py:164-176), rotational_verlet
-// (:179-193) and shoulders (simulation/agents.py:473-486).  With `fused`,
-// also Reset (simulation/logic.py:59-64), the next step's reductions and the
-// clearing of this step's cell counts.
+// (:179-193) and shoulders (simulation/agents.py:473-486) of agent i.
 template <typename T>
-__global__ void __launch_bounds__(256) k_integrate(Agents<T> A, int model, DevState *st, double dt_min, double dt_max,
-                            int fused, int do_reset, int *count,
-                            unsigned long long *partials) {
-  const double dt = adaptive_dt(st, dt_min, dt_max);
-  const int n = A.n;
-  unsigned long long v[R_COUNT] = {kOrdPosInf, kOrdPosInf, kOrdNegInf, kOrdNegInf, 0ull,
-                                   kOrdNegInf, 0ull};
-  double vmax = 0.0;
-  if (fused) {
-    const int clear = st->C_clear;
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < clear; c += gridDim.x * blockDim.x)
-      count[c] = 0;
+__device__ __forceinline__ void verlet_agent(const Agents<T> &A, int model, int i, double dt,
+                                             bool do_reset, double2 &x_out, V2<T> &v_out) {
+  const double m = A.mass[i];
+  const V2<T> f = A.force[i], fp = A.force_prev[i];
+  V2<T> vel = A.vel[i];
+  double2 x = A.pos[i];
+  const T tm = (T)m;
+  const T aox = fp.x / tm, aoy = fp.y / tm;
+  const T anx = f.x / tm, any = f.y / tm;
+  vel.x += (aox + anx) / T(2) * (T)dt;
+  vel.y += (aoy + any) / T(2) * (T)dt;
+  if (sizeof(T) == 8) {
+    x.x += (double)vel.x * dt + (double)anx / 2.0 * (dt * dt);
+    x.y += (double)vel.y * dt + (double)any / 2.0 * (dt * dt);
+  } else {
+    // fp32 mode: fp64 position accumulation of the fp32 increments
+    x.x += (double)vel.x * dt + (double)(anx / T(2)) * (dt * dt);
+    x.y += (double)vel.y * dt + (double)(any / T(2)) * (dt * dt);
   }
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
-    const double m = A.mass[i];
-    V2<T> f = A.force[i], fp = A.force_prev[i], vel = A.vel[i];
-    double2 x = A.pos[i];
-    const T tm = (T)m;
-    T aox = fp.x / tm, aoy = fp.y / tm;
-    T anx = f.x / tm, any = f.y / tm;
-    vel.x += (aox + anx) / T(2) * (T)dt;
-    vel.y += (aoy + any) / T(2) * (T)dt;
-    if (sizeof(T) == 8) {
-      x.x += (double)vel.x * dt + (double)anx / 2.0 * (dt * dt);
-      x.y += (double)vel.y * dt + (double)any / 2.0 * (dt * dt);
-    } else {
-      // fp32 mode: fp64 position accumulation of the fp32 increments
-      x.x += (double)vel.x * dt + (double)(anx / T(2)) * (dt * dt);
-      x.y += (double)vel.y * dt + (double)(any / T(2)) * (dt * dt);
-    }
-    A.force_prev[i] = f;
-    A.vel[i] = vel;
-    A.pos[i] = x;
-    if (do_reset) {
-      V2<T> z;
-      z.x = T(0);
-      z.y = T(0);
-      A.force[i] = z;
-    }
-    if (model) {
-      const double I = A.inertia[i];
-      double ao = A.torque_prev[i] / I;
-      double an = A.torque[i] / I;
-      A.torque_prev[i] = A.torque[i];
-      double w = A.omega[i] + (ao + an) / 2.0 * dt;
-      double ph = A.phi[i] + (w * dt + an / 2.0 * (dt * dt));
-      ph = wrap_to_pi(ph);
-      A.omega[i] = w;
-      A.phi[i] = ph;
-      const double rts = A.r_ts[i];
-      double ox = sin(ph) * rts, oy = (-cos(ph)) * rts;
-      double2 l, r;
-      l.x = x.x - ox;
-      l.y = x.y - oy;
-      r.x = x.x + ox;
-      r.y = x.y + oy;
-      A.ls[i] = l;
-      A.rs[i] = r;
-      if (do_reset) A.torque[i] = 0.0;
-    }
-    if (fused) {
-      if (!(isfinite(x.x) && isfinite(x.y))) v[R_NONFINITE] = 1ull;
-      unsigned long long bx = ord_bits(x.x), by = ord_bits(x.y);
-      v[R_MINX] = bx;
-      v[R_MINY] = by;
-      v[R_MAXX] = bx;
-      v[R_MAXY] = by;
-      double l = dhypot((double)vel.x, (double)vel.y);
-      if (l > vmax) vmax = l;
-      v[R_V0MAX] = ord_bits(A.v0[i]);
-    }
+  A.force_prev[i] = f;
+  A.vel[i] = vel;
+  A.pos[i] = x;
+  if (do_reset) {
+    V2<T> z;
+    z.x = T(0);
+    z.y = T(0);
+    A.force[i] = z;
   }
-  if (fused) {
-    v[R_VMAX] = (unsigned long long)__double_as_longlong(vmax);
-    if (block_reduce_commit(st, v, 1, partials) && threadIdx.x == 0) {
-      st->dt = dt;
-      st->time_tot += dt;
-      st->step += 1;
-      st->C_clear = 0;
-    }
-  } else if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (model) {
+    const double I = A.inertia[i];
+    const double ao = A.torque_prev[i] / I;
+    const double an = A.torque[i] / I;
+    A.torque_prev[i] = A.torque[i];
+    const double w = A.omega[i] + (ao + an) / 2.0 * dt;
+    double ph = A.phi[i] + (w * dt + an / 2.0 * (dt * dt));
+    ph = wrap_to_pi(ph);
+    A.omega[i] = w;
+    A.phi[i] = ph;
+    const double rts = A.r_ts[i];
+    const double ox = sin(ph) * rts, oy = (-cos(ph)) * rts;
+    A.ls[i] = make_double2(x.x - ox, x.y - oy);
+    A.rs[i] = make_double2(x.x + ox, x.y + oy);
+    if (do_reset) A.torque[i] = 0.0;
+  }
+  x_out = x;
+  v_out = vel;
+}
+
+// Node mode: dt from the partials the preceding k_reduce wrote.
+template <typename T>
+__global__ void __launch_bounds__(256) k_integrate_node(Agents<T> A, int model, Work W,
+                                                        double dt_min, double dt_max) {
+  const Red r = reduce_partials(W);
+  const double dt = adaptive_dt(r, dt_min, dt_max);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    for (int k = 0; k < R_COUNT; ++k) W.st->red[k] = r.v[k];
+    W.st->dt = dt;
+    W.st->time_tot += dt;
+  }
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gridDim.x * blockDim.x) {
+    double2 x;
+    V2<T> v;
+    verlet_agent(A, model, i, dt, false, x, v);
+  }
+}
+
+// Fused step: dt from the values K_a reduced (st->red); Verlet and Reset of
+// every agent (grid-stride, at most kMaxParts blocks); partials of the new
+// state for the next step; zero this step's cell counts.
+template <typename T>
+__global__ void __launch_bounds__(256) k_integrate(Agents<T> A, int model, Work W,
+                                                   double dt_min, double dt_max, int do_reset) {
+  DevState *st = W.st;
+  Red prev;
+  for (int k = 0; k < R_COUNT; ++k) prev.v[k] = st->red[k];
+  const double dt = adaptive_dt(prev, dt_min, dt_max);
+  const int stride = gridDim.x * blockDim.x;
+  const int clear = st->C_clear;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < clear; c += stride) W.count[c] = 0;
+  Red r;
+  r.init();
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += stride) {
+    double2 x;
+    V2<T> v;
+    verlet_agent(A, model, i, dt, do_reset != 0, x, v);
+    r.add_agent(x, dhypot((double)v.x, (double)v.y), A.v0[i]);
+  }
+  r = block_reduce(r);
+  store_partial(W, r);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->dt = dt;
     st->time_tot += dt;
+    st->step += 1;
+    st->oob = 0;
   }
 }
 
@@ -1176,14 +1227,13 @@ __global__ void __launch_bounds__(256) k_integrate(Agents<T> A, int model, DevSt
 // ---------------------------------------------------------------------------
 
 inline int grid_for(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
-
 inline int check_launch() { return hipGetLastError() == hipSuccess ? CD_OK : CD_ERR_HIP; }
 
 int check_agents(const cd_agents_t *a) {
   if (!a) return CD_ERR_ARG;
   if (a->model != CD_CIRCULAR && a->model != CD_THREE_CIRCLE) return CD_ERR_TYPE;
   if (a->real != CD_F64 && a->real != CD_F32) return CD_ERR_ARG;
-  if (a->n < 0 || a->n > (int64_t)0x7fffffff) return CD_ERR_ARG;
+  if (a->n < 0 || a->n > (int64_t)0x1fffffff) return CD_ERR_ARG;
   if (a->model == CD_THREE_CIRCLE && a->real != CD_F64) return CD_ERR_UNSUPPORTED;
   if (a->n == 0) return CD_OK;
   if (!a->position || !a->velocity || !a->target_direction || !a->force || !a->force_prev ||
@@ -1199,91 +1249,157 @@ int check_agents(const cd_agents_t *a) {
   return CD_OK;
 }
 
-int check_ws(const cd_agents_t *a, void *ws, int64_t ws_bytes, int64_t cap) {
-  if (!ws || cap <= 0 || cap > (int64_t)0x3fffffff) return CD_ERR_ARG;
-  Layout L = make_layout(a->n, cap, a->model);
-  if (ws_bytes < L.total) return CD_ERR_WORKSPACE;
+bool ws_sizes_ok(const cd_workspace_t *w) {
+  return w && w->n >= 0 && w->n <= (int64_t)0x1fffffff && w->cell_capacity > 0 &&
+         w->cell_capacity <= (int64_t)0x1fffffff && w->cell_slots > 0 &&
+         w->cell_slots <= 4096 && w->cell_capacity * w->cell_slots <= ((int64_t)1 << 31) &&
+         (w->model == CD_CIRCULAR || w->model == CD_THREE_CIRCLE);
+}
+
+int check_ws(const cd_agents_t *a, const cd_workspace_t *w) {
+  if (!ws_sizes_ok(w) || !w->ptr) return CD_ERR_ARG;
+  if (w->n != a->n || w->model != a->model) return CD_ERR_WORKSPACE;
+  const Layout L = make_layout(w->n, w->cell_capacity, w->cell_slots, w->model);
+  if (w->bytes < L.total) return CD_ERR_WORKSPACE;
   return CD_OK;
 }
 
 constexpr int kBS = 256;
-int g_pair_block = 128;  // threads per block of the pair kernels (autotuned)
-int g_reduce_blocks = 512;
+int g_force_block = 256;  // threads per block of the force kernel (4 per agent)
+
+inline int part_blocks(int64_t n) { return std::max(1, std::min(grid_for(n, kBS), kMaxParts)); }
+// blocks of a kernel that combines the partials in every block
+constexpr int kMaxConsumerBlocks = 1024;
+inline int consumer_blocks(int64_t n) {
+  return std::max(1, std::min(grid_for(n, kBS), kMaxConsumerBlocks));
+}
 
 template <typename T>
-int launch_reduce(const Agents<T> &A, const Work &W, hipStream_t s) {
-  int blocks = grid_for(A.n, kBS);
-  if (blocks > g_reduce_blocks) blocks = g_reduce_blocks;
-  if (blocks < 1) blocks = 1;
-  hipLaunchKernelGGL(k_reduce<T>, dim3(blocks), dim3(kBS), 0, s, A, W.st, W.count,
-                     W.partials);
+int launch_reduce(const Agents<T> &A, const Work &W, bool clear_counts, hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce<T>, dim3(part_blocks(A.n)), dim3(kBS), 0, s, A, W,
+                     clear_counts ? 1 : 0);
   return check_launch();
 }
 
-// Optional per-kernel timing: when `ev` is not null, ev[k] is recorded
-// before kernel k and ev[k+1] after it (events on the launch stream).
 inline void mark(hipEvent_t *ev, int k, hipStream_t s) {
   if (ev) (void)hipEventRecord(ev[k], s);
 }
 
+// K_a + K_b (slot-list binning and gather).
 template <typename T>
-int launch_binning(const Agents<T> &A, const Work &W, double cell, hipStream_t s,
-                   hipEvent_t *ev = nullptr) {
-  const int n = A.n;
+int launch_ell(const Agents<T> &A, const Work &W, double cell, bool binning, hipStream_t s,
+               hipEvent_t *ev = nullptr) {
   mark(ev, 0, s);
-  hipLaunchKernelGGL(k_bin_count, dim3(grid_for(n, kBS)), dim3(kBS), 0, s,
-                     (const double2 *)A.pos, n, cell, W.st, W.count, W.cell_of, W.rank, W.cap);
+  hipLaunchKernelGGL(k_bin_insert, dim3(consumer_blocks(A.n)), dim3(kBS), 0, s,
+                     (const double2 *)A.pos, A.n, cell, W, binning ? 1 : 0);
   mark(ev, 1, s);
-  hipLaunchKernelGGL(k_bin_scan, dim3(W.ntiles), dim3(kScanThreads), 0, s, W.st,
-                     (const int *)W.count, W.offset, W.tiles);
+  if (binning)
+    hipLaunchKernelGGL(k_cell_gather<T>, dim3(grid_for(W.cap, kBS)), dim3(kBS), 0, s, A, W);
   mark(ev, 2, s);
-  hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(n, kBS)), dim3(kBS), 0, s, n,
-                     (const DevState *)W.st, (const int *)W.cell_of, (const int *)W.rank,
-                     (const int *)W.offset, W.pidx);
-  mark(ev, 3, s);
-  // one thread per cell of the capacity (the live cell count is only known
-  // on the device; threads past it return at once)
-  int64_t cells = W.cap;
-  hipLaunchKernelGGL(k_bin_sort_gather<T>, dim3(grid_for(cells, kBS)), dim3(kBS), 0, s, A,
-                     (const DevState *)W.st, (const int *)W.offset, W.pidx, W.scell, W.spos,
-                     (V2<T> *)W.svel, W.sr, W.sls, W.srs, W.sphi, W.srt, W.srs_r, W.srts);
-  mark(ev, 4, s);
   return check_launch();
 }
 
 template <typename T>
-int launch_forces(const cd_agents_t *a, const Work &W, DevState *st, const FusedArgs &fa,
-                  bool fused, hipStream_t s) {
-  Agents<T> A = typed<T>(a);
-  const Slots S = slots_of(W);
+int launch_forces(const cd_agents_t *a, const Work &W, const FusedArgs &fa, bool fused,
+                  hipStream_t s) {
+  const Agents<T> A = typed<T>(a);
   const int threads = 4 * (int)a->n;
-  const dim3 grid(grid_for(threads, g_pair_block)), block(g_pair_block);
+  const dim3 grid(std::max(1, grid_for(threads, g_force_block))), block(g_force_block);
   if (a->model == CD_THREE_CIRCLE) {
     if (fused)
-      hipLaunchKernelGGL((k_forces<T, 1, true>), grid, block, 0, s, A, st, fa, S);
+      hipLaunchKernelGGL((k_forces<T, 1, true>), grid, block, 0, s, A, W, fa);
     else
-      hipLaunchKernelGGL((k_forces<T, 1, false>), grid, block, 0, s, A, st, fa, S);
+      hipLaunchKernelGGL((k_forces<T, 1, false>), grid, block, 0, s, A, W, fa);
   } else {
     if (fused)
-      hipLaunchKernelGGL((k_forces<T, 0, true>), grid, block, 0, s, A, st, fa, S);
+      hipLaunchKernelGGL((k_forces<T, 0, true>), grid, block, 0, s, A, W, fa);
     else
-      hipLaunchKernelGGL((k_forces<T, 0, false>), grid, block, 0, s, A, st, fa, S);
+      hipLaunchKernelGGL((k_forces<T, 0, false>), grid, block, 0, s, A, W, fa);
   }
   return check_launch();
 }
 
 template <typename T>
-int do_agent_agent(const cd_agents_t *a, double cell, void *ws, int64_t cap, hipStream_t s) {
-  Agents<T> A = typed<T>(a);
-  Layout L = make_layout(a->n, cap, a->model);
-  Work W = carve(ws, L, cap);
-  int rc = launch_reduce(A, W, s);
-  if (rc) return rc;
-  rc = launch_binning(A, W, cell, s);
-  if (rc) return rc;
+int do_agent_agent(const cd_agents_t *a, double cell, const cd_workspace_t *w, hipStream_t s) {
+  const Agents<T> A = typed<T>(a);
+  const Work W = carve(w);
+  int rc;
+  if ((rc = launch_reduce(A, W, true, s))) return rc;
+  if ((rc = launch_ell(A, W, cell, true, s))) return rc;
   FusedArgs fa;
   memset(&fa, 0, sizeof(fa));
-  return launch_forces<T>(a, W, W.st, fa, false, s);
+  return launch_forces<T>(a, W, fa, false, s);
+}
+
+template <typename T>
+int do_add_to_cells(const cd_agents_t *a, double cell, const cd_workspace_t *w, hipStream_t s) {
+  const Agents<T> A = typed<T>(a);
+  const Work W = carve(w);
+  int rc;
+  if ((rc = launch_reduce(A, W, true, s))) return rc;
+  hipLaunchKernelGGL(k_bl_count, dim3(consumer_blocks(A.n)), dim3(kBS), 0, s,
+                     (const double2 *)A.pos, A.n, cell, W);
+  hipLaunchKernelGGL(k_bl_scan, dim3(W.ntiles), dim3(kScanThreads), 0, s, W);
+  hipLaunchKernelGGL(k_bl_scatter, dim3(grid_for(A.n, kBS)), dim3(kBS), 0, s, A.n, W);
+  hipLaunchKernelGGL(k_bl_sort, dim3(grid_for(W.cap, kBS)), dim3(kBS), 0, s, W);
+  return check_launch();
+}
+
+FusedArgs fused_args(const cd_step_params_t *p) {
+  FusedArgs fa;
+  fa.flags = p->flags;
+  fa.nav_mode = p->nav_mode;
+  fa.seed = p->seed;
+  fa.maps = p->nav_maps;
+  fa.nav_T = p->nav_targets;
+  fa.nav_rows = p->nav_rows;
+  fa.nav_cols = p->nav_cols;
+  fa.nav_minx = p->nav_minx;
+  fa.nav_miny = p->nav_miny;
+  fa.nav_step = p->nav_step;
+  fa.segs = (const double4 *)p->segments;
+  fa.W = (p->flags & CD_USE_OBSTACLE) ? (int)p->n_segments : 0;
+  return fa;
+}
+
+template <typename T>
+int step_impl(const cd_agents_t *a, const cd_step_params_t *p, const cd_workspace_t *w,
+              hipStream_t s, hipEvent_t *ev = nullptr) {
+  const Agents<T> A = typed<T>(a);
+  const Work W = carve(w);
+  const bool pairs = (p->flags & CD_USE_AGENT_AGENT) != 0;
+  int rc;
+  if ((rc = launch_ell(A, W, p->cell_size, pairs, s, ev))) return rc;
+  if ((rc = launch_forces<T>(a, W, fused_args(p), true, s))) return rc;
+  mark(ev, 3, s);
+  hipLaunchKernelGGL(k_integrate<T>, dim3(part_blocks(A.n)), dim3(kBS), 0, s, A, (int)a->model,
+                     W, p->dt_min, p->dt_max, (p->flags & CD_USE_RESET) ? 1 : 0);
+  mark(ev, 4, s);
+  return check_launch();
+}
+
+int check_step(const cd_agents_t *a, const cd_step_params_t *p, const cd_workspace_t *w) {
+  int rc = check_agents(a);
+  if (rc) return rc;
+  if (!p) return CD_ERR_ARG;
+  if ((p->flags & CD_USE_AGENT_AGENT) && !(p->cell_size > 0.0)) return CD_ERR_ARG;
+  if (!(p->dt_min >= 0.0) || !(p->dt_max >= 0.0)) return CD_ERR_ARG;
+  if ((p->flags & CD_USE_OBSTACLE) &&
+      (p->n_segments < 0 || p->n_segments > 0x7fffffff || (p->n_segments > 0 && !p->segments)))
+    return CD_ERR_ARG;
+  if ((p->flags & CD_USE_NAV) &&
+      (!p->nav_maps || p->nav_rows < 2 || p->nav_cols < 2 || p->nav_targets < 0 ||
+       !(p->nav_step > 0.0) || (p->nav_mode != 0 && p->nav_mode != 1)))
+    return CD_ERR_ARG;
+  return check_ws(a, w);
+}
+
+__global__ void k_clear_status(DevState *st) { st->status = 0; }
+__global__ void k_set_step(DevState *st, unsigned long long step) { st->step = step; }
+__global__ void k_philox(const uint32_t *in, uint32_t *out) {
+  uint32_t c[4] = {in[0], in[1], in[2], in[3]};
+  philox4x32(c, in[4], in[5]);
+  for (int k = 0; k < 4; ++k) out[k] = c[k];
 }
 
 }  // namespace
@@ -1294,9 +1410,9 @@ int do_agent_agent(const cd_agents_t *a, double cell, void *ws, int64_t cap, hip
 
 extern "C" {
 
-int64_t cd_workspace_bytes(int64_t n, int64_t cell_capacity, int32_t model, cd_layout_t *layout) {
-  if (n < 0 || cell_capacity <= 0) return -1;
-  Layout L = make_layout(n, cell_capacity, model);
+int64_t cd_workspace_bytes(const cd_workspace_t *w, cd_layout_t *layout) {
+  if (!ws_sizes_ok(w)) return -1;
+  const Layout L = make_layout(w->n, w->cell_capacity, w->cell_slots, w->model);
   if (layout) {
     layout->points_indices = L.pidx;
     layout->cells_count = L.count;
@@ -1307,22 +1423,21 @@ int64_t cd_workspace_bytes(int64_t n, int64_t cell_capacity, int32_t model, cd_l
   return L.total;
 }
 
-int cd_workspace_init(void *ws, int64_t ws_bytes, int64_t n, int64_t cap, int32_t model,
-                      cd_stream_t stream) {
-  if (!ws || n < 0 || cap <= 0) return CD_ERR_ARG;
-  Layout L = make_layout(n, cap, model);
-  if (ws_bytes < L.total) return CD_ERR_WORKSPACE;
-  Work W = carve(ws, L, cap);
-  hipLaunchKernelGGL(k_ws_init, dim3(grid_for(cap + 1, kBS)), dim3(kBS), 0, (hipStream_t)stream,
-                     W.st, W.count, (int)cap, W.tiles, W.ntiles);
+int cd_workspace_init(const cd_workspace_t *w, cd_stream_t stream) {
+  if (!ws_sizes_ok(w) || !w->ptr) return CD_ERR_ARG;
+  const Layout L = make_layout(w->n, w->cell_capacity, w->cell_slots, w->model);
+  if (w->bytes < L.total) return CD_ERR_WORKSPACE;
+  const Work W = carve(w);
+  hipLaunchKernelGGL(k_ws_init, dim3(std::min(grid_for(w->cell_capacity + 1, kBS), 4096)),
+                     dim3(kBS), 0, (hipStream_t)stream, W);
   return check_launch();
 }
 
-int cd_read_state(const void *ws, cd_state_t *out, cd_stream_t stream) {
-  if (!ws || !out) return CD_ERR_ARG;
+int cd_read_state(const cd_workspace_t *w, cd_state_t *out, cd_stream_t stream) {
+  if (!w || !w->ptr || !out) return CD_ERR_ARG;
   DevState h;
-  if (hipMemcpyAsync(&h, ws, sizeof(DevState), hipMemcpyDeviceToHost, (hipStream_t)stream) !=
-      hipSuccess)
+  if (hipMemcpyAsync(&h, w->ptr, sizeof(DevState), hipMemcpyDeviceToHost,
+                     (hipStream_t)stream) != hipSuccess)
     return CD_ERR_HIP;
   if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess) return CD_ERR_HIP;
   out->dt = h.dt;
@@ -1334,64 +1449,42 @@ int cd_read_state(const void *ws, cd_state_t *out, cd_stream_t stream) {
   out->status = h.status;
   out->step = (int64_t)h.step;
   double vm;
-  memcpy(&vm, &h.red[0][R_VMAX], 8);
+  memcpy(&vm, &h.red[R_VMAX], 8);
   out->v_max = vm;
-  unsigned long long b = h.red[0][R_V0MAX];
+  unsigned long long b = h.red[R_V0MAX];
   b = (b & 0x8000000000000000ull) ? (b & 0x7fffffffffffffffull) : ~b;
   memcpy(&vm, &b, 8);
   out->v0_max = vm;
   return CD_OK;
 }
 
-}  // extern "C"
-
-namespace {
-__global__ void k_clear_status(DevState *st) { st->status = 0; }
-__global__ void k_philox(const uint32_t *in, uint32_t *out) {
-  uint32_t c[4] = {in[0], in[1], in[2], in[3]};
-  philox4x32(c, in[4], in[5]);
-  for (int k = 0; k < 4; ++k) out[k] = c[k];
-}
-}  // namespace
-
-extern "C" {
-
-int cd_clear_status(void *ws, cd_stream_t stream) {
-  if (!ws) return CD_ERR_ARG;
-  hipLaunchKernelGGL(k_clear_status, dim3(1), dim3(1), 0, (hipStream_t)stream, (DevState *)ws);
+int cd_clear_status(const cd_workspace_t *w, cd_stream_t stream) {
+  if (!w || !w->ptr) return CD_ERR_ARG;
+  hipLaunchKernelGGL(k_clear_status, dim3(1), dim3(1), 0, (hipStream_t)stream,
+                     (DevState *)w->ptr);
   return check_launch();
 }
 
-int cd_add_to_cells(const cd_agents_t *a, double cell_size, void *ws, int64_t ws_bytes,
-                    int64_t cap, cd_stream_t stream) {
+int cd_add_to_cells(const cd_agents_t *a, double cell_size, const cd_workspace_t *w,
+                    cd_stream_t stream) {
   int rc = check_agents(a);
   if (rc) return rc;
   if (!(cell_size > 0.0)) return CD_ERR_ARG;
-  if ((rc = check_ws(a, ws, ws_bytes, cap))) return rc;
+  if ((rc = check_ws(a, w))) return rc;
   if (a->n == 0) return CD_OK;
-  hipStream_t s = (hipStream_t)stream;
-  Layout L = make_layout(a->n, cap, a->model);
-  Work W = carve(ws, L, cap);
-  if (a->real == CD_F64) {
-    Agents<double> A = typed<double>(a);
-    if ((rc = launch_reduce(A, W, s))) return rc;
-    return launch_binning(A, W, cell_size, s);
-  }
-  Agents<float> A = typed<float>(a);
-  if ((rc = launch_reduce(A, W, s))) return rc;
-  return launch_binning(A, W, cell_size, s);
+  if (a->real == CD_F64) return do_add_to_cells<double>(a, cell_size, w, (hipStream_t)stream);
+  return do_add_to_cells<float>(a, cell_size, w, (hipStream_t)stream);
 }
 
-int cd_agent_agent_block_list(const cd_agents_t *a, double cell_size, void *ws, int64_t ws_bytes,
-                              int64_t cap, cd_stream_t stream) {
+int cd_agent_agent_block_list(const cd_agents_t *a, double cell_size, const cd_workspace_t *w,
+                              cd_stream_t stream) {
   int rc = check_agents(a);
   if (rc) return rc;
   if (!(cell_size > 0.0)) return CD_ERR_ARG;
-  if ((rc = check_ws(a, ws, ws_bytes, cap))) return rc;
+  if ((rc = check_ws(a, w))) return rc;
   if (a->n == 0) return CD_OK;
-  if (a->real == CD_F64)
-    return do_agent_agent<double>(a, cell_size, ws, cap, (hipStream_t)stream);
-  return do_agent_agent<float>(a, cell_size, ws, cap, (hipStream_t)stream);
+  if (a->real == CD_F64) return do_agent_agent<double>(a, cell_size, w, (hipStream_t)stream);
+  return do_agent_agent<float>(a, cell_size, w, (hipStream_t)stream);
 }
 
 int cd_agent_obstacle(const cd_agents_t *a, const double *segments, int64_t W,
@@ -1473,31 +1566,25 @@ int cd_force_fluctuation(const cd_agents_t *a, uint64_t seed, uint64_t step, cd_
   return check_launch();
 }
 
-int cd_velocity_verlet_integrator(const cd_agents_t *a, double dt_min, double dt_max, void *ws,
-                                  int64_t ws_bytes, cd_stream_t stream) {
+int cd_velocity_verlet_integrator(const cd_agents_t *a, double dt_min, double dt_max,
+                                  const cd_workspace_t *w, cd_stream_t stream) {
   int rc = check_agents(a);
   if (rc) return rc;
-  const int64_t need = align_up(4096) + align_up(partials_bytes(a->n));
-  if (!ws || ws_bytes < need) return CD_ERR_WORKSPACE;
+  if ((rc = check_ws(a, w))) return rc;
   if (!(dt_min >= 0.0) || !(dt_max >= 0.0)) return CD_ERR_ARG;
   if (a->n == 0) return CD_OK;
   hipStream_t s = (hipStream_t)stream;
-  DevState *st = (DevState *)ws;
-  unsigned long long *partials = (unsigned long long *)((char *)ws + align_up(4096));
-  const int rblocks = std::min(grid_for(a->n, kBS), g_reduce_blocks);
+  const Work W = carve(w);
   if (a->real == CD_F64) {
-    Agents<double> A = typed<double>(a);
-    // count clearing is skipped (null counts): the integrator owns no binning
-    hipLaunchKernelGGL(k_reduce<double>, dim3(rblocks), dim3(kBS), 0, s, A, st, (int *)nullptr,
-                       partials);
-    hipLaunchKernelGGL(k_integrate<double>, dim3(grid_for(A.n, kBS)), dim3(kBS), 0, s, A,
-                       (int)a->model, st, dt_min, dt_max, 0, 0, (int *)nullptr, partials);
+    const Agents<double> A = typed<double>(a);
+    if ((rc = launch_reduce(A, W, false, s))) return rc;
+    hipLaunchKernelGGL(k_integrate_node<double>, dim3(consumer_blocks(A.n)), dim3(kBS), 0, s,
+                       A, (int)a->model, W, dt_min, dt_max);
   } else {
-    Agents<float> A = typed<float>(a);
-    hipLaunchKernelGGL(k_reduce<float>, dim3(rblocks), dim3(kBS), 0, s, A, st, (int *)nullptr,
-                       partials);
-    hipLaunchKernelGGL(k_integrate<float>, dim3(grid_for(A.n, kBS)), dim3(kBS), 0, s, A,
-                       (int)a->model, st, dt_min, dt_max, 0, 0, (int *)nullptr, partials);
+    const Agents<float> A = typed<float>(a);
+    if ((rc = launch_reduce(A, W, false, s))) return rc;
+    hipLaunchKernelGGL(k_integrate_node<float>, dim3(consumer_blocks(A.n)), dim3(kBS), 0, s,
+                       A, (int)a->model, W, dt_min, dt_max);
   }
   return check_launch();
 }
@@ -1515,115 +1602,53 @@ int cd_reset(const cd_agents_t *a, cd_stream_t stream) {
   return check_launch();
 }
 
-}  // extern "C"
-
-namespace {
-
-int check_step(const cd_agents_t *a, const cd_step_params_t *p, void *ws, int64_t ws_bytes) {
-  int rc = check_agents(a);
-  if (rc) return rc;
-  if (!p) return CD_ERR_ARG;
-  if ((p->flags & CD_USE_AGENT_AGENT) && !(p->cell_size > 0.0)) return CD_ERR_ARG;
-  if ((p->flags & CD_USE_OBSTACLE) && (p->n_segments < 0 || (p->n_segments > 0 && !p->segments)))
-    return CD_ERR_ARG;
-  if ((p->flags & CD_USE_NAV) &&
-      (!p->nav_maps || p->nav_rows < 2 || p->nav_cols < 2 || !(p->nav_step > 0.0) ||
-       (p->nav_mode != 0 && p->nav_mode != 1)))
-    return CD_ERR_ARG;
-  return check_ws(a, ws, ws_bytes, p->cell_capacity);
-}
-
-template <typename T>
-int step_impl(const cd_agents_t *a, const cd_step_params_t *p, void *ws, hipStream_t s,
-              hipEvent_t *ev = nullptr) {
-  Agents<T> A = typed<T>(a);
-  Layout L = make_layout(a->n, p->cell_capacity, a->model);
-  Work W = carve(ws, L, p->cell_capacity);
-  int rc;
-  const bool pairs = (p->flags & CD_USE_AGENT_AGENT) != 0;
-  if (pairs) {
-    if ((rc = launch_binning(A, W, p->cell_size, s, ev))) return rc;
-  } else {
-    for (int k = 0; k < 5; ++k) mark(ev, k, s);
-  }
-  FusedArgs fa;
-  fa.flags = p->flags;
-  if (!pairs) fa.flags &= ~CD_USE_AGENT_AGENT;
-  fa.nav_mode = p->nav_mode;
-  fa.seed = p->seed;
-  fa.maps = p->nav_maps;
-  fa.nav_T = p->nav_targets;
-  fa.nav_rows = p->nav_rows;
-  fa.nav_cols = p->nav_cols;
-  fa.nav_minx = p->nav_minx;
-  fa.nav_miny = p->nav_miny;
-  fa.nav_step = p->nav_step;
-  fa.segs = (const double4 *)p->segments;
-  fa.W = (p->flags & CD_USE_OBSTACLE) ? (int)p->n_segments : 0;
-  if ((rc = launch_forces<T>(a, W, W.st, fa, true, s))) return rc;
-  mark(ev, 5, s);
-  if ((rc = check_launch())) return rc;
-  hipLaunchKernelGGL(k_integrate<T>, dim3(grid_for(A.n, kBS)), dim3(kBS), 0, s, A,
-                     (int)a->model, W.st, p->dt_min, p->dt_max, 1,
-                     (p->flags & CD_USE_RESET) ? 1 : 0, W.count, W.partials);
-  mark(ev, 6, s);
-  return check_launch();
-}
-
-__global__ void k_set_step(DevState *st, unsigned long long step) { st->step = step; }
-
-}  // namespace
-
-extern "C" {
-
-int cd_step_prepare(const cd_agents_t *a, const cd_step_params_t *p, void *ws, int64_t ws_bytes,
+int cd_step_prepare(const cd_agents_t *a, const cd_step_params_t *p, const cd_workspace_t *w,
                     uint64_t first_step, cd_stream_t stream) {
-  int rc = check_step(a, p, ws, ws_bytes);
+  int rc = check_step(a, p, w);
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
-  Layout L = make_layout(a->n, p->cell_capacity, a->model);
-  Work W = carve(ws, L, p->cell_capacity);
+  const Work W = carve(w);
   hipLaunchKernelGGL(k_set_step, dim3(1), dim3(1), 0, s, W.st, (unsigned long long)first_step);
   if ((rc = check_launch())) return rc;
   if (a->n == 0) return CD_OK;
-  if (a->real == CD_F64) return launch_reduce(typed<double>(a), W, s);
-  return launch_reduce(typed<float>(a), W, s);
+  if (a->real == CD_F64) return launch_reduce(typed<double>(a), W, true, s);
+  return launch_reduce(typed<float>(a), W, true, s);
 }
 
-int cd_step(const cd_agents_t *a, const cd_step_params_t *p, void *ws, int64_t ws_bytes,
+int cd_step(const cd_agents_t *a, const cd_step_params_t *p, const cd_workspace_t *w,
             cd_stream_t stream) {
-  int rc = check_step(a, p, ws, ws_bytes);
+  int rc = check_step(a, p, w);
   if (rc) return rc;
   if (a->n == 0) return CD_OK;
-  if (a->real == CD_F64) return step_impl<double>(a, p, ws, (hipStream_t)stream);
-  return step_impl<float>(a, p, ws, (hipStream_t)stream);
+  if (a->real == CD_F64) return step_impl<double>(a, p, w, (hipStream_t)stream);
+  return step_impl<float>(a, p, w, (hipStream_t)stream);
 }
 
-int cd_step_timed(const cd_agents_t *a, const cd_step_params_t *p, void *ws, int64_t ws_bytes,
+int cd_step_timed(const cd_agents_t *a, const cd_step_params_t *p, const cd_workspace_t *w,
                   int64_t steps, double *kernel_ms, cd_stream_t stream) {
-  int rc = check_step(a, p, ws, ws_bytes);
+  int rc = check_step(a, p, w);
   if (rc) return rc;
   if (!kernel_ms || steps < 0) return CD_ERR_ARG;
-  for (int k = 0; k < 6; ++k) kernel_ms[k] = 0.0;
+  for (int k = 0; k < 4; ++k) kernel_ms[k] = 0.0;
   if (a->n == 0 || steps == 0) return CD_OK;
   hipStream_t s = (hipStream_t)stream;
-  hipEvent_t ev[7];
-  for (int k = 0; k < 7; ++k)
+  hipEvent_t ev[5];
+  for (int k = 0; k < 5; ++k)
     if (hipEventCreate(&ev[k]) != hipSuccess) return CD_ERR_HIP;
   for (int64_t it = 0; it < steps && rc == CD_OK; ++it) {
-    rc = a->real == CD_F64 ? step_impl<double>(a, p, ws, s, ev) : step_impl<float>(a, p, ws, s, ev);
+    rc = a->real == CD_F64 ? step_impl<double>(a, p, w, s, ev) : step_impl<float>(a, p, w, s, ev);
     if (rc) break;
-    if (hipEventSynchronize(ev[6]) != hipSuccess) {
+    if (hipEventSynchronize(ev[4]) != hipSuccess) {
       rc = CD_ERR_HIP;
       break;
     }
-    for (int k = 0; k < 6; ++k) {
+    for (int k = 0; k < 4; ++k) {
       float ms = 0.f;
       if (hipEventElapsedTime(&ms, ev[k], ev[k + 1]) != hipSuccess) rc = CD_ERR_HIP;
       kernel_ms[k] += ms;
     }
   }
-  for (int k = 0; k < 7; ++k) (void)hipEventDestroy(ev[k]);
+  for (int k = 0; k < 5; ++k) (void)hipEventDestroy(ev[k]);
   return rc;
 }
 
@@ -1638,7 +1663,7 @@ int cd_graph_end(cd_stream_t stream, void **graph_exec) {
   hipGraph_t g = nullptr;
   if (hipStreamEndCapture((hipStream_t)stream, &g) != hipSuccess || !g) return CD_ERR_HIP;
   hipGraphExec_t e = nullptr;
-  hipError_t err = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+  const hipError_t err = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (err != hipSuccess) return CD_ERR_HIP;
   *graph_exec = (void *)e;
@@ -1663,6 +1688,6 @@ int cd_philox4x32(const uint32_t *in, uint32_t *out, cd_stream_t stream) {
   return check_launch();
 }
 
-const char *cd_version(void) { return "cdamd 0.1 gfx950"; }
+const char *cd_version(void) { return "cdamd 0.2 gfx950"; }
 
 }  // extern "C"

@@ -100,52 +100,77 @@ def cell_capacity_for(extent_x, extent_y, cell_size, margin=0.5):
     return max(nx * ny, 16)
 
 
-class Workspace:
-    """Caller-owned workspace of the C ABI (block lists, sorted copies, state)."""
+def cell_slots_for(cell_size, r_min=0.2, slack=1.25):
+    """Agent slots per cell: the most non-overlapping discs of radius r_min
+    whose centres fit in a cell (hexagonal packing density over the cell
+    grown by r_min on each side), with slack for compressed crowds.  Cells
+    holding more take the exact slow path of the force kernel."""
+    r = max(float(r_min), 0.05)
+    k = slack * (cell_size + 2 * r) ** 2 * 0.9069 / (np.pi * r * r)
+    return int(min(max(4 * int(np.ceil(k / 4.0)), 4), 1024))
 
-    def __init__(self, n, cell_capacity, model, device=None, stream=None):
+
+def min_body_radius(array):
+    """Smallest radius bounding how close two agents' centres get."""
+    if 'position_ls' in array.dtype.names:
+        return float(np.min(array['r_t'])) if len(array) else 0.2
+    return float(np.min(array['radius'])) if len(array) else 0.2
+
+
+class Workspace:
+    """Caller-owned workspace of the C ABI (slot lists, block lists, state)."""
+
+    def __init__(self, n, cell_capacity, model, cell_slots=16, device=None, stream=None):
         require_gpu()
         L = _lib.lib()
-        self.n = int(n)
-        self.cell_capacity = int(cell_capacity)
-        self.model = int(model)
+        self.desc = _lib.CDWorkspace()
+        d = self.desc
+        d.n = int(n)
+        d.cell_capacity = int(cell_capacity)
+        d.cell_slots = int(cell_slots)
+        d.model = int(model)
         self.layout = _lib.CDLayout()
-        nbytes = L.cd_workspace_bytes(self.n, self.cell_capacity, self.model,
-                                      C.byref(self.layout))
+        nbytes = L.cd_workspace_bytes(C.byref(d), C.byref(self.layout))
         if nbytes <= 0:
-            raise InvalidValue('invalid workspace size request')
-        self.nbytes = int(nbytes)
+            raise InvalidValue('invalid workspace sizes (n=%d, cells=%d, slots=%d)'
+                               % (n, cell_capacity, cell_slots))
         dev = torch.device(device) if device is not None else torch.device('cuda')
-        self.buf = torch.empty(self.nbytes, dtype=torch.uint8, device=dev)
-        _lib.check(L.cd_workspace_init(self.ptr, self.nbytes, self.n, self.cell_capacity,
-                                       self.model, stream_handle(stream)), 'workspace_init')
+        self.buf = torch.empty(int(nbytes), dtype=torch.uint8, device=dev)
+        d.ptr = self.buf.data_ptr()
+        d.bytes = int(nbytes)
+        _lib.check(L.cd_workspace_init(C.byref(d), stream_handle(stream)), 'workspace_init')
 
-    @property
-    def ptr(self):
-        return C.c_void_p(self.buf.data_ptr())
+    n = property(lambda self: self.desc.n)
+    cell_capacity = property(lambda self: self.desc.cell_capacity)
+    cell_slots = property(lambda self: self.desc.cell_slots)
+    nbytes = property(lambda self: self.desc.bytes)
+
+    def ref(self):
+        return C.byref(self.desc)
 
     def state(self, stream=None):
         """Synchronous read of the device scalars."""
         st = _lib.CDState()
-        _lib.check(_lib.lib().cd_read_state(self.ptr, C.byref(st), stream_handle(stream)),
+        _lib.check(_lib.lib().cd_read_state(self.ref(), C.byref(st), stream_handle(stream)),
                    'read_state')
         return st
 
     def clear_status(self, stream=None):
-        _lib.check(_lib.lib().cd_clear_status(self.ptr, stream_handle(stream)), 'clear_status')
+        _lib.check(_lib.lib().cd_clear_status(self.ref(), stream_handle(stream)), 'clear_status')
 
     def _i32(self, off, count):
         return self.buf[off:off + 4 * count].view(torch.int32)
 
     def block_list(self, stream=None):
         """(points_indices, cells_count, cells_offset, grid_shape, origin, cell_of)
-        of the last binning, as numpy int64 (synchronous)."""
+        of the last cd_add_to_cells, as numpy int64 (synchronous)."""
         st = self.state(stream)
         nx, ny = int(st.shape[0]), int(st.shape[1])
         cells = nx * ny
         lay = self.layout
-        pidx = self._i32(lay.points_indices, self.n).cpu().numpy().astype(np.int64)
+        n = self.n
+        pidx = self._i32(lay.points_indices, n).cpu().numpy().astype(np.int64)
         cnt = self._i32(lay.cells_count, cells).cpu().numpy().astype(np.int64)
         off = self._i32(lay.cells_offset, cells).cpu().numpy().astype(np.int64)
-        cell_of = self._i32(lay.cell_of, self.n).cpu().numpy().astype(np.int64)
+        cell_of = self._i32(lay.cell_of, n).cpu().numpy().astype(np.int64)
         return pidx, cnt, off, (nx, ny), np.array(st.origin[:]), cell_of

@@ -16,7 +16,7 @@ import numpy as np
 
 from . import core
 from .agents import is_model
-from .device import Workspace, cell_capacity_for
+from .device import Workspace, cell_capacity_for, cell_slots_for, min_body_radius
 from .exceptions import InvalidType
 from .simulation import LogicNodeBase
 
@@ -61,7 +61,7 @@ class Integrator(LogicNode):
     def update(self):
         d = self.device_agents
         if self.simulation._integrator_ws is None:
-            self.simulation._integrator_ws = Workspace(d.n, 16, d.model)
+            self.simulation._integrator_ws = Workspace(d.n, 16, d.model, cell_slots=1)
             ws = self.simulation._integrator_ws
             data = self.simulation.data
             data._time_offset = float(dict.__getitem__(data, 'time_tot'))
@@ -113,7 +113,9 @@ class AgentAgentInteractions(LogicNode):
                 p = self.simulation.agents._array['position']
                 span = p.max(0) - p.min(0) if len(p) else np.ones(2)
                 cap = cell_capacity_for(span[0], span[1], self.cell_size)
-            self._ws = Workspace(d.n, cap, d.model)
+            arr = self.simulation.agents._array
+            self._ws = Workspace(d.n, cap, d.model,
+                                 cell_slots=cell_slots_for(self.cell_size, min_body_radius(arr)))
         core.agent_agent_block_list(d, self.cell_size, self._ws)
 
 

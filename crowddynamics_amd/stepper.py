@@ -19,7 +19,8 @@ import torch
 
 from . import _lib
 from .core import segments_to_device
-from .device import DeviceAgents, Workspace, cell_capacity_for, stream_handle
+from .device import (DeviceAgents, Workspace, cell_capacity_for, cell_slots_for,
+                     min_body_radius, stream_handle)
 from .exceptions import CrowdDynamicsException, InvalidValue
 
 ALL_NODES = (_lib.CD_USE_FLUCT | _lib.CD_USE_NAV | _lib.CD_USE_ADJUST |
@@ -33,7 +34,8 @@ class GridOverflow(CrowdDynamicsException):
 class Stepper:
     def __init__(self, array, segments=None, nav=None, cell_size=3.6, dt_min=0.01,
                  dt_max=0.01, seed=1, flags=ALL_NODES, precision='f64', cell_capacity=None,
-                 bounds=None, first_step=0, device=None, device_agents=None):
+                 bounds=None, first_step=0, device=None, device_agents=None,
+                 cell_slots=None):
         if device_agents is not None:
             self.agents = device_agents
         else:
@@ -60,7 +62,10 @@ class Stepper:
                 x0, y0 = p.min(0) if len(p) else (0.0, 0.0)
                 x1, y1 = p.max(0) if len(p) else (1.0, 1.0)
             cell_capacity = cell_capacity_for(x1 - x0, y1 - y0, cell_size)
-        self.ws = Workspace(self.n, cell_capacity, self.agents.model, device=dev)
+        if cell_slots is None:
+            cell_slots = cell_slots_for(cell_size, min_body_radius(array))
+        self.ws = Workspace(self.n, cell_capacity, self.agents.model, cell_slots=cell_slots,
+                            device=dev)
         p = _lib.CDStepParams()
         p.cell_size = float(cell_size)
         p.dt_min = float(dt_min)
@@ -76,7 +81,6 @@ class Stepper:
             p.nav_minx = float(nav['minx'])
             p.nav_miny = float(nav['miny'])
             p.nav_step = float(nav['step'])
-        p.cell_capacity = self.ws.cell_capacity
         self.params = p
         self.flags = flags
         self.prepare(first_step)
@@ -85,15 +89,15 @@ class Stepper:
         """Seed the device reductions from the current state (also after any
         host-side change of positions or velocities)."""
         _lib.check(_lib.lib().cd_step_prepare(self.agents.ref(), C.byref(self.params),
-                                              self.ws.ptr, self.ws.nbytes, int(first_step),
-                                              stream_handle()), 'step_prepare')
+                                              self.ws.ref(), int(first_step), stream_handle()),
+                   'step_prepare')
 
     def step(self, k=1):
         L = _lib.lib()
         s = stream_handle()
         for _ in range(k):
-            _lib.check(L.cd_step(self.agents.ref(), C.byref(self.params), self.ws.ptr,
-                                 self.ws.nbytes, s), 'step')
+            _lib.check(L.cd_step(self.agents.ref(), C.byref(self.params), self.ws.ref(), s),
+                       'step')
 
     def capture(self, k):
         """HIP graph of k consecutive steps (launched on the current stream)."""
@@ -117,11 +121,24 @@ class Stepper:
     def state(self):
         return self.ws.state()
 
+    def timed(self, steps):
+        """Per-kernel GPU milliseconds (bin/reduce, gather, forces, integrate)
+        summed over `steps` steps, from HIP events on the current stream."""
+        ms = (C.c_double * 4)()
+        _lib.check(_lib.lib().cd_step_timed(self.agents.ref(), C.byref(self.params),
+                                            self.ws.ref(), int(steps), ms, stream_handle()),
+                   'step_timed')
+        return list(ms)
+
     def check(self):
         """Raise if the device recorded a grid overflow or non-finite positions."""
         st = self.state()
         if st.status & _lib.CD_STATUS_NONFINITE:
             raise CrowdDynamicsException('non-finite agent position')
+        if st.status & _lib.CD_STATUS_SLOTS_FULL:
+            import warnings
+            warnings.warn('a cell held more agents than its %d slots; the exact slow path '
+                          'was used' % self.ws.cell_slots)
         if st.status & _lib.CD_STATUS_GRID_OVERFLOW:
             raise GridOverflow('agents spread beyond the cell capacity (%d cells)'
                                % self.ws.cell_capacity)

@@ -1,14 +1,14 @@
 /*
  * cdamd.h -- C ABI of the MI355X (gfx950) per-timestep agent update.
  *
- * All pointers in cd_agents_t and every `ws` workspace are DEVICE pointers
+ * All pointers in cd_agents_t and the workspace memory are DEVICE pointers
  * owned by the caller.  Every call is stream-ordered on `stream` (a
  * hipStream_t), performs no host synchronisation and no allocation, and may
  * be captured into a HIP graph, except the calls documented as synchronous
- * (cd_read_state, cd_graph_*).  Calls return 0 on success or a negative
- * CD_ERR_* code; kernels record data-dependent failures (grid overflow,
- * non-finite positions) as sticky bits in the workspace state, read back by
- * cd_read_state.
+ * (cd_read_state, cd_step_timed, cd_graph_*).  Calls return 0 on success or
+ * a negative CD_ERR_* code; kernels record data-dependent conditions (grid
+ * overflow, non-finite positions, full cell slot lists) as sticky bits in
+ * the workspace state, read back by cd_read_state.
  *
  * Each entry point names the reference interface it replaces (paths relative
  * to the reference repository root).
@@ -29,8 +29,8 @@ enum {
   CD_OK = 0,
   CD_ERR_ARG = -1,         /* null pointer, bad size or bad parameter */
   CD_ERR_TYPE = -2,        /* not a circular / three-circle agent array (InvalidType) */
-  CD_ERR_WORKSPACE = -3,   /* workspace smaller than cd_workspace_bytes() */
-  CD_ERR_UNSUPPORTED = -4, /* combination not built (e.g. three-circle in fp32) */
+  CD_ERR_WORKSPACE = -3,   /* workspace smaller than cd_workspace_bytes() or for other sizes */
+  CD_ERR_UNSUPPORTED = -4, /* combination not built (three-circle in fp32) */
   CD_ERR_HIP = -5          /* a HIP runtime call or launch failed */
 };
 
@@ -39,8 +39,9 @@ enum { CD_F64 = 0, CD_F32 = 1 };
 
 /* Sticky status bits kept in the workspace state. */
 enum {
-  CD_STATUS_GRID_OVERFLOW = 1, /* binning grid larger than cell_capacity: pairs skipped */
-  CD_STATUS_NONFINITE = 2      /* a non-finite position reached the binning */
+  CD_STATUS_GRID_OVERFLOW = 1, /* grid larger than cell_capacity (or stale box): pairs skipped */
+  CD_STATUS_NONFINITE = 2,     /* a non-finite position reached the binning */
+  CD_STATUS_SLOTS_FULL = 4     /* a cell held more than cell_slots agents: slow exact path used */
 };
 
 /* Device structure-of-arrays view of agent_type_circular /
@@ -65,6 +66,19 @@ typedef struct cd_agents {
   const int64_t *target;
 } cd_agents_t;
 
+/* Caller-owned device workspace and the sizes it was laid out for:
+ * n agents, a binning grid of at most cell_capacity cells, cell_slots agent
+ * slots per cell (cells holding more take an exact slow path). */
+typedef struct cd_workspace {
+  void *ptr;
+  int64_t bytes;
+  int64_t n;
+  int64_t cell_capacity;
+  int64_t cell_slots;
+  int32_t model;
+  int32_t _pad;
+} cd_workspace_t;
+
 /* Host copy of the workspace state scalars. */
 typedef struct cd_state {
   double dt;          /* dt of the last integration (simulation.data['dt']) */
@@ -73,12 +87,12 @@ typedef struct cd_state {
   int64_t shape[2];   /* binning grid shape (nx, ny) of the last binning */
   int64_t status;     /* CD_STATUS_* bits */
   int64_t step;       /* steps taken by cd_step */
-  double v_max;       /* max |v| the last integration used */
-  double v0_max;      /* max target_velocity the last integration used */
+  double v_max;       /* max |v| of the last reduction */
+  double v0_max;      /* max target_velocity of the last reduction */
 } cd_state_t;
 
-/* Offsets (bytes from the workspace base) of the arrays a caller may read.
- * All int32. */
+/* Offsets (bytes from the workspace base) of the block-list arrays that
+ * cd_add_to_cells fills.  All int32. */
 typedef struct cd_layout {
   int64_t points_indices; /* [n]: agents sorted stably by flat cell */
   int64_t cells_count;    /* [cell_capacity] */
@@ -89,17 +103,15 @@ typedef struct cd_layout {
 
 /* ---- workspace --------------------------------------------------------- */
 
-/* Bytes of workspace for n agents, a grid of at most cell_capacity cells and
- * the given model.  Fills *layout when not NULL. */
-int64_t cd_workspace_bytes(int64_t n, int64_t cell_capacity, int32_t model,
-                           cd_layout_t *layout);
+/* Bytes of workspace for the sizes in *ws (ptr and bytes are ignored).
+ * Fills *layout when not NULL.  Negative for invalid sizes. */
+int64_t cd_workspace_bytes(const cd_workspace_t *ws, cd_layout_t *layout);
 /* Zeroes the workspace state and counters (once, before first use). */
-int cd_workspace_init(void *ws, int64_t ws_bytes, int64_t n, int64_t cell_capacity,
-                      int32_t model, cd_stream_t stream);
+int cd_workspace_init(const cd_workspace_t *ws, cd_stream_t stream);
 /* Synchronous: copies the state scalars to *out. */
-int cd_read_state(const void *ws, cd_state_t *out, cd_stream_t stream);
+int cd_read_state(const cd_workspace_t *ws, cd_state_t *out, cd_stream_t stream);
 /* Clears the sticky status bits (stream-ordered). */
-int cd_clear_status(void *ws, cd_stream_t stream);
+int cd_clear_status(const cd_workspace_t *ws, cd_stream_t stream);
 
 /* ---- block-list binning -------------------------------------------------- */
 
@@ -108,16 +120,16 @@ int cd_clear_status(void *ws, cd_stream_t stream);
  * into square cells of side cell_size anchored at the positions' minimum.
  * Results in the workspace arrays named by cd_layout_t and the state's
  * origin/shape.  Deterministic: points_indices is the stable order. */
-int cd_add_to_cells(const cd_agents_t *agents, double cell_size, void *ws,
-                    int64_t ws_bytes, int64_t cell_capacity, cd_stream_t stream);
+int cd_add_to_cells(const cd_agents_t *agents, double cell_size, const cd_workspace_t *ws,
+                    cd_stream_t stream);
 
 /* ---- LogicNode operators (crowddynamics/simulation/logic.py) ------------- */
 
 /* agent_agent_block_list(agents, cell_size) (crowddynamics/core/interactions.py:191-205):
  * bins, then adds every adjacent pair's social + contact force (and torque)
  * to force (torque). */
-int cd_agent_agent_block_list(const cd_agents_t *agents, double cell_size, void *ws,
-                              int64_t ws_bytes, int64_t cell_capacity, cd_stream_t stream);
+int cd_agent_agent_block_list(const cd_agents_t *agents, double cell_size,
+                              const cd_workspace_t *ws, cd_stream_t stream);
 
 /* agent_obstacle(agents, obstacles) (crowddynamics/core/interactions.py:208-214):
  * segments is obstacle_type_linear[W] = {p0: f8[2], p1: f8[2]}, [W][4] fp64. */
@@ -154,7 +166,7 @@ int cd_force_fluctuation(const cd_agents_t *agents, uint64_t seed, uint64_t step
  * device, then the reference's translational (and rotational + shoulders)
  * update.  dt and time_tot are kept in the workspace state. */
 int cd_velocity_verlet_integrator(const cd_agents_t *agents, double dt_min, double dt_max,
-                                  void *ws, int64_t ws_bytes, cd_stream_t stream);
+                                  const cd_workspace_t *ws, cd_stream_t stream);
 
 /* Reset.update (crowddynamics/simulation/logic.py:59-64) */
 int cd_reset(const cd_agents_t *agents, cd_stream_t stream);
@@ -180,27 +192,27 @@ typedef struct cd_step_params {
   const double *nav_maps;
   int64_t nav_targets, nav_rows, nav_cols;
   double nav_minx, nav_miny, nav_step;
-  int64_t cell_capacity;
 } cd_step_params_t;
 
 /* One MultiAgentSimulation.update() (crowddynamics/simulation/multiagent.py:51-55)
  * of the tree Reset << (Integrator << (Fluctuation, Adjusting << (Navigation,
  * Orientation), AgentAgentInteractions, AgentObstacleInteractions))
- * (crowddynamics/examples/simulations.py:123-136) in six kernels.  The
- * fluctuation step counter lives on the device.  cd_step_prepare must run
- * once before the first cd_step (and after any host-side change of
- * positions or velocities). */
-int cd_step_prepare(const cd_agents_t *agents, const cd_step_params_t *params, void *ws,
-                    int64_t ws_bytes, uint64_t first_step, cd_stream_t stream);
-int cd_step(const cd_agents_t *agents, const cd_step_params_t *params, void *ws,
-            int64_t ws_bytes, cd_stream_t stream);
+ * (crowddynamics/examples/simulations.py:123-136) in four kernels.  The
+ * fluctuation step counter and every reduction stay on the device.
+ * cd_step_prepare must run once before the first cd_step (and after any
+ * host-side change of positions or velocities). */
+int cd_step_prepare(const cd_agents_t *agents, const cd_step_params_t *params,
+                    const cd_workspace_t *ws, uint64_t first_step, cd_stream_t stream);
+int cd_step(const cd_agents_t *agents, const cd_step_params_t *params, const cd_workspace_t *ws,
+            cd_stream_t stream);
 
-/* Synchronous: runs `steps` cd_step calls and returns in kernel_ms[6] the
- * summed GPU time of each of the six step kernels (count, scan, scatter,
- * sort+gather, forces, integrate), measured with HIP events recorded on
- * `stream` around each launch.  No reference counterpart (measurement). */
-int cd_step_timed(const cd_agents_t *agents, const cd_step_params_t *params, void *ws,
-                  int64_t ws_bytes, int64_t steps, double *kernel_ms, cd_stream_t stream);
+/* Synchronous: runs `steps` cd_step calls and returns in kernel_ms[4] the
+ * summed GPU time of each step kernel (bin/reduce, sort+gather, forces,
+ * integrate), measured with HIP events recorded on `stream` around each
+ * launch.  No reference counterpart (measurement). */
+int cd_step_timed(const cd_agents_t *agents, const cd_step_params_t *params,
+                  const cd_workspace_t *ws, int64_t steps, double *kernel_ms,
+                  cd_stream_t stream);
 
 /* ---- HIP graphs (synchronous helpers) ------------------------------------- */
 
@@ -211,8 +223,9 @@ int cd_graph_destroy(void *graph_exec);
 
 /* ---- misc ----------------------------------------------------------------- */
 
-/* Philox4x32-10 on the device for one counter (testing the noise stream). */
-int cd_philox4x32(const uint32_t *ctr_key_dev, uint32_t *out_dev, cd_stream_t stream);
+/* Philox4x32-10 on the device for one counter (testing the noise stream):
+ * in = {ctr0..3, key0, key1}, out = 4 words. */
+int cd_philox4x32(const uint32_t *in_dev, uint32_t *out_dev, cd_stream_t stream);
 const char *cd_version(void);
 
 #ifdef __cplusplus

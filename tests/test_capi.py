@@ -30,11 +30,25 @@ def test_bindings_cover_header():
 def test_workspace_sizing_is_host_only():
     L = _lib.lib()
     lay = _lib.CDLayout()
-    n = L.cd_workspace_bytes(20000, 60000, 0, ctypes.byref(lay))
+    w = _lib.CDWorkspace()
+    w.n, w.cell_capacity, w.cell_slots, w.model = 20000, 60000, 12, 0
+    n = L.cd_workspace_bytes(ctypes.byref(w), ctypes.byref(lay))
     assert n == lay.total_bytes > 0
     assert lay.points_indices % 256 == 0 and lay.cells_offset % 256 == 0
-    assert L.cd_workspace_bytes(-1, 10, 0, None) < 0
+    w.model = 1
+    assert L.cd_workspace_bytes(ctypes.byref(w), None) > n      # three-circle slot data
+    w.n = -1
+    assert L.cd_workspace_bytes(ctypes.byref(w), None) < 0
+    w.n, w.cell_slots = 10, 0
+    assert L.cd_workspace_bytes(ctypes.byref(w), None) < 0
     assert L.cd_version().startswith(b'cdamd')
+
+
+def test_cell_slot_sizing():
+    from crowddynamics_amd.device import cell_slots_for
+    assert cell_slots_for(0.6, 0.22) >= 8
+    assert cell_slots_for(3.6, 0.22) >= 100
+    assert cell_slots_for(3.6, 0.22) % 4 == 0
 
 
 def test_argument_validation_without_device():

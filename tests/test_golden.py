@@ -50,7 +50,7 @@ def test_gpu_matches_golden():
     a = g['initial'].copy()
     a['force'] = 0.0
     dev = DeviceAgents(a)
-    ws = Workspace(len(a), 4096, 0)
+    ws = Workspace(len(a), 4096, 0, cell_slots=128)
     core.agent_agent_block_list(dev, float(g['cell_size']), ws)
     out = a.copy()
     dev.download(out)
@@ -68,7 +68,7 @@ def test_gpu_matches_golden():
     a['force'] = 0.0
     a['torque'] = 0.0
     dev = DeviceAgents(a)
-    ws = Workspace(len(a), 4096, 1)
+    ws = Workspace(len(a), 4096, 1, cell_slots=32)
     core.agent_agent_block_list(dev, float(t['cell_size']), ws)
     out = a.copy()
     dev.download(out)
