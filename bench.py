@@ -92,6 +92,103 @@ def cpu_baseline(scn, seconds, threads=None):
     return scn.n * steps / el, steps, el
 
 
+def bench_ensemble(n_scenarios, steps, loop_steps, stream):
+    """Batched ensemble of uneven rooms (fp32, fp64 positions): autotuned
+    batched steps replayed from a HIP graph versus a per-scenario loop of
+    plain GPU steps (one Stepper per scenario, same cell size and
+    precision)."""
+    import torch
+    from crowddynamics_amd import examples as E
+    from crowddynamics_amd.autotune import neutral_cell_sizes, max_body_radius, tune_ensemble
+    from crowddynamics_amd.ensemble import Ensemble
+    from crowddynamics_amd.stepper import Stepper
+    log('ensemble: %d scenarios' % n_scenarios)
+    scns = E.ensemble(n_scenarios)
+    n = sum(s.n for s in scns)
+    with torch.cuda.stream(stream):
+        sizes = neutral_cell_sizes(max_body_radius(scns))[:3]
+        tuned = tune_ensemble(scns, precision='f32', cell_sizes=sizes, steps=5, warmup=2)
+        log('autotune: cell %.3f, force block %d (%.3f ms/step)' % (
+            tuned['cell_size'], tuned['force_threads'], tuned['ms_per_step']))
+        ens = Ensemble(scns, precision='f32', cell_size=tuned['cell_size'],
+                       force_threads=tuned['force_threads'])
+        ens.step(10)
+        G = 50
+        graph = ens.capture(G)
+        reps = max(1, steps // G)
+        stream.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            graph.launch()
+        stream.synchronize()
+        el = time.perf_counter() - t0
+        ens.check()
+        kms = ens.timed(20)
+        del graph
+        batched = n * reps * G / el
+        del ens
+        log('ensemble batched: %.3e agent-steps/s (%.3f ms/step)' % (batched, 1e3 * el / (reps * G)))
+        steppers = [Stepper(s.agents.copy(), segments=s.segments, cell_size=tuned['cell_size'],
+                            dt_min=s.dt_min, dt_max=s.dt_max, seed=s.seed, precision='f32',
+                            bounds=s.bounds) for s in scns]
+        for st in steppers:
+            st.step(2)
+        stream.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(loop_steps):
+            for st in steppers:
+                st.step(1)
+        stream.synchronize()
+        el_loop = time.perf_counter() - t0
+        loop = n * loop_steps / el_loop
+        for st in steppers:
+            st.check()
+        del steppers
+    log('ensemble per-scenario loop: %.3e agent-steps/s' % loop)
+    return dict(scenarios=n_scenarios, agents=n, precision='f32 (f64 positions and dt)',
+                steps=reps * G, value=batched, unit='agent-steps/s',
+                ms_per_step=1e3 * el / (reps * G),
+                kernel_ms={k: kms[i] / 20 for i, k in enumerate(KERNELS)},
+                autotuned=dict(cell_size=tuned['cell_size'],
+                               force_threads=tuned['force_threads'],
+                               candidates=[dict(ms=t[0], cell_size=t[1], force_threads=t[2])
+                                           for t in tuned['table']]),
+                per_scenario_loop=dict(value=loop, steps=loop_steps,
+                                       ms_per_step=1e3 * el_loop / loop_steps),
+                speedup_vs_loop=batched / loop)
+
+
+def bench_corridor(steps, stream):
+    """50k ThreeCircle agents, 300x150 m corridor with 200 obstacles, fp64."""
+    import torch
+    from crowddynamics_amd import examples as E
+    from crowddynamics_amd.stepper import Stepper
+    log('corridor: 50k three-circle agents')
+    scn = E.corridor()
+    with torch.cuda.stream(stream):
+        st = Stepper(scn.agents.copy(), segments=scn.segments, cell_size=scn.cell_size,
+                     dt_min=scn.dt_min, dt_max=scn.dt_max, seed=scn.seed, precision='f64',
+                     bounds=scn.bounds)
+        st.step(10)
+        G = 50
+        graph = st.capture(G)
+        reps = max(1, steps // G)
+        stream.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            graph.launch()
+        stream.synchronize()
+        el = time.perf_counter() - t0
+        st.check()
+        kms = st.timed(20)
+        del graph
+    v = scn.n * reps * G / el
+    log('corridor: %.3e agent-steps/s' % v)
+    return dict(agents=scn.n, steps=reps * G, value=v, unit='agent-steps/s',
+                ms_per_step=1e3 * el / (reps * G), precision='f64',
+                kernel_ms={k: kms[i] / 20 for i, k in enumerate(KERNELS)})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -102,6 +199,11 @@ def main():
     ap.add_argument('--cpu-seconds', type=float, default=10.0)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--n', type=int, default=20000)
+    ap.add_argument('--ensemble-scenarios', type=int, default=256)
+    ap.add_argument('--ensemble-steps', type=int, default=1000)
+    ap.add_argument('--ensemble-loop-steps', type=int, default=20)
+    ap.add_argument('--no-ensemble', action='store_true')
+    ap.add_argument('--no-corridor', action='store_true')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU: the per-step update runs on a single MI355X '
@@ -185,6 +287,14 @@ def main():
     roofline = dict(kernel=dom, bound=rk['bound'], achieved=rk['achieved'], peak=rk['peak'],
                     unit=rk['unit'], frac=rk['frac'], traffic=traffic)
 
+    ensemble = None
+    if not args.no_ensemble:
+        ensemble = bench_ensemble(args.ensemble_scenarios, args.ensemble_steps,
+                                  args.ensemble_loop_steps, stream)
+    corridor = None
+    if not args.no_corridor:
+        corridor = bench_corridor(300, stream)
+
     cpu = None
     if not args.no_cpu:
         log('cpu baseline (oracle, 1 thread, %.0f s sample)' % args.cpu_seconds)
@@ -222,6 +332,8 @@ def main():
         'kernel_ms': {p: {k: v for k, v in d.items()} for p, d in kernel_ms.items()},
         'rooflines': per_kernel,
         'pair_tally': t0_tally,
+        'ensemble': ensemble,
+        'corridor_three_circle': corridor,
     }
     if cpu:
         line['speedup_vs_cpu_1core'] = r['value'] / cpu['value']

@@ -52,7 +52,16 @@ class CDAgents(C.Structure):
 class CDWorkspace(C.Structure):
     _fields_ = [('ptr', _vp), ('bytes', C.c_int64), ('n', C.c_int64),
                 ('cell_capacity', C.c_int64), ('cell_slots', C.c_int64), ('model', C.c_int32),
-                ('_pad', C.c_int32)]
+                ('max_scenarios', C.c_int32), ('n_scenarios', C.c_int32),
+                ('blocks_f', C.c_int32), ('blocks_e', C.c_int32),
+                ('force_threads', C.c_int32)]
+
+
+class CDScenario(C.Structure):
+    _fields_ = [('agent_begin', C.c_int64), ('agent_end', C.c_int64),
+                ('cell_begin', C.c_int64), ('cell_capacity', C.c_int64),
+                ('seg_begin', C.c_int64), ('seg_end', C.c_int64), ('seed', C.c_uint64),
+                ('dt_min', C.c_double), ('dt_max', C.c_double)]
 
 
 class CDState(C.Structure):
@@ -97,6 +106,10 @@ SIGNATURES = {
     'cd_reset': (C.c_int, [_P(CDAgents), _vp]),
     'cd_step_prepare': (C.c_int, [_P(CDAgents), _P(CDStepParams), _W, C.c_uint64, _vp]),
     'cd_step': (C.c_int, [_P(CDAgents), _P(CDStepParams), _W, _vp]),
+    'cd_batch_prepare': (C.c_int, [_P(CDAgents), _P(CDStepParams), _P(CDScenario), C.c_int32,
+                                   _W, C.c_uint64, _vp]),
+    'cd_batch_step': (C.c_int, [_P(CDAgents), _P(CDStepParams), _W, _vp]),
+    'cd_batch_read_states': (C.c_int, [_W, _P(CDState), C.c_int32, _vp]),
     'cd_step_timed': (C.c_int, [_P(CDAgents), _P(CDStepParams), _W, C.c_int64,
                                 _P(C.c_double), _vp]),
     'cd_graph_begin': (C.c_int, [_vp]),

@@ -26,6 +26,8 @@ __device__ __forceinline__ double dsqrt(double a) { return sqrt(a); }
 __device__ __forceinline__ float dsqrt(float a) { return sqrtf(a); }
 __device__ __forceinline__ double dexp(double a) { return exp(a); }
 __device__ __forceinline__ float dexp(float a) { return expf(a); }
+__device__ __forceinline__ bool is_nan(double a) { return __builtin_isnan(a); }
+__device__ __forceinline__ bool is_nan(float a) { return __builtin_isnan(a); }
 
 // Ordered-integer image of a double: unsigned comparison of the images
 // equals floating-point comparison (for non-NaN values).
@@ -105,7 +107,7 @@ __device__ __forceinline__ bool pair_circular(T x, T y, T vx, T vy, T r_tot, con
   T b = -(x * vx + y * vy);
   T c = (x * x + y * y) - r_tot * r_tot;
   T ds = dsqrt(b * b - a * c);
-  if (!(__builtin_isnan(ds) || ds == T(0) || a == T(0))) {
+  if (!(is_nan(ds) || ds == T(0) || a == T(0))) {
     T tau = (b - ds) / a;
     if (!(tau <= T(0) || tau > T(kTauMax))) {
       T gx = (vx - (vx * b + x * a) / ds) / a;

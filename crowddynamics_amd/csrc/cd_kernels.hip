@@ -3,8 +3,9 @@
 //
 // Data layout in HBM: one contiguous array per agent field (vectors
 // interleaved [n][2]), plus a caller-owned workspace holding the device state
-// scalars, per-block reduction partials, per-cell agent slot lists with
-// gathered neighbour data, and the block-list arrays of cd_add_to_cells.
+// scalars, the scenario table and per-scenario state of a batch, per-block
+// reduction partials, per-cell agent slot lists with gathered neighbour data,
+// and the block-list arrays of cd_add_to_cells.
 //
 // Binning contract (the reference binds cell_lists, an un-vendored
 // dependency, at crowddynamics/core/interactions.py:30,192-195):
@@ -14,23 +15,27 @@
 //   agents in the same or adjacent cells (|dl_x| <= 1 and |dl_y| <= 1)
 //   interacts once.
 //
-// One fused step is four kernels:
-//   K_a  k_bin_insert     reduce the previous step's partials (bounding box,
-//                         max |v|, max v0); bin every agent into its cell's
-//                         slot list
+// One step is four kernels, each launched once over all scenarios of a
+// batch (a single simulation is a batch of one):
+//   K_a  k_bin_insert     combine the previous step's partials per scenario
+//                         (bounding box, max |v|, max v0); bin every agent
+//                         into its cell's slot list
 //   K_b  k_cell_gather    per cell: order the slot list by agent index and
 //                         gather the neighbour data the force kernel reads
 //   K_e  k_forces         four lanes per agent: stencil rows, walls, own terms
 //   K_f  k_integrate      Verlet update, Reset, per-block partials for the
 //                         next step, clear the cell counts
-// Pair forces are owner-computes: each agent sums its own side of every pair,
-// so forces need no atomics and results are run-to-run deterministic.
+// Every block of K_a, K_e and K_f serves a single scenario (block maps built
+// by cd_batch_prepare).  Pair forces are owner-computes: each agent sums its
+// own side of every pair, so forces need no atomics and results are
+// run-to-run deterministic.
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string.h>
 
 #include <algorithm>
+#include <vector>
 
 #include "../../include/cdamd.h"
 #include "cd_device.h"
@@ -45,37 +50,63 @@ namespace {
 
 enum RedField { R_MINX = 0, R_MINY, R_MAXX, R_MAXY, R_VMAX, R_V0MAX, R_NONFINITE, R_COUNT };
 
+// Global device state.
 struct DevState {
+  long long status;
+  unsigned long long step;
+  unsigned int scan_epoch, scan_ticket;
+};
+
+// Scenario table entry (device).
+struct ScnDesc {
+  int a0, a1;    // agents [a0, a1)
+  int c0, ccap;  // grid cells [c0, c0 + ccap)
+  int w0, w1;    // wall segments [w0, w1)
+  int f0, nf;    // blocks (and partials) [f0, f0 + nf) of the per-scenario kernels
+  unsigned long long seed;
+  double dt_min, dt_max;
+};
+
+// Per-scenario state (device).
+struct ScnState {
   unsigned long long red[8];  // last reduced values (R_*), ord-bit form except R_VMAX
   double dt, time_tot;
   double origin[2];
   int nx, ny;
-  int valid;       // grid of the last binning fits the capacity
-  int C;           // cells of the last binning
-  int C_clear;     // cells whose counts the next clearing pass must zero
-  int oob;         // an agent fell outside the grid (stale bounding box)
-  int nparts;      // partials written by the last reduction producer
-  int pad0;
-  unsigned int scan_epoch, scan_ticket;
-  long long status;
-  unsigned long long step;
+  int valid;    // grid of the last binning fits the scenario's cells
+  int C;        // cells of the last binning
+  int C_clear;  // cells whose counts the next clearing pass must zero
+  int oob;      // an agent fell outside the grid (stale bounding box)
 };
 
-constexpr int kMaxParts = 512;      // blocks of any partial-producing kernel
+constexpr int kBS = 256;              // threads per block of per-agent kernels
+constexpr int kMaxPartsPerScn = 128;  // blocks of the partial-producing kernels per scenario
+constexpr int kSingleParts = 512;     // the same for a single (unbatched) scenario
+constexpr int kAgentsPerForceBlock = 64;  // default agents per force block (256 threads)
+constexpr int kMinForceAgents = 16;
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 16;
 constexpr int kScanTile = kScanThreads * kScanItems;
 
 constexpr int64_t kAlign = 256;
 inline int64_t align_up(int64_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
+inline int grid_for(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
+
+inline int64_t max_blocks_f(int64_t n, int64_t S) {
+  return std::max<int64_t>(kSingleParts, n / kBS + S + 1);
+}
+inline int64_t max_blocks_e(int64_t n, int64_t S) { return n / kMinForceAgents + S + 1; }
 
 struct Layout {
-  int64_t state, partials, tiles, cell_of, rank, pidx, count, offset;
+  int64_t state, scn_desc, scn_state, map_f, map_e, partials, tiles;
+  int64_t cell_of, rank, pidx, count, offset;
   int64_t ell_ids, e_pos, e_vel, e_r, e_ls, e_rs, e_phi, e_rt, e_rs_r, e_rts;
   int64_t total;
 };
 
-Layout make_layout(int64_t n, int64_t cap, int64_t K, int32_t model) {
+Layout make_layout(const cd_workspace_t *w) {
+  const int64_t n = w->n, cap = w->cell_capacity, K = w->cell_slots;
+  const int64_t S = std::max<int64_t>(1, w->max_scenarios);
   Layout L;
   int64_t o = 0;
   auto take = [&](int64_t bytes) {
@@ -83,8 +114,12 @@ Layout make_layout(int64_t n, int64_t cap, int64_t K, int32_t model) {
     o = align_up(o + bytes);
     return r;
   };
-  L.state = take(4096);
-  L.partials = take(8 * 8 * (kMaxParts + 1));
+  L.state = take(256);
+  L.scn_desc = take(sizeof(ScnDesc) * S);
+  L.scn_state = take(sizeof(ScnState) * S);
+  L.map_f = take(8 * max_blocks_f(n, S));
+  L.map_e = take(8 * max_blocks_e(n, S));
+  L.partials = take(8 * 8 * max_blocks_f(n, S));
   L.tiles = take(8 * ((cap + kScanTile - 1) / kScanTile + 1));
   L.cell_of = take(4 * n);
   L.rank = take(4 * n);
@@ -96,7 +131,7 @@ Layout make_layout(int64_t n, int64_t cap, int64_t K, int32_t model) {
   L.e_pos = take(16 * slots);
   L.e_vel = take(16 * slots);
   L.e_r = take(8 * slots);
-  if (model == CD_THREE_CIRCLE) {
+  if (w->model == CD_THREE_CIRCLE) {
     L.e_ls = take(16 * slots);
     L.e_rs = take(16 * slots);
     L.e_phi = take(8 * slots);
@@ -174,8 +209,18 @@ struct Work {
   int cap, K, ntiles;
 };
 
+// Scenario view of a launch: the table (null for a single scenario, which is
+// then `one`), the per-scenario states and the block maps (null: block b of a
+// single scenario is its b-th block).
+struct Batch {
+  const ScnDesc *desc;
+  ScnState *sst;
+  const int2 *map_f, *map_e;
+  ScnDesc one;
+};
+
 Work carve(const cd_workspace_t *w) {
-  const Layout L = make_layout(w->n, w->cell_capacity, w->cell_slots, w->model);
+  const Layout L = make_layout(w);
   char *b = (char *)w->ptr;
   Work W;
   W.st = (DevState *)(b + L.state);
@@ -202,16 +247,66 @@ Work carve(const cd_workspace_t *w) {
   return W;
 }
 
+inline int single_parts(int64_t n) {
+  return std::max(1, std::min(grid_for(n, kBS), kSingleParts));
+}
+
+// Batch view of a single scenario covering every agent, cell and wall.
+Batch single_batch(const cd_workspace_t *w, int64_t n_segments, uint64_t seed, double dt_min,
+                   double dt_max) {
+  const Layout L = make_layout(w);
+  Batch B;
+  B.desc = nullptr;
+  B.sst = (ScnState *)((char *)w->ptr + L.scn_state);
+  B.map_f = nullptr;
+  B.map_e = nullptr;
+  B.one.a0 = 0;
+  B.one.a1 = (int)w->n;
+  B.one.c0 = 0;
+  B.one.ccap = (int)w->cell_capacity;
+  B.one.w0 = 0;
+  B.one.w1 = (int)n_segments;
+  B.one.f0 = 0;
+  B.one.nf = single_parts(w->n);
+  B.one.seed = seed;
+  B.one.dt_min = dt_min;
+  B.one.dt_max = dt_max;
+  return B;
+}
+
+Batch batch_of(const cd_workspace_t *w) {
+  const Layout L = make_layout(w);
+  char *b = (char *)w->ptr;
+  Batch B;
+  memset(&B, 0, sizeof(B));
+  B.desc = (const ScnDesc *)(b + L.scn_desc);
+  B.sst = (ScnState *)(b + L.scn_state);
+  B.map_f = (const int2 *)(b + L.map_f);
+  B.map_e = (const int2 *)(b + L.map_e);
+  return B;
+}
+
+__device__ __forceinline__ ScnDesc desc_of(const Batch &B, int s) {
+  return B.desc ? B.desc[s] : B.one;
+}
+// (scenario, block index within the scenario) of this block
+__device__ __forceinline__ int2 block_f(const Batch &B) {
+  return B.map_f ? B.map_f[blockIdx.x] : make_int2(0, (int)blockIdx.x);
+}
+__device__ __forceinline__ int2 block_e(const Batch &B) {
+  return B.map_e ? B.map_e[blockIdx.x] : make_int2(0, (int)blockIdx.x);
+}
+
 constexpr unsigned long long kOrdPosInf = 0xFFF0000000000000ull;  // ord_bits(+inf)
 constexpr unsigned long long kOrdNegInf = 0x000FFFFFFFFFFFFFull;  // ord_bits(-inf)
 
-__global__ void __launch_bounds__(256) k_ws_init(Work W) {
+__global__ void __launch_bounds__(256) k_ws_init(Work W, ScnState *sst, int S) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int stride = gridDim.x * blockDim.x;
   if (t == 0) memset(W.st, 0, sizeof(DevState));
+  for (int s = t; s < S; s += stride) memset(&sst[s], 0, sizeof(ScnState));
   for (int c = t; c < W.cap; c += stride) W.count[c] = 0;
   for (int k = t; k < W.ntiles + 1; k += stride) W.tiles[k] = 0ull;
-  for (int k = t; k < 8 * (kMaxParts + 1); k += stride) W.partials[k] = 0ull;
 }
 
 // ---------------------------------------------------------------------------
@@ -285,47 +380,48 @@ __device__ Red block_reduce(Red r) {
   return r;
 }
 
-// Producer side: each block stores its reduction as one partial.  The next
-// kernel on the stream combines the partials (kernel boundaries make them
-// visible), so producers need neither fences nor atomics.
+// Producer side: each block stores its reduction as one partial (index =
+// its block).  The next kernel on the stream combines a scenario's
+// partials; kernel boundaries make them visible, so producers need neither
+// fences nor atomics.
 __device__ __forceinline__ void store_partial(const Work &W, const Red &r) {
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0)
     for (int k = 0; k < R_COUNT; ++k) W.partials[blockIdx.x * 8 + k] = r.v[k];
-    if (blockIdx.x == 0) W.st->nparts = gridDim.x;
-  }
 }
 
-// Consumer side: every block combines all partials (at most kMaxParts).
-__device__ Red reduce_partials(const Work &W) {
+// Consumer side: every block combines its scenario's partials.
+__device__ Red reduce_partials(const Work &W, const ScnDesc &d) {
   Red r;
   r.init();
-  const int np = W.st->nparts;
-  for (int b = threadIdx.x; b < np; b += blockDim.x) r.combine(&W.partials[b * 8]);
+  for (int b = threadIdx.x; b < d.nf; b += blockDim.x) r.combine(&W.partials[(d.f0 + b) * 8]);
   return block_reduce(r);
 }
 
-__device__ __forceinline__ double red_vmax(const Red &r) {
-  return __longlong_as_double((long long)r.v[R_VMAX]);
+__device__ __forceinline__ double red_vmax(const unsigned long long *v) {
+  return __longlong_as_double((long long)v[R_VMAX]);
 }
 
-// Partials of the current state (node mode and cd_step_prepare); also zeroes
-// the counts of the previous binning.
+// Partials of the current state (single-scenario operators and
+// cd_step_prepare); also zeroes the counts of the previous binning.
 template <typename T>
-__global__ void __launch_bounds__(256) k_reduce(Agents<T> A, Work W, int clear_counts) {
+__global__ void __launch_bounds__(256) k_reduce(Agents<T> A, Work W, Batch B, int clear_counts) {
+  const int2 sk = block_f(B);
+  const ScnDesc d = desc_of(B, sk.x);
+  ScnState *ss = &B.sst[sk.x];
+  const int t0 = sk.y * blockDim.x + threadIdx.x, stride = d.nf * blockDim.x;
   if (clear_counts) {
-    const int clear = W.st->C_clear;
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < clear; c += gridDim.x * blockDim.x)
-      W.count[c] = 0;
+    const int clear = ss->C_clear;
+    for (int c = t0; c < clear; c += stride) W.count[d.c0 + c] = 0;
   }
   Red r;
   r.init();
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gridDim.x * blockDim.x) {
+  for (int i = d.a0 + t0; i < d.a1; i += stride) {
     const V2<T> w = A.vel[i];
     r.add_agent(A.pos[i], dhypot((double)w.x, (double)w.y), A.v0[i]);
   }
   r = block_reduce(r);
   store_partial(W, r);
-  if (blockIdx.x == 0 && threadIdx.x == 0) W.st->oob = 0;
+  if (sk.y == 0 && threadIdx.x == 0) ss->oob = 0;
 }
 
 // Grid of the binning from a reduced bounding box (contract in the header).
@@ -354,24 +450,25 @@ __device__ __forceinline__ Grid grid_of(const Red &r, double cell, int cap) {
   return g;
 }
 
-// Records the reduction and the grid in the state (block 0, thread 0).
-__device__ __forceinline__ void publish_grid(const Work &W, const Red &r, const Grid &g,
-                                             bool binning) {
-  DevState *st = W.st;
-  for (int k = 0; k < R_COUNT; ++k) st->red[k] = r.v[k];
-  if (r.v[R_NONFINITE]) st->status |= CD_STATUS_NONFINITE;
+// Records a scenario's reduction and grid (one thread per scenario).
+__device__ __forceinline__ void publish_grid(const Work &W, ScnState *ss, const Red &r,
+                                             const Grid &g, bool binning, bool has_agents) {
+  for (int k = 0; k < R_COUNT; ++k) ss->red[k] = r.v[k];
+  if (r.v[R_NONFINITE])
+    atomicOr((unsigned long long *)&W.st->status, (unsigned long long)CD_STATUS_NONFINITE);
   if (!binning) return;
-  st->origin[0] = g.ox;
-  st->origin[1] = g.oy;
-  st->nx = g.nx;
-  st->ny = g.ny;
-  st->valid = g.ok ? 1 : 0;
-  st->C = g.ok ? g.nx * g.ny : 0;
-  st->C_clear = g.ok ? g.nx * g.ny : 0;
-  if (!g.ok) st->status |= CD_STATUS_GRID_OVERFLOW;
+  ss->origin[0] = g.ox;
+  ss->origin[1] = g.oy;
+  ss->nx = g.nx;
+  ss->ny = g.ny;
+  ss->valid = g.ok ? 1 : 0;
+  ss->C = g.ok ? g.nx * g.ny : 0;
+  ss->C_clear = g.ok ? g.nx * g.ny : 0;
+  if (!g.ok && has_agents)  // an empty scenario has no grid and no overflow
+    atomicOr((unsigned long long *)&W.st->status, (unsigned long long)CD_STATUS_GRID_OVERFLOW);
 }
 
-// Cell of a position, or -1 outside the grid.
+// Local cell of a position, or -1 outside the grid.
 __device__ __forceinline__ int cell_of_pos(double2 p, const Grid &g, double cell) {
   const double qx = (p.x - g.ox) / cell, qy = (p.y - g.oy) / cell;
   if (!(qx >= 0.0 && qy >= 0.0 && qx < (double)g.nx && qy < (double)g.ny)) return -1;
@@ -382,58 +479,77 @@ __device__ __forceinline__ int cell_of_pos(double2 p, const Grid &g, double cell
 // K_a / K_b: per-cell slot lists
 // ---------------------------------------------------------------------------
 
-// K_a: reduce the partials; when `binning`, bin agent i into its cell's slot
-// list (slots beyond cell_slots are not stored; the cell count tells).
-__global__ void __launch_bounds__(256) k_bin_insert(const double2 *pos, int n, double cell,
-                                                    Work W, int binning) {
-  const Red r = reduce_partials(W);
-  const Grid g = grid_of(r, cell, W.cap);
-  if (blockIdx.x == 0 && threadIdx.x == 0) publish_grid(W, r, g, binning != 0);
+// K_a: combine the scenario's partials; when `binning`, bin its agents into
+// their cells' slot lists (slots beyond cell_slots are not stored; the cell
+// count tells).  Cell indices are global: scenario base + local cell.
+__global__ void __launch_bounds__(256) k_bin_insert(const double2 *pos, double cell, Work W,
+                                                    Batch B, int binning) {
+  const int2 sk = block_f(B);
+  const ScnDesc d = desc_of(B, sk.x);
+  ScnState *ss = &B.sst[sk.x];
+  const int t0 = sk.y * blockDim.x + threadIdx.x, stride = d.nf * blockDim.x;
+  const int i0 = d.a0 + t0;
+  const double2 p0 = (binning && i0 < d.a1) ? pos[i0] : make_double2(0.0, 0.0);
+  const Red r = reduce_partials(W, d);
+  const Grid g = grid_of(r, cell, d.ccap);
+  if (sk.y == 0 && threadIdx.x == 0) publish_grid(W, ss, r, g, binning != 0, d.a1 > d.a0);
   if (!binning || !g.ok) return;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const int c = cell_of_pos(pos[i], g, cell);
-    if (c < 0) {  // positions changed after the bounding box was reduced
-      atomicOr(&W.st->oob, 1);
+  for (int i = i0; i < d.a1; i += stride) {
+    const int lc = cell_of_pos(i == i0 ? p0 : pos[i], g, cell);
+    if (lc < 0) {  // positions changed after the bounding box was reduced
+      atomicOr(&ss->oob, 1);
       W.cell_of[i] = -1;
       continue;
     }
+    const int c = d.c0 + lc;
     W.cell_of[i] = c;
     const int k = atomicAdd(&W.count[c], 1);
     if (k < W.K) W.ell_ids[c * W.K + k] = i;
   }
 }
 
-// K_b: one thread per cell.  Orders the cell's slot list by agent index and
-// gathers the neighbour data; cells with more agents than slots are flagged
-// (the force kernel visits their members through the exact slow path).
+// K_b: one thread per cell of the workspace grid.  Orders the cell's slot
+// list by agent index and gathers the neighbour data; cells with more agents
+// than slots are flagged (the force kernel visits their members through the
+// exact slow path).
 template <typename T>
 __global__ void __launch_bounds__(256) k_cell_gather(Agents<T> A, Work W) {
-  DevState *st = W.st;
-  if (!st->valid) return;
-  if (st->oob) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      st->valid = 0;
-      st->status |= CD_STATUS_GRID_OVERFLOW;
-    }
-    return;
-  }
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= st->C) return;
+  if (c >= W.cap) return;
   const int cnt = W.count[c];
   if (cnt == 0) return;
   if (cnt > W.K) {
-    atomicOr((unsigned long long *)&st->status, (unsigned long long)CD_STATUS_SLOTS_FULL);
+    atomicOr((unsigned long long *)&W.st->status, (unsigned long long)CD_STATUS_SLOTS_FULL);
     return;
   }
   int *ids = W.ell_ids + (size_t)c * W.K;
-  for (int s = 1; s < cnt; ++s) {  // insertion sort (cells hold a few agents)
-    const int key = ids[s];
-    int t = s - 1;
-    while (t >= 0 && ids[t] > key) {
-      ids[t + 1] = ids[t];
-      --t;
+  if (cnt <= 8) {
+    // odd-even transposition network on registers (padding sorts last)
+    int v[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) v[s] = s < cnt ? ids[s] : 0x7fffffff;
+#pragma unroll
+    for (int ph = 0; ph < 8; ++ph) {
+#pragma unroll
+      for (int s = ph & 1; s + 1 < 8; s += 2) {
+        const int lo = min(v[s], v[s + 1]), hi = max(v[s], v[s + 1]);
+        v[s] = lo;
+        v[s + 1] = hi;
+      }
     }
-    ids[t + 1] = key;
+#pragma unroll
+    for (int s = 0; s < 8; ++s)
+      if (s < cnt) ids[s] = v[s];
+  } else {
+    for (int s = 1; s < cnt; ++s) {  // insertion sort of a crowded cell
+      const int key = ids[s];
+      int t = s - 1;
+      while (t >= 0 && ids[t] > key) {
+        ids[t + 1] = ids[t];
+        --t;
+      }
+      ids[t + 1] = key;
+    }
   }
   V2<T> *e_vel = (V2<T> *)W.e_vel;
   for (int s = 0; s < cnt; ++s) {
@@ -454,29 +570,33 @@ __global__ void __launch_bounds__(256) k_cell_gather(Agents<T> A, Work W) {
 }
 
 // ---------------------------------------------------------------------------
-// Block-list arrays of cd_add_to_cells (counting sort with offsets)
+// Block-list arrays of cd_add_to_cells (single scenario; counting sort with
+// offsets)
 // ---------------------------------------------------------------------------
 
-__global__ void __launch_bounds__(256) k_bl_count(const double2 *pos, int n, double cell,
-                                                  Work W) {
-  const Red r = reduce_partials(W);
-  const Grid g = grid_of(r, cell, W.cap);
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    publish_grid(W, r, g, true);
+__global__ void __launch_bounds__(256) k_bl_count(const double2 *pos, double cell, Work W,
+                                                  Batch B) {
+  const int2 sk = block_f(B);
+  const ScnDesc d = desc_of(B, sk.x);
+  ScnState *ss = &B.sst[sk.x];
+  const Red r = reduce_partials(W, d);
+  const Grid g = grid_of(r, cell, d.ccap);
+  if (sk.y == 0 && threadIdx.x == 0) {
+    publish_grid(W, ss, r, g, true, d.a1 > d.a0);
     W.st->scan_epoch = (W.st->scan_epoch + 1u) & 0x3fffffffu;
     if (W.st->scan_epoch == 0u) W.st->scan_epoch = 1u;
     W.st->scan_ticket = 0u;
   }
   if (!g.ok) return;
-  for (int j = i; j < n; j += gridDim.x * blockDim.x) {
-    const int c = cell_of_pos(pos[j], g, cell);
+  const int stride = d.nf * blockDim.x;
+  for (int i = d.a0 + sk.y * blockDim.x + threadIdx.x; i < d.a1; i += stride) {
+    const int c = cell_of_pos(pos[i], g, cell);
     if (c < 0) {
-      atomicOr(&W.st->oob, 1);
+      atomicOr(&ss->oob, 1);
       continue;
     }
-    W.cell_of[j] = c;
-    W.rank[j] = atomicAdd(&W.count[c], 1);
+    W.cell_of[i] = c;
+    W.rank[i] = atomicAdd(&W.count[c], 1);
   }
 }
 
@@ -493,19 +613,19 @@ __device__ __forceinline__ unsigned long long scan_pack(unsigned epoch, unsigned
   return ((unsigned long long)epoch << 34) | ((unsigned long long)flag << 32) | (unsigned)v;
 }
 
-__global__ void __launch_bounds__(kScanThreads) k_bl_scan(Work W) {
+__global__ void __launch_bounds__(kScanThreads) k_bl_scan(Work W, ScnState *ss) {
   __shared__ int s_tile, s_prefix;
   __shared__ int s_wave[kScanThreads / 64];
   DevState *st = W.st;
-  if (st->oob) {
+  if (ss->oob) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-      st->valid = 0;
+      ss->valid = 0;
       st->status |= CD_STATUS_GRID_OVERFLOW;
     }
     return;
   }
-  if (!st->valid) return;
-  const int C = st->C;
+  if (!ss->valid) return;
+  const int C = ss->C;
   const unsigned epoch = st->scan_epoch;
   const int ntiles = (C + kScanTile - 1) / kScanTile;
   if (threadIdx.x == 0) s_tile = (int)atomicAdd(&st->scan_ticket, 1u);
@@ -576,7 +696,7 @@ __global__ void __launch_bounds__(kScanThreads) k_bl_scan(Work W) {
     }
     if (lane == 0) {
       if (failed) {
-        st->valid = 0;
+        ss->valid = 0;
         st->status |= CD_STATUS_GRID_OVERFLOW;
       }
       if (tile != 0)
@@ -596,18 +716,18 @@ __global__ void __launch_bounds__(kScanThreads) k_bl_scan(Work W) {
   if (tile == ntiles - 1 && t == kScanThreads - 1) W.offset[C] = s_prefix + agg;
 }
 
-__global__ void __launch_bounds__(256) k_bl_scatter(int n, Work W) {
-  if (!W.st->valid) return;
+__global__ void __launch_bounds__(256) k_bl_scatter(int n, Work W, const ScnState *ss) {
+  if (!ss->valid) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   W.pidx[W.offset[W.cell_of[i]] + W.rank[i]] = i;
 }
 
 // Orders each cell's segment by agent index (stable counting sort).
-__global__ void __launch_bounds__(256) k_bl_sort(Work W) {
-  if (!W.st->valid) return;
+__global__ void __launch_bounds__(256) k_bl_sort(Work W, const ScnState *ss) {
+  if (!ss->valid) return;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= W.st->C) return;
+  if (c >= ss->C) return;
   const int s0 = W.offset[c], s1 = W.offset[c + 1];
   for (int s = s0 + 1; s < s1; ++s) {
     const int key = W.pidx[s];
@@ -626,7 +746,8 @@ __global__ void __launch_bounds__(256) k_bl_sort(Work W) {
 
 // Fluctuation noise of agent i (core/motion/fluctuation.py:14-62): force
 // (mass * sigma * |TN(0,3)|) * (cos phi, sin phi), phi ~ U(0, 2 pi); torque
-// inertia * sigma_t * TN(-3, 3) when `torque`.
+// inertia * sigma_t * TN(-3, 3) when `torque`.  `i` is the agent's counter
+// (its index within its scenario).
 __device__ __forceinline__ void fluct_noise(uint64_t seed, uint64_t step, int i, double mass,
                                             double sig_f, bool torque, double inertia,
                                             double sig_t, double &fx, double &fy, double &m) {
@@ -849,22 +970,21 @@ __global__ void __launch_bounds__(256) k_walls(Agents<T> A, int model, const dou
 // K_e: forces (owner-computes over the 3x3 cell stencil)
 // ---------------------------------------------------------------------------
 //
-// Four lanes serve agent i = thread / 4 (agent index order, so the agent's
-// own data loads are coalesced).  Lanes 0..2 sum the agent's side of every
-// pair in stencil row dx = -1, 0, +1 (cells ly-1..ly+1 in slot order within
-// each cell); lane 3 evaluates the agent's own terms (fluctuation,
-// navigation, orientation, adjusting), issuing its loads before the pair
-// work; wall contacts are split over the four lanes.  The partial sums meet
-// on lane 3 through wavefront shuffles in a fixed order (deterministic).
+// Four lanes serve one agent (16, 32 or 64 agents per block, all of one
+// scenario, in agent index order so own-data loads are coalesced).
+// Lanes 0..2 sum the agent's side of every pair in stencil row dx = -1, 0, +1
+// (cells ly-1..ly+1 in slot order within each cell); lane 3 evaluates the
+// agent's own terms (fluctuation, navigation, orientation, adjusting), issuing
+// its loads and the noise arithmetic before the pair work; wall contacts are
+// split over the four lanes.  The partial sums meet on lane 3 through
+// wavefront shuffles in a fixed order (deterministic).
 
 struct FusedArgs {
   int flags, nav_mode;
-  uint64_t seed;
   const double *maps;
   int64_t nav_T, nav_rows, nav_cols;
   double nav_minx, nav_miny, nav_step;
   const double4 *segs;
-  int W;
 };
 
 template <typename T>
@@ -943,11 +1063,12 @@ __device__ __forceinline__ void pair_add_tc(int i, int j, const TCView &me, cons
   }
 }
 
-// Pairs of agent i with the agents of cell c.
+// Pairs of agent i with the agents of global cell c (scenario agents a0..a1).
 template <typename T, int MODEL>
-__device__ __forceinline__ void cell_pairs(const Agents<T> &A, const Work &W, int c, int i,
-                                           double2 xi, V2<T> vi, double ri, const Owner<T> &o,
-                                           const TCView &me, T &fx, T &fy, double &tq) {
+__device__ __forceinline__ void cell_pairs(const Agents<T> &A, const Work &W, int c, int a0,
+                                           int a1, int i, double2 xi, V2<T> vi, double ri,
+                                           const Owner<T> &o, const TCView &me, T &fx, T &fy,
+                                           double &tq) {
   const int cnt = W.count[c];
   if (cnt <= 0) return;
   if (cnt <= W.K) {
@@ -966,8 +1087,8 @@ __device__ __forceinline__ void cell_pairs(const Agents<T> &A, const Work &W, in
     return;
   }
   // more agents than slots: visit the cell's members in agent index order
-  // (the contract's order) by scanning the cell of every agent
-  for (int j = 0; j < A.n; ++j) {
+  // (the contract's order) by scanning the cell of every agent of the scenario
+  for (int j = a0; j < a1; ++j) {
     if (j == i || W.cell_of[j] != c) continue;
     if constexpr (MODEL == 0) {
       pair_add_c<T>(xi, vi, ri, o, A.pos[j], A.vel[j], A.radius[j], fx, fy);
@@ -982,17 +1103,18 @@ __device__ __forceinline__ void cell_pairs(const Agents<T> &A, const Work &W, in
 // standard logic tree (fluctuation, navigation, orientation, adjusting,
 // pairs, walls), added to force (torque).
 template <typename T, int MODEL, bool FUSED>
-__global__ void __launch_bounds__(256) k_forces(Agents<T> A, Work W, FusedArgs fa) {
+__global__ void __launch_bounds__(256) k_forces(Agents<T> A, Work W, Batch B, FusedArgs fa) {
   __shared__ double4 sseg[kSegTile];
-  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = gt >> 2;
+  const int2 sk = block_e(B);
+  const ScnDesc d = desc_of(B, sk.x);
+  const ScnState *ss = &B.sst[sk.x];
+  const int i = d.a0 + sk.y * (int)(blockDim.x >> 2) + (threadIdx.x >> 2);
   const int q = threadIdx.x & 3;
   const int lane0 = (threadIdx.x & 63) & ~3;
-  const DevState *st = W.st;
   const bool pairs_on = FUSED ? ((fa.flags & CD_USE_AGENT_AGENT) != 0) : true;
-  const bool binned = pairs_on && st->valid && !st->oob;
+  const bool binned = pairs_on && ss->valid && !ss->oob;
   if (!FUSED && !binned) return;  // node mode: binning invalid, nothing to add
-  const bool have = i < A.n;
+  const bool have = i < d.a1;
   double2 pos = make_double2(0.0, 0.0);
   V2<T> vel;
   vel.x = T(0);
@@ -1012,6 +1134,7 @@ __global__ void __launch_bounds__(256) k_forces(Agents<T> A, Work W, FusedArgs f
   e.y = T(0);
   bool nav_hit = false;
   double nex = 0.0, ney = 0.0;
+  double nfx = 0.0, nfy = 0.0, nm = 0.0;
   if (have && q == 3) {
     f = A.force[i];
     if (MODEL) tq = A.torque[i];
@@ -1021,13 +1144,18 @@ __global__ void __launch_bounds__(256) k_forces(Agents<T> A, Work W, FusedArgs f
         nav_hit = nav_lookup(pos.x, pos.y, A.target[i], fa.maps, fa.nav_T, fa.nav_rows,
                              fa.nav_cols, fa.nav_minx, fa.nav_miny, fa.nav_step, fa.nav_mode,
                              nex, ney);
+      // the noise is pure arithmetic: evaluate it while the other lanes wait
+      // for their neighbour data
+      if (fa.flags & CD_USE_FLUCT)
+        fluct_noise(d.seed, W.st->step, i - d.a0, A.mass[i], A.sig_f[i], MODEL != 0,
+                    MODEL ? A.inertia[i] : 0.0, MODEL ? A.sig_t[i] : 0.0, nfx, nfy, nm);
     }
   }
   T px = T(0), py = T(0);  // this lane's pair partial sum
   double ptq = 0.0;
   if (have && binned && q < 3) {
-    const int c = W.cell_of[i];
-    const int ny = st->ny, nx = st->nx;
+    const int c = W.cell_of[i] - d.c0;
+    const int ny = ss->ny, nx = ss->nx;
     const int lx = c / ny, ly = c - lx * ny;
     const int ox = lx + q - 1;
     if (ox >= 0 && ox < nx) {
@@ -1037,16 +1165,18 @@ __global__ void __launch_bounds__(256) k_forces(Agents<T> A, Work W, FusedArgs f
       const double ri = A.radius[i];
       const int ylo = max(ly - 1, 0), yhi = min(ly + 1, ny - 1);
       for (int oy = ylo; oy <= yhi; ++oy)
-        cell_pairs<T, MODEL>(A, W, ox * ny + oy, i, pos, vel, ri, o, me, px, py, ptq);
+        cell_pairs<T, MODEL>(A, W, d.c0 + ox * ny + oy, d.a0, d.a1, i, pos, vel, ri, o, me, px,
+                             py, ptq);
     }
   }
   T wx = T(0), wy = T(0);  // this lane's wall partial sum
   double wtq = 0.0;
   if (FUSED && (fa.flags & CD_USE_OBSTACLE)) {
-    for (int w0 = 0; w0 < fa.W; w0 += kSegTile) {
-      const int cnt = min(kSegTile, fa.W - w0);
+    const int nw = d.w1 - d.w0;
+    for (int w0 = 0; w0 < nw; w0 += kSegTile) {
+      const int cnt = min(kSegTile, nw - w0);
       __syncthreads();
-      for (int k = threadIdx.x; k < cnt; k += blockDim.x) sseg[k] = fa.segs[w0 + k];
+      for (int k = threadIdx.x; k < cnt; k += blockDim.x) sseg[k] = fa.segs[d.w0 + w0 + k];
       __syncthreads();
       if (have) walls_agent(A, MODEL, i, pos, vel, sseg, cnt, wx, wy, wtq, q, 4);
     }
@@ -1073,9 +1203,6 @@ __global__ void __launch_bounds__(256) k_forces(Agents<T> A, Work W, FusedArgs f
   if (!have || q != 3) return;
   if (FUSED) {
     if (fa.flags & CD_USE_FLUCT) {
-      double nfx, nfy, nm;
-      fluct_noise(fa.seed, st->step, i, A.mass[i], A.sig_f[i], MODEL != 0,
-                  MODEL ? A.inertia[i] : 0.0, MODEL ? A.sig_t[i] : 0.0, nfx, nfy, nm);
       f.x += (T)nfx;
       f.y += (T)nfy;
       if (MODEL) tq += nm;
@@ -1111,10 +1238,11 @@ __global__ void __launch_bounds__(256) k_forces(Agents<T> A, Work W, FusedArgs f
 // ---------------------------------------------------------------------------
 
 // adaptive_timestep (core/integrator.py:81-97) from reduced values.
-__device__ __forceinline__ double adaptive_dt(const Red &r, double dt_min, double dt_max) {
-  const double v_max = red_vmax(r);
+__device__ __forceinline__ double adaptive_dt(const unsigned long long *v, double dt_min,
+                                              double dt_max) {
+  const double v_max = red_vmax(v);
   if (v_max == 0.0) return dt_max;
-  const double v0max = from_ord_bits(r.v[R_V0MAX]);
+  const double v0max = from_ord_bits(v[R_V0MAX]);
   const double dx_max = 1.1 * v0max * dt_max;
   const double dt = dx_max / v_max;
   if (dt > dt_max) return dt_max;
@@ -1173,40 +1301,45 @@ __device__ __forceinline__ void verlet_agent(const Agents<T> &A, int model, int 
   v_out = vel;
 }
 
-// Node mode: dt from the partials the preceding k_reduce wrote.
+// Integrator node: dt from the partials the preceding k_reduce wrote.
 template <typename T>
-__global__ void __launch_bounds__(256) k_integrate_node(Agents<T> A, int model, Work W,
-                                                        double dt_min, double dt_max) {
-  const Red r = reduce_partials(W);
-  const double dt = adaptive_dt(r, dt_min, dt_max);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    for (int k = 0; k < R_COUNT; ++k) W.st->red[k] = r.v[k];
-    W.st->dt = dt;
-    W.st->time_tot += dt;
+__global__ void __launch_bounds__(256) k_integrate_node(Agents<T> A, int model, Work W, Batch B) {
+  const int2 sk = block_f(B);
+  const ScnDesc d = desc_of(B, sk.x);
+  ScnState *ss = &B.sst[sk.x];
+  const Red r = reduce_partials(W, d);
+  const double dt = adaptive_dt(r.v, d.dt_min, d.dt_max);
+  if (sk.y == 0 && threadIdx.x == 0) {
+    for (int k = 0; k < R_COUNT; ++k) ss->red[k] = r.v[k];
+    ss->dt = dt;
+    ss->time_tot += dt;
   }
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gridDim.x * blockDim.x) {
+  const int stride = d.nf * blockDim.x;
+  for (int i = d.a0 + sk.y * blockDim.x + threadIdx.x; i < d.a1; i += stride) {
     double2 x;
     V2<T> v;
     verlet_agent(A, model, i, dt, false, x, v);
   }
 }
 
-// Fused step: dt from the values K_a reduced (st->red); Verlet and Reset of
-// every agent (grid-stride, at most kMaxParts blocks); partials of the new
-// state for the next step; zero this step's cell counts.
+// Fused step: dt from the values K_a reduced (scenario state); Verlet and
+// Reset of the scenario's agents (grid-stride over its blocks); partials of
+// the new state for the next step; zero this step's cell counts.
 template <typename T>
-__global__ void __launch_bounds__(256) k_integrate(Agents<T> A, int model, Work W,
-                                                   double dt_min, double dt_max, int do_reset) {
-  DevState *st = W.st;
-  Red prev;
-  for (int k = 0; k < R_COUNT; ++k) prev.v[k] = st->red[k];
-  const double dt = adaptive_dt(prev, dt_min, dt_max);
-  const int stride = gridDim.x * blockDim.x;
-  const int clear = st->C_clear;
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < clear; c += stride) W.count[c] = 0;
+__global__ void __launch_bounds__(256) k_integrate(Agents<T> A, int model, Work W, Batch B,
+                                                   int do_reset) {
+  const int2 sk = block_f(B);
+  const ScnDesc d = desc_of(B, sk.x);
+  ScnState *ss = &B.sst[sk.x];
+  unsigned long long prev[R_COUNT];
+  for (int k = 0; k < R_COUNT; ++k) prev[k] = ss->red[k];
+  const double dt = adaptive_dt(prev, d.dt_min, d.dt_max);
+  const int t0 = sk.y * blockDim.x + threadIdx.x, stride = d.nf * blockDim.x;
+  const int clear = ss->C_clear;
+  for (int c = t0; c < clear; c += stride) W.count[d.c0 + c] = 0;
   Red r;
   r.init();
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += stride) {
+  for (int i = d.a0 + t0; i < d.a1; i += stride) {
     double2 x;
     V2<T> v;
     verlet_agent(A, model, i, dt, do_reset != 0, x, v);
@@ -1214,19 +1347,18 @@ __global__ void __launch_bounds__(256) k_integrate(Agents<T> A, int model, Work 
   }
   r = block_reduce(r);
   store_partial(W, r);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    st->dt = dt;
-    st->time_tot += dt;
-    st->step += 1;
-    st->oob = 0;
+  if (sk.y == 0 && threadIdx.x == 0) {
+    ss->dt = dt;
+    ss->time_tot += dt;
+    ss->oob = 0;
   }
+  if (blockIdx.x == 0 && threadIdx.x == 0) W.st->step += 1;
 }
 
 // ---------------------------------------------------------------------------
 // Host helpers
 // ---------------------------------------------------------------------------
 
-inline int grid_for(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
 inline int check_launch() { return hipGetLastError() == hipSuccess ? CD_OK : CD_ERR_HIP; }
 
 int check_agents(const cd_agents_t *a) {
@@ -1253,31 +1385,28 @@ bool ws_sizes_ok(const cd_workspace_t *w) {
   return w && w->n >= 0 && w->n <= (int64_t)0x1fffffff && w->cell_capacity > 0 &&
          w->cell_capacity <= (int64_t)0x1fffffff && w->cell_slots > 0 &&
          w->cell_slots <= 4096 && w->cell_capacity * w->cell_slots <= ((int64_t)1 << 31) &&
+         w->max_scenarios >= 0 && w->max_scenarios <= (1 << 20) &&
+         (w->force_threads == 0 || w->force_threads == 64 || w->force_threads == 128 ||
+          w->force_threads == 256) &&
          (w->model == CD_CIRCULAR || w->model == CD_THREE_CIRCLE);
 }
 
 int check_ws(const cd_agents_t *a, const cd_workspace_t *w) {
   if (!ws_sizes_ok(w) || !w->ptr) return CD_ERR_ARG;
   if (w->n != a->n || w->model != a->model) return CD_ERR_WORKSPACE;
-  const Layout L = make_layout(w->n, w->cell_capacity, w->cell_slots, w->model);
-  if (w->bytes < L.total) return CD_ERR_WORKSPACE;
+  if (w->bytes < make_layout(w).total) return CD_ERR_WORKSPACE;
   return CD_OK;
 }
 
-constexpr int kBS = 256;
-int g_force_block = 256;  // threads per block of the force kernel (4 per agent)
-
-inline int part_blocks(int64_t n) { return std::max(1, std::min(grid_for(n, kBS), kMaxParts)); }
-// blocks of a kernel that combines the partials in every block
-constexpr int kMaxConsumerBlocks = 1024;
-inline int consumer_blocks(int64_t n) {
-  return std::max(1, std::min(grid_for(n, kBS), kMaxConsumerBlocks));
+// threads per block of the force kernel (4 per agent)
+inline int force_threads(const cd_workspace_t *w) {
+  return w->force_threads > 0 ? (int)w->force_threads : 4 * kAgentsPerForceBlock;
 }
 
 template <typename T>
-int launch_reduce(const Agents<T> &A, const Work &W, bool clear_counts, hipStream_t s) {
-  hipLaunchKernelGGL(k_reduce<T>, dim3(part_blocks(A.n)), dim3(kBS), 0, s, A, W,
-                     clear_counts ? 1 : 0);
+int launch_reduce(const Agents<T> &A, const Work &W, const Batch &B, int blocks,
+                  bool clear_counts, hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce<T>, dim3(blocks), dim3(kBS), 0, s, A, W, B, clear_counts ? 1 : 0);
   return check_launch();
 }
 
@@ -1285,71 +1414,47 @@ inline void mark(hipEvent_t *ev, int k, hipStream_t s) {
   if (ev) (void)hipEventRecord(ev[k], s);
 }
 
-// K_a + K_b (slot-list binning and gather).
-template <typename T>
-int launch_ell(const Agents<T> &A, const Work &W, double cell, bool binning, hipStream_t s,
-               hipEvent_t *ev = nullptr) {
-  mark(ev, 0, s);
-  hipLaunchKernelGGL(k_bin_insert, dim3(consumer_blocks(A.n)), dim3(kBS), 0, s,
-                     (const double2 *)A.pos, A.n, cell, W, binning ? 1 : 0);
-  mark(ev, 1, s);
-  if (binning)
-    hipLaunchKernelGGL(k_cell_gather<T>, dim3(grid_for(W.cap, kBS)), dim3(kBS), 0, s, A, W);
-  mark(ev, 2, s);
-  return check_launch();
+// Grid sizes of one step: per-scenario kernels (f) and force kernel (e).
+struct Launch {
+  int blocks_f, blocks_e, threads_e;
+};
+
+Launch single_launch(const cd_workspace_t *w) {
+  Launch L;
+  L.blocks_f = single_parts(w->n);
+  L.blocks_e = std::max(1, grid_for(w->n, force_threads(w) / 4));
+  L.threads_e = force_threads(w);
+  return L;
 }
 
 template <typename T>
-int launch_forces(const cd_agents_t *a, const Work &W, const FusedArgs &fa, bool fused,
-                  hipStream_t s) {
+int launch_forces(const cd_agents_t *a, const Work &W, const Batch &B, const FusedArgs &fa,
+                  bool fused, const Launch &L, hipStream_t s) {
   const Agents<T> A = typed<T>(a);
-  const int threads = 4 * (int)a->n;
-  const dim3 grid(std::max(1, grid_for(threads, g_force_block))), block(g_force_block);
+  const dim3 grid(L.blocks_e), block(L.threads_e);
   if (a->model == CD_THREE_CIRCLE) {
     if (fused)
-      hipLaunchKernelGGL((k_forces<T, 1, true>), grid, block, 0, s, A, W, fa);
+      hipLaunchKernelGGL((k_forces<T, 1, true>), grid, block, 0, s, A, W, B, fa);
     else
-      hipLaunchKernelGGL((k_forces<T, 1, false>), grid, block, 0, s, A, W, fa);
+      hipLaunchKernelGGL((k_forces<T, 1, false>), grid, block, 0, s, A, W, B, fa);
   } else {
     if (fused)
-      hipLaunchKernelGGL((k_forces<T, 0, true>), grid, block, 0, s, A, W, fa);
+      hipLaunchKernelGGL((k_forces<T, 0, true>), grid, block, 0, s, A, W, B, fa);
     else
-      hipLaunchKernelGGL((k_forces<T, 0, false>), grid, block, 0, s, A, W, fa);
+      hipLaunchKernelGGL((k_forces<T, 0, false>), grid, block, 0, s, A, W, B, fa);
   }
   return check_launch();
 }
 
+// One step: K_a, K_b, K_e, K_f (events ev[0..4] around them when timed).
 template <typename T>
-int do_agent_agent(const cd_agents_t *a, double cell, const cd_workspace_t *w, hipStream_t s) {
+int step_kernels(const cd_agents_t *a, const cd_step_params_t *p, const Work &W, const Batch &B,
+                 const Launch &L, hipStream_t s, hipEvent_t *ev) {
   const Agents<T> A = typed<T>(a);
-  const Work W = carve(w);
-  int rc;
-  if ((rc = launch_reduce(A, W, true, s))) return rc;
-  if ((rc = launch_ell(A, W, cell, true, s))) return rc;
-  FusedArgs fa;
-  memset(&fa, 0, sizeof(fa));
-  return launch_forces<T>(a, W, fa, false, s);
-}
-
-template <typename T>
-int do_add_to_cells(const cd_agents_t *a, double cell, const cd_workspace_t *w, hipStream_t s) {
-  const Agents<T> A = typed<T>(a);
-  const Work W = carve(w);
-  int rc;
-  if ((rc = launch_reduce(A, W, true, s))) return rc;
-  hipLaunchKernelGGL(k_bl_count, dim3(consumer_blocks(A.n)), dim3(kBS), 0, s,
-                     (const double2 *)A.pos, A.n, cell, W);
-  hipLaunchKernelGGL(k_bl_scan, dim3(W.ntiles), dim3(kScanThreads), 0, s, W);
-  hipLaunchKernelGGL(k_bl_scatter, dim3(grid_for(A.n, kBS)), dim3(kBS), 0, s, A.n, W);
-  hipLaunchKernelGGL(k_bl_sort, dim3(grid_for(W.cap, kBS)), dim3(kBS), 0, s, W);
-  return check_launch();
-}
-
-FusedArgs fused_args(const cd_step_params_t *p) {
+  const bool pairs = (p->flags & CD_USE_AGENT_AGENT) != 0;
   FusedArgs fa;
   fa.flags = p->flags;
   fa.nav_mode = p->nav_mode;
-  fa.seed = p->seed;
   fa.maps = p->nav_maps;
   fa.nav_T = p->nav_targets;
   fa.nav_rows = p->nav_rows;
@@ -1358,23 +1463,54 @@ FusedArgs fused_args(const cd_step_params_t *p) {
   fa.nav_miny = p->nav_miny;
   fa.nav_step = p->nav_step;
   fa.segs = (const double4 *)p->segments;
-  fa.W = (p->flags & CD_USE_OBSTACLE) ? (int)p->n_segments : 0;
-  return fa;
+  mark(ev, 0, s);
+  hipLaunchKernelGGL(k_bin_insert, dim3(L.blocks_f), dim3(kBS), 0, s, (const double2 *)A.pos,
+                     p->cell_size, W, B, pairs ? 1 : 0);
+  mark(ev, 1, s);
+  if (pairs)
+    hipLaunchKernelGGL(k_cell_gather<T>, dim3(grid_for(W.cap, kBS)), dim3(kBS), 0, s, A, W);
+  mark(ev, 2, s);
+  int rc = launch_forces<T>(a, W, B, fa, true, L, s);
+  if (rc) return rc;
+  mark(ev, 3, s);
+  hipLaunchKernelGGL(k_integrate<T>, dim3(L.blocks_f), dim3(kBS), 0, s, A, (int)a->model, W, B,
+                     (p->flags & CD_USE_RESET) ? 1 : 0);
+  mark(ev, 4, s);
+  return check_launch();
 }
 
 template <typename T>
-int step_impl(const cd_agents_t *a, const cd_step_params_t *p, const cd_workspace_t *w,
-              hipStream_t s, hipEvent_t *ev = nullptr) {
+int do_agent_agent(const cd_agents_t *a, double cell, const cd_workspace_t *w, hipStream_t s) {
   const Agents<T> A = typed<T>(a);
   const Work W = carve(w);
-  const bool pairs = (p->flags & CD_USE_AGENT_AGENT) != 0;
+  const Batch B = single_batch(w, 0, 0, 0.0, 0.0);
+  const Launch L = single_launch(w);
   int rc;
-  if ((rc = launch_ell(A, W, p->cell_size, pairs, s, ev))) return rc;
-  if ((rc = launch_forces<T>(a, W, fused_args(p), true, s))) return rc;
-  mark(ev, 3, s);
-  hipLaunchKernelGGL(k_integrate<T>, dim3(part_blocks(A.n)), dim3(kBS), 0, s, A, (int)a->model,
-                     W, p->dt_min, p->dt_max, (p->flags & CD_USE_RESET) ? 1 : 0);
-  mark(ev, 4, s);
+  if ((rc = launch_reduce(A, W, B, L.blocks_f, true, s))) return rc;
+  hipLaunchKernelGGL(k_bin_insert, dim3(L.blocks_f), dim3(kBS), 0, s, (const double2 *)A.pos,
+                     cell, W, B, 1);
+  hipLaunchKernelGGL(k_cell_gather<T>, dim3(grid_for(W.cap, kBS)), dim3(kBS), 0, s, A, W);
+  if ((rc = check_launch())) return rc;
+  FusedArgs fa;
+  memset(&fa, 0, sizeof(fa));
+  return launch_forces<T>(a, W, B, fa, false, L, s);
+}
+
+template <typename T>
+int do_add_to_cells(const cd_agents_t *a, double cell, const cd_workspace_t *w, hipStream_t s) {
+  const Agents<T> A = typed<T>(a);
+  const Work W = carve(w);
+  const Batch B = single_batch(w, 0, 0, 0.0, 0.0);
+  const Launch L = single_launch(w);
+  int rc;
+  if ((rc = launch_reduce(A, W, B, L.blocks_f, true, s))) return rc;
+  hipLaunchKernelGGL(k_bl_count, dim3(L.blocks_f), dim3(kBS), 0, s, (const double2 *)A.pos,
+                     cell, W, B);
+  hipLaunchKernelGGL(k_bl_scan, dim3(W.ntiles), dim3(kScanThreads), 0, s, W, B.sst);
+  hipLaunchKernelGGL(k_bl_scatter, dim3(grid_for(A.n, kBS)), dim3(kBS), 0, s, A.n, W,
+                     (const ScnState *)B.sst);
+  hipLaunchKernelGGL(k_bl_sort, dim3(grid_for(W.cap, kBS)), dim3(kBS), 0, s, W,
+                     (const ScnState *)B.sst);
   return check_launch();
 }
 
@@ -1394,12 +1530,75 @@ int check_step(const cd_agents_t *a, const cd_step_params_t *p, const cd_workspa
   return check_ws(a, w);
 }
 
+// Scenario table and block maps of a batch (host side).
+struct HostBatch {
+  std::vector<ScnDesc> desc;
+  std::vector<int2> map_f, map_e;
+};
+
+int build_batch(const cd_scenario_t *sc, int S, const cd_workspace_t *w, int64_t n_segments,
+                HostBatch &H) {
+  H.desc.resize(S);
+  int f = 0;
+  int64_t prev_end = 0;
+  for (int s = 0; s < S; ++s) {
+    const cd_scenario_t &c = sc[s];
+    if (c.agent_begin < prev_end || c.agent_end < c.agent_begin || c.agent_end > w->n ||
+        c.cell_begin < 0 || c.cell_capacity <= 0 ||
+        c.cell_begin + c.cell_capacity > w->cell_capacity || c.seg_begin < 0 ||
+        c.seg_end < c.seg_begin || c.seg_end > n_segments || !(c.dt_min >= 0.0) ||
+        !(c.dt_max >= 0.0))
+      return CD_ERR_ARG;
+    prev_end = c.agent_end;
+    ScnDesc &d = H.desc[s];
+    d.a0 = (int)c.agent_begin;
+    d.a1 = (int)c.agent_end;
+    d.c0 = (int)c.cell_begin;
+    d.ccap = (int)c.cell_capacity;
+    d.w0 = (int)c.seg_begin;
+    d.w1 = (int)c.seg_end;
+    d.seed = c.seed;
+    d.dt_min = c.dt_min;
+    d.dt_max = c.dt_max;
+    const int64_t ns = c.agent_end - c.agent_begin;
+    d.nf = std::max(1, std::min(grid_for(ns, kBS), kMaxPartsPerScn));
+    d.f0 = f;
+    for (int k = 0; k < d.nf; ++k) H.map_f.push_back(make_int2(s, k));
+    f += d.nf;
+    const int ne = grid_for(ns, force_threads(w) / 4);
+    for (int k = 0; k < ne; ++k) H.map_e.push_back(make_int2(s, k));
+  }
+  const int64_t Smax = std::max<int64_t>(1, w->max_scenarios);
+  if ((int64_t)H.map_f.size() > max_blocks_f(w->n, Smax) ||
+      (int64_t)H.map_e.size() > max_blocks_e(w->n, Smax))
+    return CD_ERR_WORKSPACE;
+  return CD_OK;
+}
+
 __global__ void k_clear_status(DevState *st) { st->status = 0; }
 __global__ void k_set_step(DevState *st, unsigned long long step) { st->step = step; }
 __global__ void k_philox(const uint32_t *in, uint32_t *out) {
   uint32_t c[4] = {in[0], in[1], in[2], in[3]};
   philox4x32(c, in[4], in[5]);
   for (int k = 0; k < 4; ++k) out[k] = c[k];
+}
+
+void to_state(const DevState &g, const ScnState &h, cd_state_t *out) {
+  out->dt = h.dt;
+  out->time_tot = h.time_tot;
+  out->origin[0] = h.origin[0];
+  out->origin[1] = h.origin[1];
+  out->shape[0] = h.nx;
+  out->shape[1] = h.ny;
+  out->status = g.status;
+  out->step = (int64_t)g.step;
+  double vm;
+  memcpy(&vm, &h.red[R_VMAX], 8);
+  out->v_max = vm;
+  unsigned long long b = h.red[R_V0MAX];
+  b = (b & 0x8000000000000000ull) ? (b & 0x7fffffffffffffffull) : ~b;
+  memcpy(&vm, &b, 8);
+  out->v0_max = vm;
 }
 
 }  // namespace
@@ -1412,7 +1611,7 @@ extern "C" {
 
 int64_t cd_workspace_bytes(const cd_workspace_t *w, cd_layout_t *layout) {
   if (!ws_sizes_ok(w)) return -1;
-  const Layout L = make_layout(w->n, w->cell_capacity, w->cell_slots, w->model);
+  const Layout L = make_layout(w);
   if (layout) {
     layout->points_indices = L.pidx;
     layout->cells_count = L.count;
@@ -1425,37 +1624,36 @@ int64_t cd_workspace_bytes(const cd_workspace_t *w, cd_layout_t *layout) {
 
 int cd_workspace_init(const cd_workspace_t *w, cd_stream_t stream) {
   if (!ws_sizes_ok(w) || !w->ptr) return CD_ERR_ARG;
-  const Layout L = make_layout(w->n, w->cell_capacity, w->cell_slots, w->model);
+  const Layout L = make_layout(w);
   if (w->bytes < L.total) return CD_ERR_WORKSPACE;
   const Work W = carve(w);
-  hipLaunchKernelGGL(k_ws_init, dim3(std::min(grid_for(w->cell_capacity + 1, kBS), 4096)),
-                     dim3(kBS), 0, (hipStream_t)stream, W);
+  const int S = (int)std::max<int64_t>(1, w->max_scenarios);
+  hipLaunchKernelGGL(k_ws_init, dim3(std::min(grid_for(w->cell_capacity + S + 1, kBS), 4096)),
+                     dim3(kBS), 0, (hipStream_t)stream, W, (ScnState *)((char *)w->ptr + L.scn_state),
+                     S);
   return check_launch();
 }
 
-int cd_read_state(const cd_workspace_t *w, cd_state_t *out, cd_stream_t stream) {
-  if (!w || !w->ptr || !out) return CD_ERR_ARG;
-  DevState h;
-  if (hipMemcpyAsync(&h, w->ptr, sizeof(DevState), hipMemcpyDeviceToHost,
-                     (hipStream_t)stream) != hipSuccess)
+int cd_batch_read_states(const cd_workspace_t *w, cd_state_t *out, int32_t count,
+                         cd_stream_t stream) {
+  if (!w || !w->ptr || !out || count < 1 || count > std::max<int64_t>(1, w->max_scenarios))
+    return CD_ERR_ARG;
+  const Layout L = make_layout(w);
+  DevState g;
+  std::vector<ScnState> h(count);
+  hipStream_t s = (hipStream_t)stream;
+  if (hipMemcpyAsync(&g, w->ptr, sizeof(DevState), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipMemcpyAsync(h.data(), (char *)w->ptr + L.scn_state, sizeof(ScnState) * count,
+                     hipMemcpyDeviceToHost, s) != hipSuccess)
     return CD_ERR_HIP;
-  if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess) return CD_ERR_HIP;
-  out->dt = h.dt;
-  out->time_tot = h.time_tot;
-  out->origin[0] = h.origin[0];
-  out->origin[1] = h.origin[1];
-  out->shape[0] = h.nx;
-  out->shape[1] = h.ny;
-  out->status = h.status;
-  out->step = (int64_t)h.step;
-  double vm;
-  memcpy(&vm, &h.red[R_VMAX], 8);
-  out->v_max = vm;
-  unsigned long long b = h.red[R_V0MAX];
-  b = (b & 0x8000000000000000ull) ? (b & 0x7fffffffffffffffull) : ~b;
-  memcpy(&vm, &b, 8);
-  out->v0_max = vm;
+  if (hipStreamSynchronize(s) != hipSuccess) return CD_ERR_HIP;
+  for (int k = 0; k < count; ++k) to_state(g, h[k], &out[k]);
   return CD_OK;
+}
+
+int cd_read_state(const cd_workspace_t *w, cd_state_t *out, cd_stream_t stream) {
+  if (!ws_sizes_ok(w)) return CD_ERR_ARG;
+  return cd_batch_read_states(w, out, 1, stream);
 }
 
 int cd_clear_status(const cd_workspace_t *w, cd_stream_t stream) {
@@ -1575,16 +1773,18 @@ int cd_velocity_verlet_integrator(const cd_agents_t *a, double dt_min, double dt
   if (a->n == 0) return CD_OK;
   hipStream_t s = (hipStream_t)stream;
   const Work W = carve(w);
+  const Batch B = single_batch(w, 0, 0, dt_min, dt_max);
+  const Launch L = single_launch(w);
   if (a->real == CD_F64) {
     const Agents<double> A = typed<double>(a);
-    if ((rc = launch_reduce(A, W, false, s))) return rc;
-    hipLaunchKernelGGL(k_integrate_node<double>, dim3(consumer_blocks(A.n)), dim3(kBS), 0, s,
-                       A, (int)a->model, W, dt_min, dt_max);
+    if ((rc = launch_reduce(A, W, B, L.blocks_f, false, s))) return rc;
+    hipLaunchKernelGGL(k_integrate_node<double>, dim3(L.blocks_f), dim3(kBS), 0, s, A,
+                       (int)a->model, W, B);
   } else {
     const Agents<float> A = typed<float>(a);
-    if ((rc = launch_reduce(A, W, false, s))) return rc;
-    hipLaunchKernelGGL(k_integrate_node<float>, dim3(consumer_blocks(A.n)), dim3(kBS), 0, s,
-                       A, (int)a->model, W, dt_min, dt_max);
+    if ((rc = launch_reduce(A, W, B, L.blocks_f, false, s))) return rc;
+    hipLaunchKernelGGL(k_integrate_node<float>, dim3(L.blocks_f), dim3(kBS), 0, s, A,
+                       (int)a->model, W, B);
   }
   return check_launch();
 }
@@ -1611,8 +1811,10 @@ int cd_step_prepare(const cd_agents_t *a, const cd_step_params_t *p, const cd_wo
   hipLaunchKernelGGL(k_set_step, dim3(1), dim3(1), 0, s, W.st, (unsigned long long)first_step);
   if ((rc = check_launch())) return rc;
   if (a->n == 0) return CD_OK;
-  if (a->real == CD_F64) return launch_reduce(typed<double>(a), W, true, s);
-  return launch_reduce(typed<float>(a), W, true, s);
+  const Batch B = single_batch(w, p->n_segments, p->seed, p->dt_min, p->dt_max);
+  const Launch L = single_launch(w);
+  if (a->real == CD_F64) return launch_reduce(typed<double>(a), W, B, L.blocks_f, true, s);
+  return launch_reduce(typed<float>(a), W, B, L.blocks_f, true, s);
 }
 
 int cd_step(const cd_agents_t *a, const cd_step_params_t *p, const cd_workspace_t *w,
@@ -1620,8 +1822,60 @@ int cd_step(const cd_agents_t *a, const cd_step_params_t *p, const cd_workspace_
   int rc = check_step(a, p, w);
   if (rc) return rc;
   if (a->n == 0) return CD_OK;
-  if (a->real == CD_F64) return step_impl<double>(a, p, w, (hipStream_t)stream);
-  return step_impl<float>(a, p, w, (hipStream_t)stream);
+  const Work W = carve(w);
+  const Batch B = single_batch(w, (p->flags & CD_USE_OBSTACLE) ? p->n_segments : 0, p->seed,
+                               p->dt_min, p->dt_max);
+  const Launch L = single_launch(w);
+  if (a->real == CD_F64)
+    return step_kernels<double>(a, p, W, B, L, (hipStream_t)stream, nullptr);
+  return step_kernels<float>(a, p, W, B, L, (hipStream_t)stream, nullptr);
+}
+
+int cd_batch_prepare(const cd_agents_t *a, const cd_step_params_t *p,
+                     const cd_scenario_t *scenarios, int32_t n_scenarios, cd_workspace_t *w,
+                     uint64_t first_step, cd_stream_t stream) {
+  int rc = check_step(a, p, w);
+  if (rc) return rc;
+  if (!scenarios || n_scenarios < 1 || n_scenarios > w->max_scenarios) return CD_ERR_ARG;
+  HostBatch H;
+  if ((rc = build_batch(scenarios, n_scenarios, w, p->n_segments, H))) return rc;
+  const Layout Lw = make_layout(w);
+  char *b = (char *)w->ptr;
+  hipStream_t s = (hipStream_t)stream;
+  if (hipMemcpyAsync(b + Lw.scn_desc, H.desc.data(), sizeof(ScnDesc) * H.desc.size(),
+                     hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(b + Lw.map_f, H.map_f.data(), sizeof(int2) * H.map_f.size(),
+                     hipMemcpyHostToDevice, s) != hipSuccess ||
+      (!H.map_e.empty() &&
+       hipMemcpyAsync(b + Lw.map_e, H.map_e.data(), sizeof(int2) * H.map_e.size(),
+                      hipMemcpyHostToDevice, s) != hipSuccess) ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return CD_ERR_HIP;
+  w->n_scenarios = n_scenarios;
+  w->blocks_f = (int32_t)H.map_f.size();
+  w->blocks_e = (int32_t)H.map_e.size();
+  const Work W = carve(w);
+  const Batch B = batch_of(w);
+  hipLaunchKernelGGL(k_set_step, dim3(1), dim3(1), 0, s, W.st, (unsigned long long)first_step);
+  if ((rc = check_launch())) return rc;
+  if (a->real == CD_F64) return launch_reduce(typed<double>(a), W, B, w->blocks_f, true, s);
+  return launch_reduce(typed<float>(a), W, B, w->blocks_f, true, s);
+}
+
+int cd_batch_step(const cd_agents_t *a, const cd_step_params_t *p, const cd_workspace_t *w,
+                  cd_stream_t stream) {
+  int rc = check_step(a, p, w);
+  if (rc) return rc;
+  if (w->n_scenarios < 1 || w->blocks_f < 1) return CD_ERR_ARG;
+  const Work W = carve(w);
+  const Batch B = batch_of(w);
+  Launch L = single_launch(w);
+  L.blocks_f = w->blocks_f;
+  L.blocks_e = std::max(1, (int)w->blocks_e);
+  if (w->blocks_e < 1) return CD_OK;  // no agents
+  if (a->real == CD_F64)
+    return step_kernels<double>(a, p, W, B, L, (hipStream_t)stream, nullptr);
+  return step_kernels<float>(a, p, W, B, L, (hipStream_t)stream, nullptr);
 }
 
 int cd_step_timed(const cd_agents_t *a, const cd_step_params_t *p, const cd_workspace_t *w,
@@ -1632,11 +1886,22 @@ int cd_step_timed(const cd_agents_t *a, const cd_step_params_t *p, const cd_work
   for (int k = 0; k < 4; ++k) kernel_ms[k] = 0.0;
   if (a->n == 0 || steps == 0) return CD_OK;
   hipStream_t s = (hipStream_t)stream;
+  const Work W = carve(w);
+  const bool batched = w->n_scenarios >= 1 && w->blocks_f >= 1;
+  const Batch B = batched ? batch_of(w)
+                          : single_batch(w, (p->flags & CD_USE_OBSTACLE) ? p->n_segments : 0,
+                                         p->seed, p->dt_min, p->dt_max);
+  Launch L = single_launch(w);
+  if (batched) {
+    L.blocks_f = w->blocks_f;
+    L.blocks_e = w->blocks_e;
+  }
   hipEvent_t ev[5];
   for (int k = 0; k < 5; ++k)
     if (hipEventCreate(&ev[k]) != hipSuccess) return CD_ERR_HIP;
   for (int64_t it = 0; it < steps && rc == CD_OK; ++it) {
-    rc = a->real == CD_F64 ? step_impl<double>(a, p, w, s, ev) : step_impl<float>(a, p, w, s, ev);
+    rc = a->real == CD_F64 ? step_kernels<double>(a, p, W, B, L, s, ev)
+                           : step_kernels<float>(a, p, W, B, L, s, ev);
     if (rc) break;
     if (hipEventSynchronize(ev[4]) != hipSuccess) {
       rc = CD_ERR_HIP;
@@ -1688,6 +1953,6 @@ int cd_philox4x32(const uint32_t *in, uint32_t *out, cd_stream_t stream) {
   return check_launch();
 }
 
-const char *cd_version(void) { return "cdamd 0.2 gfx950"; }
+const char *cd_version(void) { return "cdamd 0.3 gfx950"; }
 
 }  // extern "C"

@@ -120,7 +120,8 @@ def min_body_radius(array):
 class Workspace:
     """Caller-owned workspace of the C ABI (slot lists, block lists, state)."""
 
-    def __init__(self, n, cell_capacity, model, cell_slots=16, device=None, stream=None):
+    def __init__(self, n, cell_capacity, model, cell_slots=16, device=None, stream=None,
+                 max_scenarios=0, force_threads=0):
         require_gpu()
         L = _lib.lib()
         self.desc = _lib.CDWorkspace()
@@ -129,6 +130,8 @@ class Workspace:
         d.cell_capacity = int(cell_capacity)
         d.cell_slots = int(cell_slots)
         d.model = int(model)
+        d.max_scenarios = int(max_scenarios)
+        d.force_threads = int(force_threads)
         self.layout = _lib.CDLayout()
         nbytes = L.cd_workspace_bytes(C.byref(d), C.byref(self.layout))
         if nbytes <= 0:
@@ -154,6 +157,13 @@ class Workspace:
         _lib.check(_lib.lib().cd_read_state(self.ref(), C.byref(st), stream_handle(stream)),
                    'read_state')
         return st
+
+    def states(self, count, stream=None):
+        """Synchronous read of the device scalars of scenarios [0, count)."""
+        out = (_lib.CDState * int(count))()
+        _lib.check(_lib.lib().cd_batch_read_states(self.ref(), out, int(count),
+                                                   stream_handle(stream)), 'batch_read_states')
+        return list(out)
 
     def clear_status(self, stream=None):
         _lib.check(_lib.lib().cd_clear_status(self.ref(), stream_handle(stream)), 'clear_status')

@@ -1,4 +1,4 @@
-"""Fused multi-step driver over cd_step (six kernels per step) with HIP
+"""Fused multi-step driver over cd_step (four kernels per step) with HIP
 graph capture.
 
 ``Stepper`` runs one MultiAgentSimulation.update() of the standard logic
@@ -31,11 +31,32 @@ class GridOverflow(CrowdDynamicsException):
     """The binning grid outgrew the workspace's cell capacity."""
 
 
+def make_params(cell_size, dt_min, dt_max, flags, seed, segments, maps, nav):
+    """cd_step_params_t of a step; `segments` [W, 4] and `maps` are device
+    tensors that must outlive the parameters."""
+    p = _lib.CDStepParams()
+    p.cell_size = float(cell_size)
+    p.dt_min = float(dt_min)
+    p.dt_max = float(dt_max)
+    p.flags = int(flags)
+    p.seed = int(seed)
+    p.segments = C.c_void_p(segments.data_ptr() if len(segments) else 0)
+    p.n_segments = len(segments)
+    if maps is not None:
+        p.nav_mode = int(nav.get('mode', 0))
+        p.nav_maps = C.c_void_p(maps.data_ptr())
+        p.nav_targets, p.nav_rows, p.nav_cols = maps.shape[:3]
+        p.nav_minx = float(nav['minx'])
+        p.nav_miny = float(nav['miny'])
+        p.nav_step = float(nav['step'])
+    return p
+
+
 class Stepper:
     def __init__(self, array, segments=None, nav=None, cell_size=3.6, dt_min=0.01,
                  dt_max=0.01, seed=1, flags=ALL_NODES, precision='f64', cell_capacity=None,
                  bounds=None, first_step=0, device=None, device_agents=None,
-                 cell_slots=None):
+                 cell_slots=None, force_threads=0):
         if device_agents is not None:
             self.agents = device_agents
         else:
@@ -65,23 +86,9 @@ class Stepper:
         if cell_slots is None:
             cell_slots = cell_slots_for(cell_size, min_body_radius(array))
         self.ws = Workspace(self.n, cell_capacity, self.agents.model, cell_slots=cell_slots,
-                            device=dev)
-        p = _lib.CDStepParams()
-        p.cell_size = float(cell_size)
-        p.dt_min = float(dt_min)
-        p.dt_max = float(dt_max)
-        p.flags = int(flags)
-        p.seed = int(seed)
-        p.segments = C.c_void_p(self.segments.data_ptr() if len(self.segments) else 0)
-        p.n_segments = len(self.segments)
-        if self.maps is not None:
-            p.nav_mode = int(nav.get('mode', 0))
-            p.nav_maps = C.c_void_p(self.maps.data_ptr())
-            p.nav_targets, p.nav_rows, p.nav_cols = self.maps.shape[:3]
-            p.nav_minx = float(nav['minx'])
-            p.nav_miny = float(nav['miny'])
-            p.nav_step = float(nav['step'])
-        self.params = p
+                            device=dev, force_threads=force_threads)
+        self.params = make_params(cell_size, dt_min, dt_max, flags, seed, self.segments,
+                                  self.maps, nav)
         self.flags = flags
         self.prepare(first_step)
 
@@ -92,12 +99,13 @@ class Stepper:
                                               self.ws.ref(), int(first_step), stream_handle()),
                    'step_prepare')
 
+    _entry = 'cd_step'
+
     def step(self, k=1):
-        L = _lib.lib()
+        f = getattr(_lib.lib(), self._entry)
         s = stream_handle()
         for _ in range(k):
-            _lib.check(L.cd_step(self.agents.ref(), C.byref(self.params), self.ws.ref(), s),
-                       'step')
+            _lib.check(f(self.agents.ref(), C.byref(self.params), self.ws.ref(), s), 'step')
 
     def capture(self, k):
         """HIP graph of k consecutive steps (launched on the current stream)."""

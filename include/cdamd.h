@@ -72,12 +72,29 @@ typedef struct cd_agents {
 typedef struct cd_workspace {
   void *ptr;
   int64_t bytes;
-  int64_t n;
-  int64_t cell_capacity;
+  int64_t n;             /* agents (all scenarios of a batch together) */
+  int64_t cell_capacity; /* grid cells (all scenarios of a batch together) */
   int64_t cell_slots;
   int32_t model;
-  int32_t _pad;
+  int32_t max_scenarios; /* scenario table size (0 or 1: single scenario) */
+  /* set by cd_batch_prepare: */
+  int32_t n_scenarios, blocks_f, blocks_e;
+  int32_t force_threads; /* threads per force block, 4 per agent: 64, 128 or 256 (0: 256) */
 } cd_workspace_t;
+
+/* One scenario of a batch: its agents are [agent_begin, agent_end) of the
+ * batch arrays, its grid cells [cell_begin, cell_begin + cell_capacity) of
+ * the workspace grid, its walls [seg_begin, seg_end) of params->segments.
+ * Each scenario has its own adaptive dt and its own fluctuation key `seed`
+ * (the noise counter is the agent's index within the scenario), so a batched
+ * run reproduces the scenarios' standalone runs. */
+typedef struct cd_scenario {
+  int64_t agent_begin, agent_end;
+  int64_t cell_begin, cell_capacity;
+  int64_t seg_begin, seg_end;
+  uint64_t seed;
+  double dt_min, dt_max;
+} cd_scenario_t;
 
 /* Host copy of the workspace state scalars. */
 typedef struct cd_state {
@@ -205,6 +222,23 @@ int cd_step_prepare(const cd_agents_t *agents, const cd_step_params_t *params,
                     const cd_workspace_t *ws, uint64_t first_step, cd_stream_t stream);
 int cd_step(const cd_agents_t *agents, const cd_step_params_t *params, const cd_workspace_t *ws,
             cd_stream_t stream);
+
+/* Batched ensemble: independent scenarios advanced together, one launch per
+ * step kernel over all scenarios (four launches per step, graph-capturable).
+ * Synchronous: uploads the scenario table and block maps into the
+ * workspace (which needs max_scenarios >= n_scenarios) and fills ws->
+ * n_scenarios / blocks_f / blocks_e.  params supplies the cell size, flags,
+ * the concatenated wall segments and the navigation maps shared by all
+ * scenarios; its seed, dt_min and dt_max are replaced per scenario.  No
+ * reference counterpart (the reference runs one simulation per process). */
+int cd_batch_prepare(const cd_agents_t *agents, const cd_step_params_t *params,
+                     const cd_scenario_t *scenarios, int32_t n_scenarios, cd_workspace_t *ws,
+                     uint64_t first_step, cd_stream_t stream);
+int cd_batch_step(const cd_agents_t *agents, const cd_step_params_t *params,
+                  const cd_workspace_t *ws, cd_stream_t stream);
+/* Synchronous: state scalars of scenarios [0, count). */
+int cd_batch_read_states(const cd_workspace_t *ws, cd_state_t *out, int32_t count,
+                         cd_stream_t stream);
 
 /* Synchronous: runs `steps` cd_step calls and returns in kernel_ms[4] the
  * summed GPU time of each step kernel (bin/reduce, sort+gather, forces,

@@ -62,3 +62,55 @@ def test_argument_validation_without_device():
     assert L.cd_force_adjust_agents(ctypes.byref(a), None) == _lib.CD_ERR_UNSUPPORTED
     a.model, a.real = 0, 0
     assert L.cd_force_adjust_agents(ctypes.byref(a), None) == _lib.CD_ERR_ARG
+
+
+def _fake_circular(n):
+    """Descriptor with non-null placeholder pointers (never dereferenced:
+    every call below fails validation before touching the device)."""
+    a = _lib.CDAgents()
+    a.n, a.model, a.real = n, 0, 0
+    for f in ('position', 'velocity', 'target_direction', 'force', 'force_prev', 'radius',
+              'mass', 'target_velocity', 'tau_adj', 'k_soc', 'tau_0', 'mu', 'kappa',
+              'damping', 'std_rand_force', 'target'):
+        setattr(a, f, 0x1000)
+    return a
+
+
+def test_batch_table_validation_without_device():
+    """cd_batch_prepare rejects inconsistent scenario tables on the host."""
+    L = _lib.lib()
+    a = _fake_circular(100)
+    w = _lib.CDWorkspace()
+    w.n, w.cell_capacity, w.cell_slots, w.model, w.max_scenarios = 100, 400, 8, 0, 3
+    w.bytes = L.cd_workspace_bytes(ctypes.byref(w), None)
+    assert w.bytes > 0
+    w.ptr = 0x2000
+    p = _lib.CDStepParams()
+    p.cell_size, p.flags = 3.6, _lib.CD_USE_AGENT_AGENT
+
+    def table(rows):
+        t = (_lib.CDScenario * len(rows))()
+        for k, (a0, a1, c0, cc) in enumerate(rows):
+            t[k].agent_begin, t[k].agent_end = a0, a1
+            t[k].cell_begin, t[k].cell_capacity = c0, cc
+            t[k].dt_min, t[k].dt_max = 0.001, 0.01
+        return t
+
+    def prep(rows, count=None):
+        t = table(rows)
+        return L.cd_batch_prepare(ctypes.byref(a), ctypes.byref(p), t,
+                                  len(rows) if count is None else count,
+                                  ctypes.byref(w), 0, None)
+
+    bad = [
+        [(0, 50, 0, 100), (40, 100, 100, 100)],        # overlapping agent ranges
+        [(0, 50, 0, 100), (50, 101, 100, 100)],        # beyond the agent arrays
+        [(0, 50, 0, 100), (50, 100, 350, 100)],        # beyond the cell capacity
+        [(0, 50, 0, 0), (50, 100, 100, 100)],          # empty cell range
+        [(0, 30, 0, 100), (30, 60, 100, 100), (60, 90, 200, 100), (90, 100, 300, 50)],
+    ]
+    for rows in bad:
+        assert prep(rows) == _lib.CD_ERR_ARG, rows
+    assert prep([(0, 100, 0, 400)], count=0) == _lib.CD_ERR_ARG
+    w.force_threads = 96
+    assert L.cd_workspace_bytes(ctypes.byref(w), None) == -1
