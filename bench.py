@@ -189,6 +189,63 @@ def bench_corridor(steps, stream):
                 kernel_ms={k: kms[i] / 20 for i, k in enumerate(KERNELS)})
 
 
+def bench_adjoint(n, steps, checkpoint):
+    """Differentiable rollout: n circular agents, fixed dt 0.005 s, loss =
+    mean squared distance to a target trajectory made by the same method
+    with tau_0 and kappa scaled by 1.05 and desired speeds by 1 + 0.05 N(0,1);
+    gradients for tau_0, mu, kappa, damping and the n desired speeds from
+    the adjoint with a checkpoint every `checkpoint` steps."""
+    import torch
+    from crowddynamics_amd import examples as E
+    from crowddynamics_amd.adjoint import Rollout, rollout_positions
+    log('adjoint: n=%d, %d steps, checkpoint %d' % (n, steps, checkpoint))
+    scn = E.rollout(n=n, steps=steps)
+    arr = scn.agents
+    R = Rollout(arr, steps=steps, dt=scn.dt_max, cell_size=scn.cell_size, checkpoint=checkpoint,
+                bounds=scn.bounds)
+    dev = 'cuda'
+    f64 = torch.float64
+
+    def params(grad=False):
+        ps = [torch.tensor(3.0), torch.tensor(1.2e5), torch.tensor(4e4), torch.tensor(500.0),
+              torch.as_tensor(np.array(arr['target_velocity'], dtype=np.float64))]
+        return [p.to(device=dev, dtype=f64).requires_grad_(grad) for p in ps]
+
+    rng = np.random.default_rng(0)
+    q = params()
+    q[0] = q[0] * 1.05
+    q[2] = q[2] * 1.05
+    q[4] = q[4] * torch.as_tensor(1.0 + 0.05 * rng.standard_normal(n), device=dev)
+    with torch.no_grad():
+        Xhat = rollout_positions(R, *q)
+    p = params(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    X = rollout_positions(R, *p)
+    L = ((X[1:] - Xhat[1:]) ** 2).sum(-1).mean()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    L.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    fwd, bwd = t1 - t0, t2 - t1
+    n_par = 4 + n
+    log('adjoint: forward %.3f ms/step, backward %.3f ms/step' % (1e3 * fwd / steps,
+                                                                   1e3 * bwd / steps))
+    return dict(agents=n, steps=steps, checkpoint=checkpoint, dt=scn.dt_max,
+                loss=float(L.detach()), forward_ms_per_step=1e3 * fwd / steps,
+                backward_ms_per_step=1e3 * bwd / steps,
+                adjoint_step_ms=1e3 * (fwd + bwd) / steps,
+                gradient_s=fwd + bwd,
+                grad_norms=dict(tau_0=float(p[0].grad), mu=float(p[1].grad),
+                                kappa=float(p[2].grad), damping=float(p[3].grad),
+                                v0_l2=float(torch.linalg.vector_norm(p[4].grad))),
+                fd_equivalent=dict(
+                    note='central differences need 2 forward rollouts per parameter',
+                    parameters=n_par, seconds=2 * n_par * fwd,
+                    speedup_of_adjoint=2 * n_par * fwd / (fwd + bwd)))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -204,6 +261,8 @@ def main():
     ap.add_argument('--ensemble-loop-steps', type=int, default=20)
     ap.add_argument('--no-ensemble', action='store_true')
     ap.add_argument('--no-corridor', action='store_true')
+    ap.add_argument('--no-adjoint', action='store_true')
+    ap.add_argument('--adjoint-steps', type=int, default=500)
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU: the per-step update runs on a single MI355X '
@@ -295,6 +354,11 @@ def main():
     if not args.no_corridor:
         corridor = bench_corridor(300, stream)
 
+    adjoint = None
+    if not args.no_adjoint:
+        with torch.cuda.stream(stream):
+            adjoint = bench_adjoint(4000, args.adjoint_steps, 25)
+
     cpu = None
     if not args.no_cpu:
         log('cpu baseline (oracle, 1 thread, %.0f s sample)' % args.cpu_seconds)
@@ -334,6 +398,7 @@ def main():
         'pair_tally': t0_tally,
         'ensemble': ensemble,
         'corridor_three_circle': corridor,
+        'adjoint': adjoint,
     }
     if cpu:
         line['speedup_vs_cpu_1core'] = r['value'] / cpu['value']

@@ -64,6 +64,11 @@ class CDScenario(C.Structure):
                 ('dt_min', C.c_double), ('dt_max', C.c_double)]
 
 
+class CDAdjoint(C.Structure):
+    _fields_ = [('lam_x', _vp), ('lam_v', _vp), ('lam_g', _vp), ('lam_f', _vp),
+                ('lam_vt', _vp), ('grad', _vp)]
+
+
 class CDState(C.Structure):
     _fields_ = [('dt', C.c_double), ('time_tot', C.c_double),
                 ('origin', C.c_double * 2), ('shape', C.c_int64 * 2),
@@ -110,6 +115,8 @@ SIGNATURES = {
                                    _W, C.c_uint64, _vp]),
     'cd_batch_step': (C.c_int, [_P(CDAgents), _P(CDStepParams), _W, _vp]),
     'cd_batch_read_states': (C.c_int, [_W, _P(CDState), C.c_int32, _vp]),
+    'cd_adjoint_step': (C.c_int, [_P(CDAgents), _P(CDStepParams), _vp, _P(CDAdjoint), _W,
+                                  _vp]),
     'cd_step_timed': (C.c_int, [_P(CDAgents), _P(CDStepParams), _W, C.c_int64,
                                 _P(C.c_double), _vp]),
     'cd_graph_begin': (C.c_int, [_vp]),

@@ -1,0 +1,189 @@
+"""Differentiable rollout: trajectories of the fused step as a
+``torch.autograd.Function`` whose backward pass is the hand-written adjoint
+(cd_adjoint_step), with timestep checkpointing.
+
+Scope: circular agents in fp64 at a fixed dt (dt_min = dt_max), with the
+Reset, Integrator, Adjusting, AgentAgentInteractions and (parameter-free)
+Fluctuation nodes of the standard tree
+(crowddynamics/examples/simulations.py:123-136 without walls and
+navigation).  Differentiable inputs: tau_0, mu, kappa, damping (one value
+for all agents, or one per agent) and the per-agent target_velocity.
+
+The forward pass stores the state (position, velocity, force_prev) every
+``checkpoint`` steps; the backward pass re-runs each segment from its
+checkpoint (bit-identical to the first run), keeps the segment's states and
+sweeps the adjoint back through it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from .device import stream_handle
+from .exceptions import InvalidValue
+from .stepper import Stepper
+
+ROLLOUT_NODES = _lib.CD_USE_AGENT_AGENT | _lib.CD_USE_ADJUST | _lib.CD_USE_RESET
+
+
+class Rollout:
+    """Fixed-dt rollout of `steps` steps from the records `array`."""
+
+    def __init__(self, array, steps, dt=0.005, cell_size=3.6, checkpoint=25, seed=1,
+                 fluctuation=False, cell_capacity=None, bounds=None, device=None):
+        if 'position_ls' in array.dtype.names:
+            raise InvalidValue('the adjoint covers circular agents')
+        if steps < 1 or checkpoint < 1:
+            raise InvalidValue('steps and checkpoint must be positive')
+        flags = ROLLOUT_NODES | (_lib.CD_USE_FLUCT if fluctuation else 0)
+        self.array0 = array.copy()
+        self.steps = int(steps)
+        self.checkpoint = int(checkpoint)
+        self.dt = float(dt)
+        self.st = Stepper(array.copy(), cell_size=cell_size, dt_min=dt, dt_max=dt, seed=seed,
+                          flags=flags, precision='f64', cell_capacity=cell_capacity,
+                          bounds=bounds, device=device)
+        self.agents = self.st.agents
+        self.n = self.agents.n
+        dev = self.agents.device
+        f64 = torch.float64
+        n_ck = (self.steps + self.checkpoint - 1) // self.checkpoint
+        self.ck = {f: torch.zeros((n_ck, self.n, 2), dtype=f64, device=dev)
+                   for f in ('position', 'velocity', 'force_prev')}
+        L = self.checkpoint + 1
+        self.tape_x = torch.zeros((L, self.n, 2), dtype=f64, device=dev)
+        self.tape_v = torch.zeros((L, self.n, 2), dtype=f64, device=dev)
+        self.adj = {k: torch.zeros((self.n, 2), dtype=f64, device=dev)
+                    for k in ('lam_x', 'lam_v', 'lam_g', 'lam_f', 'lam_vt')}
+        self.grad = torch.zeros((self.n, 5), dtype=f64, device=dev)
+        a = _lib.CDAdjoint()
+        for k, t in self.adj.items():
+            setattr(a, k, C.c_void_p(t.data_ptr()))
+        a.grad = C.c_void_p(self.grad.data_ptr())
+        self.adj_struct = a
+        self.view = _lib.CDAgents.from_buffer_copy(self.agents.struct)
+
+    # -- parameters and state -------------------------------------------
+    def set_parameters(self, tau_0=None, mu=None, kappa=None, damping=None, v0=None):
+        f = self.agents.fields
+        for name, val in (('tau_0', tau_0), ('mu', mu), ('kappa', kappa),
+                          ('damping', damping), ('target_velocity', v0)):
+            if val is None:
+                continue
+            t = torch.as_tensor(val, dtype=torch.float64, device=self.agents.device).detach()
+            if t.dim() == 0:
+                f[name].fill_(t)
+            elif t.shape == (self.n,):
+                f[name].copy_(t)
+            else:
+                raise InvalidValue('%s must be a scalar or one value per agent' % name)
+
+    def _restore_initial(self):
+        self.agents.upload(self.array0, fields=('position', 'velocity', 'force', 'force_prev',
+                                                'target_direction'))
+
+    def _restore(self, s):
+        f = self.agents.fields
+        for k, t in self.ck.items():
+            f[k].copy_(t[s])
+        f['force'].zero_()
+
+    # -- passes ------------------------------------------------------------
+    def forward(self):
+        """Run the rollout from the initial records; returns positions
+        [steps + 1, n, 2] (fp64, on the device) and stores checkpoints."""
+        self._restore_initial()
+        self.st.prepare(0)
+        f = self.agents.fields
+        X = torch.empty((self.steps + 1, self.n, 2), dtype=torch.float64,
+                        device=self.agents.device)
+        X[0].copy_(f['position'])
+        for k in range(self.steps):
+            if k % self.checkpoint == 0:
+                s = k // self.checkpoint
+                for name, t in self.ck.items():
+                    t[s].copy_(f[name])
+            self.st.step(1)
+            X[k + 1].copy_(f['position'])
+        self.st.check()
+        return X
+
+    def backward(self, grad_X):
+        """Adjoint sweep for dL/dX = grad_X [steps + 1, n, 2]; returns the
+        per-agent parameter gradients [n, 5] (tau_0, mu, kappa, damping,
+        target_velocity) and dL/d(position_0, velocity_0)."""
+        grad_X = grad_X.to(torch.float64).contiguous()
+        for t in self.adj.values():
+            t.zero_()
+        self.grad.zero_()
+        self.adj['lam_x'].copy_(grad_X[self.steps])
+        L = _lib.lib()
+        s_h = stream_handle()
+        f = self.agents.fields
+        n_seg = (self.steps + self.checkpoint - 1) // self.checkpoint
+        view = self.view
+        for s in reversed(range(n_seg)):
+            k0 = s * self.checkpoint
+            k1 = min(k0 + self.checkpoint, self.steps)
+            self._restore(s)
+            self.st.prepare(k0)
+            self.tape_x[0].copy_(f['position'])
+            self.tape_v[0].copy_(f['velocity'])
+            for k in range(k0, k1 - 1):  # states k0 .. k1 - 1 are needed
+                self.st.step(1)
+                self.tape_x[k + 1 - k0].copy_(f['position'])
+                self.tape_v[k + 1 - k0].copy_(f['velocity'])
+            for k in reversed(range(k0, k1)):
+                view.position = self.tape_x[k - k0].data_ptr()
+                view.velocity = self.tape_v[k - k0].data_ptr()
+                _lib.check(L.cd_adjoint_step(C.byref(view), C.byref(self.st.params),
+                                             C.c_void_p(grad_X[k].data_ptr()),
+                                             C.byref(self.adj_struct), self.st.ws.ref(), s_h),
+                           'adjoint_step')
+        self.st.check()
+        return self.grad.clone(), self.adj['lam_x'].clone(), self.adj['lam_v'].clone()
+
+    def adjoint_step(self, grad_x=None):
+        """One cd_adjoint_step from the agents' current position and
+        velocity, updating the adjoint buffers ``adj`` and ``grad`` in place."""
+        gx = C.c_void_p(grad_x.data_ptr()) if grad_x is not None else C.c_void_p(0)
+        _lib.check(_lib.lib().cd_adjoint_step(self.agents.ref(), C.byref(self.st.params), gx,
+                                              C.byref(self.adj_struct), self.st.ws.ref(),
+                                              stream_handle()), 'adjoint_step')
+
+
+class RolloutFunction(torch.autograd.Function):
+    """X = RolloutFunction.apply(rollout, tau_0, mu, kappa, damping, v0).
+
+    Each parameter is a 0-d tensor (one value for all agents) or a tensor of
+    one value per agent; v0 is per agent.  X holds the positions of steps
+    0..steps, [steps + 1, n, 2]."""
+
+    @staticmethod
+    def forward(ctx, rollout, tau_0, mu, kappa, damping, v0):
+        rollout.set_parameters(tau_0, mu, kappa, damping, v0)
+        X = rollout.forward()
+        ctx.rollout = rollout
+        ctx.shapes = [t.shape for t in (tau_0, mu, kappa, damping, v0)]
+        ctx.save_for_backward(tau_0, mu, kappa, damping, v0)
+        return X
+
+    @staticmethod
+    def backward(ctx, grad_X):
+        r = ctx.rollout
+        r.set_parameters(*ctx.saved_tensors)
+        g, _, _ = r.backward(grad_X)
+        out = []
+        for k, shape in enumerate(ctx.shapes):
+            gk = g[:, k]
+            out.append(gk.sum().reshape(shape) if len(shape) == 0 else gk.reshape(shape))
+        return (None, *out)
+
+
+def rollout_positions(rollout, tau_0, mu, kappa, damping, v0):
+    """Positions [steps + 1, n, 2] of the rollout, differentiable in its
+    parameters."""
+    return RolloutFunction.apply(rollout, tau_0, mu, kappa, damping, v0)

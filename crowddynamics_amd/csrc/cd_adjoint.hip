@@ -1,0 +1,243 @@
+// Adjoint of the fused step (circular agents, fp64, fixed dt): the reverse
+// sweep that differentiates a trajectory loss with respect to the agents'
+// tau_0, mu, kappa, damping and target_velocity.
+//
+// Forward step k in the reference's order (core/integrator.py:164-176,
+// crowddynamics/simulation/logic.py:59-75; G is force_prev):
+//   F_k     = adjust(v_k; v0) + sum_j f(x_i - x_j, v_i - v_j; p_i) + noise_k
+//   v_{k+1} = v_k + (G_k + F_k) / (2 m) * dt
+//   x_{k+1} = x_k + v_{k+1} * dt + F_k / (2 m) * dt^2
+//   G_{k+1} = F_k
+// Given the adjoints (lx, lv, lg) of step k+1 the sweep forms
+//   lvt = lv + dt lx,   lF = lg + dt/(2m) lvt + dt^2/(2m) lx,
+//   lg_k = dt/(2m) lvt,   lv_k = lvt + (dF/dv)^T lF,   lx_k = lx + (dF/dx)^T lF,
+//   grad_p += (dF/dp)^T lF,
+// and the caller adds dL/dx_k.  The noise does not depend on the state or
+// the parameters; Reset (logic.py:59-64) makes F_k the only force term.
+//
+// Pair terms are owner-computes like the forward force kernel: agent i
+// collects, for every neighbour j, J(d; p_i)^T lF_i from its own side and
+// -J(d; p_j)^T lF_j from j's side (the pair function is odd in
+// d = (x_i - x_j, v_i - v_j), so j's side is -f(d; p_j) and its Jacobian
+// with respect to x_i is -J(d; p_j)).  The 2x4 Jacobian J and the
+// parameter derivatives come from evaluating the forward pair function on
+// dual numbers, which keeps every branch and cut-off of the step (sight,
+// tau validity, force truncation, contact).  No atomics: each agent writes
+// only its own adjoint and gradient rows.
+
+#include "cd_common.h"
+#include "cd_dual.h"
+
+namespace {
+
+struct AdjView {
+  double2 *lam_x, *lam_v, *lam_g;  // adjoint state, [n]
+  double2 *lam_f, *lam_vt;         // lF and lvt of the current step, [n]
+  double *grad;                    // [n][5]: tau_0, mu, kappa, damping, target_velocity
+  const double2 *grad_x;           // dL/dx_k or null
+};
+
+__global__ void __launch_bounds__(256) k_adj_pre(int n, const double *mass, double dt,
+                                                 AdjView J) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double c = dt / (2.0 * mass[i]);
+  const double2 lx = J.lam_x[i], lv = J.lam_v[i], lg = J.lam_g[i];
+  const double2 lvt = make_double2(lv.x + dt * lx.x, lv.y + dt * lx.y);
+  const double c2 = dt * c;
+  J.lam_f[i] = make_double2(lg.x + c * lvt.x + c2 * lx.x, lg.y + c * lvt.y + c2 * lx.y);
+  J.lam_vt[i] = lvt;
+  J.lam_g[i] = make_double2(c * lvt.x, c * lvt.y);
+}
+
+using D4 = Dual<4>;
+
+// Accumulates into acc[0..7] (lx.x, lx.y, lv.x, lv.y, tau_0, mu, kappa,
+// damping) the contributions of agent i's pair with j, d = (x, y, vx, vy).
+__device__ __forceinline__ void pair_vjp(double x, double y, double vx, double vy, double r_tot,
+                                         const Owner<double> &oi, const Owner<double> &oj,
+                                         double2 lfi, double2 lfj, double acc[8]) {
+  {  // i's side: derivatives along the relative state
+    const Owner<D4> o{D4(oi.neg_mk), D4(oi.tau_0), D4(oi.mu), D4(oi.kappa), D4(oi.damping)};
+    D4 fx, fy;
+    if (!pair_circular<D4>(D4::seed(x, 0), D4::seed(y, 1), D4::seed(vx, 2), D4::seed(vy, 3),
+                           D4(r_tot), o, fx, fy))
+      return;  // beyond sight: neither side contributes
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc[k] += lfi.x * fx.d[k] + lfi.y * fy.d[k];
+  }
+  {  // i's side: derivatives along i's parameters
+    const Owner<D4> o{D4(oi.neg_mk), D4::seed(oi.tau_0, 0), D4::seed(oi.mu, 1),
+                      D4::seed(oi.kappa, 2), D4::seed(oi.damping, 3)};
+    D4 fx, fy;
+    pair_circular<D4>(D4(x), D4(y), D4(vx), D4(vy), D4(r_tot), o, fx, fy);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc[4 + k] += lfi.x * fx.d[k] + lfi.y * fy.d[k];
+  }
+  {  // j's side, -f(d; p_j)
+    const Owner<D4> o{D4(oj.neg_mk), D4(oj.tau_0), D4(oj.mu), D4(oj.kappa), D4(oj.damping)};
+    D4 fx, fy;
+    pair_circular<D4>(D4::seed(x, 0), D4::seed(y, 1), D4::seed(vx, 2), D4::seed(vy, 3),
+                      D4(r_tot), o, fx, fy);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc[k] -= lfj.x * fx.d[k] + lfj.y * fy.d[k];
+  }
+}
+
+__device__ __forceinline__ Owner<double> owner_at(const Agents<double> &A, int j) {
+  Owner<double> o;
+  o.neg_mk = (-A.mass[j]) * A.k_soc[j];
+  o.tau_0 = A.tau_0[j];
+  o.mu = A.mu[j];
+  o.kappa = A.kappa[j];
+  o.damping = A.damping[j];
+  return o;
+}
+
+// Pair adjoints of agent i with the agents of global cell c.
+__device__ __forceinline__ void cell_vjp(const Agents<double> &A, const Work &W, const AdjView &J,
+                                         int c, int a0, int a1, int i, double2 xi, double2 vi,
+                                         double ri, const Owner<double> &oi, double2 lfi,
+                                         double acc[8]) {
+  const int cnt = W.count[c];
+  if (cnt <= 0) return;
+  if (cnt <= W.K) {
+    const size_t e0 = (size_t)c * W.K;
+    const double2 *e_vel = (const double2 *)W.e_vel;
+    for (int s = 0; s < cnt; ++s) {
+      const size_t e = e0 + s;
+      const int j = W.ell_ids[e];
+      if (j == i) continue;
+      const double2 xj = W.e_pos[e], vj = e_vel[e];
+      pair_vjp(xi.x - xj.x, xi.y - xj.y, vi.x - vj.x, vi.y - vj.y, ri + W.e_r[e], oi,
+               owner_at(A, j), lfi, J.lam_f[j], acc);
+    }
+    return;
+  }
+  for (int j = a0; j < a1; ++j) {  // more agents than slots (exact slow path)
+    if (j == i || W.cell_of[j] != c) continue;
+    const double2 xj = A.pos[j];
+    const V2<double> vj = A.vel[j];
+    pair_vjp(xi.x - xj.x, xi.y - xj.y, vi.x - vj.x, vi.y - vj.y, ri + A.radius[j], oi,
+             owner_at(A, j), lfi, J.lam_f[j], acc);
+  }
+}
+
+// Four lanes per agent as in the forward force kernel: lanes 0..2 take the
+// stencil rows dx = -1, 0, +1, lane 3 the agent's own (adjusting) terms and
+// the combination.
+__global__ void __launch_bounds__(256) k_adjoint(Agents<double> A, Work W, Batch B, AdjView J,
+                                                 int flags) {
+  const int2 sk = block_e(B);
+  const ScnDesc d = desc_of(B, sk.x);
+  const ScnState *ss = &B.sst[sk.x];
+  const int i = d.a0 + sk.y * (int)(blockDim.x >> 2) + (threadIdx.x >> 2);
+  const int q = threadIdx.x & 3;
+  const int lane0 = (threadIdx.x & 63) & ~3;
+  const bool have = i < d.a1;
+  const bool binned = (flags & CD_USE_AGENT_AGENT) && ss->valid && !ss->oob;
+  double acc[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) acc[k] = 0.0;
+  double2 lfi = make_double2(0.0, 0.0);
+  if (have) lfi = J.lam_f[i];
+  if (have && binned && q < 3) {
+    const double2 xi = A.pos[i];
+    const V2<double> v = A.vel[i];
+    const double2 vi = make_double2(v.x, v.y);
+    const int c = W.cell_of[i] - d.c0;
+    const int ny = ss->ny, nx = ss->nx;
+    const int lx = c / ny, ly = c - lx * ny;
+    const int ox = lx + q - 1;
+    if (ox >= 0 && ox < nx) {
+      const Owner<double> oi = owner_at(A, i);
+      const double ri = A.radius[i];
+      const int ylo = max(ly - 1, 0), yhi = min(ly + 1, ny - 1);
+      for (int oy = ylo; oy <= yhi; ++oy)
+        cell_vjp(A, W, J, d.c0 + ox * ny + oy, d.a0, d.a1, i, xi, vi, ri, oi, lfi, acc);
+    }
+  }
+  double sum[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const double a0 = __shfl(acc[k], lane0 + 0, 64), a1 = __shfl(acc[k], lane0 + 1, 64),
+                 a2 = __shfl(acc[k], lane0 + 2, 64);
+    sum[k] = (a0 + a1) + a2;
+  }
+  if (!have || q != 3) return;
+  double2 lv = J.lam_vt[i];
+  double g_v0 = 0.0;
+  if (flags & CD_USE_ADJUST) {
+    // force_adjust (core/motion/adjusting.py:16-51): F = (m / tau_adj) (v0 e0 - v)
+    const double s = A.mass[i] / A.tau_adj[i];
+    const V2<double> e = A.e0[i];
+    lv.x -= s * lfi.x;
+    lv.y -= s * lfi.y;
+    g_v0 = s * (e.x * lfi.x + e.y * lfi.y);
+  }
+  double2 lx = J.lam_x[i];
+  lx.x += sum[0];
+  lx.y += sum[1];
+  if (J.grad_x) {
+    const double2 g = J.grad_x[i];
+    lx.x += g.x;
+    lx.y += g.y;
+  }
+  lv.x += sum[2];
+  lv.y += sum[3];
+  J.lam_x[i] = lx;
+  J.lam_v[i] = lv;
+  double *g = J.grad + (size_t)i * 5;
+  g[0] += sum[4];
+  g[1] += sum[5];
+  g[2] += sum[6];
+  g[3] += sum[7];
+  g[4] += g_v0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cd_adjoint_step(const cd_agents_t *a, const cd_step_params_t *p, const double *grad_x,
+                    const cd_adjoint_t *adj, const cd_workspace_t *w, cd_stream_t stream) {
+  int rc = check_step(a, p, w);
+  if (rc) return rc;
+  if (!adj || (a->n > 0 && (!adj->lam_x || !adj->lam_v || !adj->lam_g || !adj->lam_f ||
+                            !adj->lam_vt || !adj->grad)))
+    return CD_ERR_ARG;
+  if (a->model != CD_CIRCULAR || a->real != CD_F64) return CD_ERR_UNSUPPORTED;
+  const int allowed = CD_USE_FLUCT | CD_USE_ADJUST | CD_USE_AGENT_AGENT | CD_USE_RESET;
+  if ((p->flags & ~allowed) || !(p->flags & CD_USE_RESET)) return CD_ERR_UNSUPPORTED;
+  if (!(p->dt_max > 0.0) || p->dt_min != p->dt_max) return CD_ERR_UNSUPPORTED;
+  if (w->n_scenarios > 0) return CD_ERR_UNSUPPORTED;
+  if (a->n == 0) return CD_OK;
+  hipStream_t s = (hipStream_t)stream;
+  const Agents<double> A = typed<double>(a);
+  const Work W = carve(w);
+  const Batch B = single_batch(w, 0, p->seed, p->dt_min, p->dt_max);
+  const Launch L = single_launch(w);
+  AdjView J;
+  J.lam_x = (double2 *)adj->lam_x;
+  J.lam_v = (double2 *)adj->lam_v;
+  J.lam_g = (double2 *)adj->lam_g;
+  J.lam_f = (double2 *)adj->lam_f;
+  J.lam_vt = (double2 *)adj->lam_vt;
+  J.grad = adj->grad;
+  J.grad_x = (const double2 *)grad_x;
+  hipLaunchKernelGGL(k_adj_pre, dim3(grid_for(a->n, kBS)), dim3(kBS), 0, s, (int)a->n, A.mass,
+                     p->dt_max, J);
+  if ((rc = check_launch())) return rc;
+  if (p->flags & CD_USE_AGENT_AGENT) {
+    if ((rc = launch_reduce(A, W, B, L.blocks_f, true, s))) return rc;
+    hipLaunchKernelGGL(k_bin_insert, dim3(L.blocks_f), dim3(kBS), 0, s, (const double2 *)A.pos,
+                       p->cell_size, W, B, 1);
+    hipLaunchKernelGGL(k_cell_gather<double>, dim3(grid_for(W.cap, kBS)), dim3(kBS), 0, s, A, W);
+    if ((rc = check_launch())) return rc;
+  }
+  hipLaunchKernelGGL(k_adjoint, dim3(L.blocks_e), dim3(L.threads_e), 0, s, A, W, B, J,
+                     (int)p->flags);
+  return check_launch();
+}
+
+}  // extern "C"

@@ -240,6 +240,26 @@ int cd_batch_step(const cd_agents_t *agents, const cd_step_params_t *params,
 int cd_batch_read_states(const cd_workspace_t *ws, cd_state_t *out, int32_t count,
                          cd_stream_t stream);
 
+/* Adjoint of one fused step (circular agents, fp64, fixed dt = dt_min =
+ * dt_max, flags within FLUCT | ADJUST | AGENT_AGENT | RESET with RESET set,
+ * single-scenario workspace).  `agents` holds the state x_k, v_k the step
+ * started from (position / velocity may point into a caller's tape).
+ * Maps the adjoint (lam_x, lam_v, lam_g) of step k+1 to that of step k in
+ * place, adds grad_x (dL/dx_k, [n][2], may be null) to lam_x, and adds the
+ * step's parameter gradients to grad ([n][5]: tau_0, mu, kappa, damping,
+ * target_velocity of each agent).  Stream-ordered, no allocation.
+ * Replaces reverse-mode differentiation through interactions.py:51-70,
+ * adjusting.py:16-51 and integrator.py:164-176, which the reference does
+ * not provide. */
+typedef struct cd_adjoint {
+  double *lam_x, *lam_v, *lam_g; /* [n][2] adjoint state, in/out */
+  double *lam_f, *lam_vt;        /* [n][2] scratch of the step */
+  double *grad;                  /* [n][5] accumulated parameter gradients */
+} cd_adjoint_t;
+int cd_adjoint_step(const cd_agents_t *agents, const cd_step_params_t *params,
+                    const double *grad_x, const cd_adjoint_t *adj, const cd_workspace_t *ws,
+                    cd_stream_t stream);
+
 /* Synchronous: runs `steps` cd_step calls and returns in kernel_ms[4] the
  * summed GPU time of each step kernel (bin/reduce, sort+gather, forces,
  * integrate), measured with HIP events recorded on `stream` around each
