@@ -26,7 +26,8 @@ sys.path.insert(0, ROOT)
 METRIC = ('agent-steps/s @20k Circular (GPU fp64/fp32 vs host numba); adjoint step time')
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
-KERNELS = ('bin_insert', 'cell_gather', 'forces', 'integrate')
+KERNELS = ('bin_insert', 'forces', 'integrate', 'step')
+STEP_KERNELS = KERNELS[:3]
 
 
 def log(*a):
@@ -61,8 +62,8 @@ def pair_tally(arr, cell, segments):
 
 
 def binning_bytes(n, cells):
-    """24 B per agent (read position, write sorted index) + 16 B per cell
-    (count and offset)."""
+    """24 B per agent (read position, write its id into its cell's list and
+    its cell index) + 16 B per cell (count written and cleared)."""
     return 24 * n + 16 * cells
 
 
@@ -108,10 +109,9 @@ def bench_ensemble(n_scenarios, steps, loop_steps, stream):
     with torch.cuda.stream(stream):
         sizes = neutral_cell_sizes(max_body_radius(scns))[:3]
         tuned = tune_ensemble(scns, precision='f32', cell_sizes=sizes, steps=5, warmup=2)
-        log('autotune: cell %.3f, force block %d (%.3f ms/step)' % (
-            tuned['cell_size'], tuned['force_threads'], tuned['ms_per_step']))
-        ens = Ensemble(scns, precision='f32', cell_size=tuned['cell_size'],
-                       force_threads=tuned['force_threads'])
+        log('autotune: cell %.3f, tile %s (%.3f ms/step)' % (
+            tuned['cell_size'], tuned['tile'], tuned['ms_per_step']))
+        ens = Ensemble(scns, precision='f32', cell_size=tuned['cell_size'], tile=tuned['tile'])
         ens.step(10)
         G = 50
         graph = ens.capture(G)
@@ -150,8 +150,8 @@ def bench_ensemble(n_scenarios, steps, loop_steps, stream):
                 ms_per_step=1e3 * el / (reps * G),
                 kernel_ms={k: kms[i] / 20 for i, k in enumerate(KERNELS)},
                 autotuned=dict(cell_size=tuned['cell_size'],
-                               force_threads=tuned['force_threads'],
-                               candidates=[dict(ms=t[0], cell_size=t[1], force_threads=t[2])
+                               tile=tuned['tile'],
+                               candidates=[dict(ms=t[0], cell_size=t[1], tile=t[2])
                                            for t in tuned['table']]),
                 per_scenario_loop=dict(value=loop, steps=loop_steps,
                                        ms_per_step=1e3 * el_loop / loop_steps),
@@ -263,6 +263,9 @@ def main():
     ap.add_argument('--no-corridor', action='store_true')
     ap.add_argument('--no-adjoint', action='store_true')
     ap.add_argument('--adjoint-steps', type=int, default=500)
+    ap.add_argument('--calibrate-copy', action='store_true',
+                    help='first run a plain 256 MiB device copy five times (a kernel of '
+                         'known traffic for scaling hardware byte counters)')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU: the per-step update runs on a single MI355X '
@@ -274,16 +277,33 @@ def main():
     from crowddynamics_amd import examples as E
     from crowddynamics_amd.stepper import Stepper
 
+    if args.calibrate_copy:
+        src = torch.ones(64 << 20, dtype=torch.float32, device='cuda')
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log('calibration copy: 5 x %d bytes read and written' % (src.numel() * 4))
+        del src, dst
+
     log('building workload: room n=%d' % args.n)
     scn = E.room(n=args.n)
     results = {}
     kernel_ms = {}
     stream = torch.cuda.Stream()
+    from crowddynamics_amd.autotune import tune_tile
+
+    def make(prec, tile):
+        return Stepper(scn.agents.copy(), segments=scn.segments, nav=scn.nav,
+                       cell_size=scn.cell_size, dt_min=scn.dt_min, dt_max=scn.dt_max,
+                       seed=scn.seed, precision=prec, bounds=scn.bounds, tile=tile)
+
+    with torch.cuda.stream(stream):
+        tile, tile_table = tune_tile(lambda t: make(args.precision, t))
+    log('force tile %s (candidates %s)' % (tile, [(t[1], round(t[0], 4)) for t in tile_table]))
     for prec in ('f64', 'f32'):
         with torch.cuda.stream(stream):
-            st = Stepper(scn.agents.copy(), segments=scn.segments, nav=scn.nav,
-                         cell_size=scn.cell_size, dt_min=scn.dt_min, dt_max=scn.dt_max,
-                         seed=scn.seed, precision=prec, bounds=scn.bounds)
+            st = make(prec, tile)
             st.step(args.warmup)
             G = max(1, min(args.graph_steps, args.steps))
             graph = st.capture(G)
@@ -320,7 +340,7 @@ def main():
     cand_step = 0.5 * (t0_tally['candidate_pairs'] + t1_tally['candidate_pairs'])
     rb = 8 if main_p == 'f64' else 4
     per_kernel = {}
-    bin_ms = km['bin_insert'] + km['cell_gather']
+    bin_ms = km['bin_insert']
     per_kernel['binning'] = dict(bound='hbm', unit='GB/s', ms=bin_ms,
                                  achieved=binning_bytes(scn.n, cells) / (bin_ms * 1e-3) / 1e9,
                                  peak=HBM_PEAK_GBS)
@@ -333,7 +353,7 @@ def main():
                                 candidate_pairs_per_s=cand_step / (km['forces'] * 1e-3))
     for v in per_kernel.values():
         v['frac'] = v['achieved'] / v['peak']
-    dom = max(KERNELS, key=lambda k: km[k])
+    dom = max(STEP_KERNELS, key=lambda k: km[k])
     if dom == 'forces':
         rk = per_kernel['forces']
         traffic = None
@@ -389,7 +409,7 @@ def main():
         'config': {'workload': 'room: 20k Circular agents, 200x100 m, 4 exits of 1.2 m, '
                                'cell 0.6 m, 0.1 m potential grid, adaptive dt',
                    'agents': scn.n, 'cell_size': scn.cell_size, 'precision': main_p,
-                   'graph_steps': args.graph_steps},
+                   'graph_steps': args.graph_steps, 'force_tile': list(tile)},
         'roofline': roofline,
         'cpu_baseline': cpu,
         'fp32_value': results['f32']['value'] if main_p == 'f64' else results['f64']['value'],

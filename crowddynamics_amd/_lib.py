@@ -54,7 +54,8 @@ class CDWorkspace(C.Structure):
                 ('cell_capacity', C.c_int64), ('cell_slots', C.c_int64), ('model', C.c_int32),
                 ('max_scenarios', C.c_int32), ('n_scenarios', C.c_int32),
                 ('blocks_f', C.c_int32), ('blocks_e', C.c_int32),
-                ('force_threads', C.c_int32)]
+                ('force_threads', C.c_int32), ('tile_x', C.c_int32), ('tile_y', C.c_int32),
+                ('blocks_t', C.c_int32), ('_pad', C.c_int32)]
 
 
 class CDScenario(C.Structure):

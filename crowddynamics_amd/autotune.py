@@ -1,5 +1,5 @@
-"""Autotuner of the batched step: picks the block-list cell size and the
-force kernel's block size for a given ensemble by timing candidates on the
+"""Autotuner of the step: picks the block-list cell size and the force
+kernel's tile shape (grid cells per workgroup) by timing candidates on the
 device.
 
 Only physics-neutral cell sizes are candidates.  An agent pair interacts
@@ -32,24 +32,42 @@ def neutral_cell_sizes(r_max, factors=(1.0, 1.15, 1.35, 1.6), sight=SIGHT_SOC):
     return [float(np.ceil(base * f * 1000.0) / 1000.0) for f in factors]
 
 
-def tune_ensemble(scenarios, precision='f32', cell_sizes=None, force_threads=(64, 128, 256),
+def tune_ensemble(scenarios, precision='f32', cell_sizes=None,
+                  tiles=((0, 0), (1, 1), (2, 2), (4, 4)),
                   steps=10, warmup=3, flags=None):
-    """Time every (cell size, force block) pair over `steps` batched steps
+    """Time every (cell size, force tile) pair over `steps` batched steps
     (after `warmup` steps) and return the fastest.
 
-    Returns dict(cell_size, force_threads, ms_per_step, table) where table
-    lists (ms_per_step, cell_size, force_threads, per-kernel ms)."""
+    The force kernel choice (tile (0, 0): agent-parallel; else grid cells
+    per LDS tile) never changes results: pair terms are summed exactly.  Returns
+    dict(cell_size, tile, ms_per_step, table) where table lists
+    (ms_per_step, cell_size, tile, per-kernel ms)."""
     if cell_sizes is None:
         cell_sizes = neutral_cell_sizes(max_body_radius(scenarios))
     kw = {} if flags is None else dict(flags=flags)
     table = []
     for cs in cell_sizes:
-        for ft in force_threads:
-            ens = Ensemble(scenarios, precision=precision, cell_size=cs, force_threads=ft, **kw)
+        for tile in tiles:
+            ens = Ensemble(scenarios, precision=precision, cell_size=cs, tile=tile, **kw)
             ens.step(warmup)
             ms = ens.timed(steps)
             ens.check()
-            table.append((sum(ms) / steps, cs, ft, [m / steps for m in ms]))
+            table.append((ms[3] / steps, cs, tuple(tile), [m / steps for m in ms]))
             del ens
     best = min(table, key=lambda t: t[0])
-    return dict(cell_size=best[1], force_threads=best[2], ms_per_step=best[0], table=table)
+    return dict(cell_size=best[1], tile=best[2], ms_per_step=best[0], table=table)
+
+
+def tune_tile(make_stepper, tiles=((0, 0), (4, 4), (8, 8), (16, 16)), steps=20, warmup=5):
+    """Fastest force tile for one simulation: make_stepper(tile) builds a
+    Stepper on a fresh copy of the state.  Returns (tile, table)."""
+    table = []
+    for tile in tiles:
+        st = make_stepper(tile)
+        st.step(warmup)
+        ms = st.timed(steps)
+        st.check()
+        table.append((ms[3] / steps, tuple(tile), [m / steps for m in ms]))
+        del st
+    best = min(table, key=lambda t: t[0])
+    return best[1], table

@@ -52,6 +52,7 @@ struct ScnDesc {
   int c0, ccap;  // grid cells [c0, c0 + ccap)
   int w0, w1;    // wall segments [w0, w1)
   int f0, nf;    // blocks (and partials) [f0, f0 + nf) of the per-scenario kernels
+  int nt;        // blocks of the tile force kernel
   unsigned long long seed;
   double dt_min, dt_max;
 };
@@ -66,6 +67,7 @@ struct ScnState {
   int C;        // cells of the last binning
   int C_clear;  // cells whose counts the next clearing pass must zero
   int oob;      // an agent fell outside the grid (stale bounding box)
+  int pairs_done;  // the force kernel served this step's agents (set by K_e, read by K_f)
 };
 
 constexpr int kBS = 256;              // threads per block of per-agent kernels
@@ -87,7 +89,7 @@ inline int64_t max_blocks_f(int64_t n, int64_t S) {
 inline int64_t max_blocks_e(int64_t n, int64_t S) { return n / kMinForceAgents + S + 1; }
 
 struct Layout {
-  int64_t state, scn_desc, scn_state, map_f, map_e, partials, tiles;
+  int64_t state, scn_desc, scn_state, map_f, map_e, map_t, partials, tiles;
   int64_t cell_of, rank, pidx, count, offset;
   int64_t ell_ids, e_pos, e_vel, e_r, e_ls, e_rs, e_phi, e_rt, e_rs_r, e_rts;
   int64_t total;
@@ -108,6 +110,7 @@ Layout make_layout(const cd_workspace_t *w) {
   L.scn_state = take(sizeof(ScnState) * S);
   L.map_f = take(8 * max_blocks_f(n, S));
   L.map_e = take(8 * max_blocks_e(n, S));
+  L.map_t = take(8 * (cap + S + 1));  // at most one tile block per cell, plus one per scenario
   L.partials = take(8 * 8 * max_blocks_f(n, S));
   L.tiles = take(8 * ((cap + kScanTile - 1) / kScanTile + 1));
   L.cell_of = take(4 * n);
@@ -204,7 +207,7 @@ struct Work {
 struct Batch {
   const ScnDesc *desc;
   ScnState *sst;
-  const int2 *map_f, *map_e;
+  const int2 *map_f, *map_e, *map_t;
   ScnDesc one;
 };
 
@@ -240,6 +243,16 @@ inline int single_parts(int64_t n) {
   return std::max(1, std::min(grid_for(n, kBS), kSingleParts));
 }
 
+// Blocks of the tiled force kernel for a scenario of `cells` grid cells at
+// most (blocks stride over the tiles of the actual grid); one when the
+// workspace selects the direct kernel (tile_x or tile_y 0).
+constexpr int kMaxTileBlocks = 4096;
+inline int tile_blocks(int64_t cells, const cd_workspace_t *w) {
+  if (w->tile_x <= 0 || w->tile_y <= 0) return 1;
+  const int64_t per = (int64_t)w->tile_x * w->tile_y;
+  return (int)std::max<int64_t>(1, std::min<int64_t>((cells + per - 1) / per, kMaxTileBlocks));
+}
+
 // Batch view of a single scenario covering every agent, cell and wall.
 Batch single_batch(const cd_workspace_t *w, int64_t n_segments, uint64_t seed, double dt_min,
                    double dt_max) {
@@ -249,6 +262,7 @@ Batch single_batch(const cd_workspace_t *w, int64_t n_segments, uint64_t seed, d
   B.sst = (ScnState *)((char *)w->ptr + L.scn_state);
   B.map_f = nullptr;
   B.map_e = nullptr;
+  B.map_t = nullptr;
   B.one.a0 = 0;
   B.one.a1 = (int)w->n;
   B.one.c0 = 0;
@@ -257,6 +271,7 @@ Batch single_batch(const cd_workspace_t *w, int64_t n_segments, uint64_t seed, d
   B.one.w1 = (int)n_segments;
   B.one.f0 = 0;
   B.one.nf = single_parts(w->n);
+  B.one.nt = tile_blocks(w->cell_capacity, w);
   B.one.seed = seed;
   B.one.dt_min = dt_min;
   B.one.dt_max = dt_max;
@@ -272,6 +287,7 @@ Batch batch_of(const cd_workspace_t *w) {
   B.sst = (ScnState *)(b + L.scn_state);
   B.map_f = (const int2 *)(b + L.map_f);
   B.map_e = (const int2 *)(b + L.map_e);
+  B.map_t = (const int2 *)(b + L.map_t);
   return B;
 }
 
@@ -471,8 +487,11 @@ __device__ __forceinline__ int cell_of_pos(double2 p, const Grid &g, double cell
 // K_a: combine the scenario's partials; when `binning`, bin its agents into
 // their cells' slot lists (slots beyond cell_slots are not stored; the cell
 // count tells).  Cell indices are global: scenario base + local cell.
+// `count_step`: the fused step counts itself here (no kernel of the step
+// reads the counter concurrently; the force and integrate kernels of the
+// step read it as step + 1).
 __global__ void __launch_bounds__(256) k_bin_insert(const double2 *pos, double cell, Work W,
-                                                    Batch B, int binning) {
+                                                    Batch B, int binning, int count_step) {
   const int2 sk = block_f(B);
   const ScnDesc d = desc_of(B, sk.x);
   ScnState *ss = &B.sst[sk.x];
@@ -482,6 +501,7 @@ __global__ void __launch_bounds__(256) k_bin_insert(const double2 *pos, double c
   const Red r = reduce_partials(W, d);
   const Grid g = grid_of(r, cell, d.ccap);
   if (sk.y == 0 && threadIdx.x == 0) publish_grid(W, ss, r, g, binning != 0, d.a1 > d.a0);
+  if (count_step && blockIdx.x == 0 && threadIdx.x == 0) W.st->step += 1;
   if (!binning || !g.ok) return;
   for (int i = i0; i < d.a1; i += stride) {
     const int lc = cell_of_pos(i == i0 ? p0 : pos[i], g, cell);
@@ -493,7 +513,10 @@ __global__ void __launch_bounds__(256) k_bin_insert(const double2 *pos, double c
     const int c = d.c0 + lc;
     W.cell_of[i] = c;
     const int k = atomicAdd(&W.count[c], 1);
-    if (k < W.K) W.ell_ids[c * W.K + k] = i;
+    if (k < W.K)
+      W.ell_ids[c * W.K + k] = i;
+    else if (k == W.K)  // the cell's list is full: its members are found by scanning cell_of
+      atomicOr((unsigned long long *)&W.st->status, (unsigned long long)CD_STATUS_SLOTS_FULL);
   }
 }
 
@@ -591,6 +614,7 @@ bool ws_sizes_ok(const cd_workspace_t *w) {
          w->max_scenarios >= 0 && w->max_scenarios <= (1 << 20) &&
          (w->force_threads == 0 || w->force_threads == 64 || w->force_threads == 128 ||
           w->force_threads == 256) &&
+         w->tile_x >= 0 && w->tile_x <= 16 && w->tile_y >= 0 && w->tile_y <= 16 &&
          (w->model == CD_CIRCULAR || w->model == CD_THREE_CIRCLE);
 }
 
@@ -619,7 +643,7 @@ inline void mark(hipEvent_t *ev, int k, hipStream_t s) {
 
 // Grid sizes of one step: per-scenario kernels (f) and force kernel (e).
 struct Launch {
-  int blocks_f, blocks_e, threads_e;
+  int blocks_f, blocks_e, threads_e, blocks_t;
 };
 
 Launch single_launch(const cd_workspace_t *w) {
@@ -627,6 +651,7 @@ Launch single_launch(const cd_workspace_t *w) {
   L.blocks_f = single_parts(w->n);
   L.blocks_e = std::max(1, grid_for(w->n, force_threads(w) / 4));
   L.threads_e = force_threads(w);
+  L.blocks_t = tile_blocks(w->cell_capacity, w);
   return L;
 }
 

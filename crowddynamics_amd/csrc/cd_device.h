@@ -82,6 +82,59 @@ __device__ __forceinline__ void truncate(T &x, T &y, T l) {
   }
 }
 
+// Exact, order-independent sum of doubles: a 128-bit two's-complement
+// fixed-point number with 64 fractional bits (range +-2^63, resolution
+// 2^-64).  Every term is converted exactly when its magnitude is at least
+// 2^-11 (and truncated to the 2^-64 grid below that); additions are exact
+// integer additions, so the sum does not depend on the order of the terms.
+// Terms that are not finite or of magnitude 2^62 or more make the sum NaN.
+struct Fix128 {
+  long long hi;
+  unsigned long long lo;
+  bool bad;
+  __host__ __device__ __forceinline__ Fix128() : hi(0), lo(0ull), bad(false) {}
+  __host__ __device__ __forceinline__ void add(double x) {
+    if (!(fabs(x) < 4611686018427387904.0)) {  // 2^62
+      bad = true;
+      return;
+    }
+    const double a = fabs(x);  // convert |x| exactly, then negate
+    const double f = floor(a);
+    const double t = (a - f) * 4294967296.0;  // fraction * 2^32, exact
+    const double th = floor(t);
+    unsigned long long l = ((unsigned long long)(unsigned int)th << 32) |
+                           (unsigned long long)(unsigned int)((t - th) * 4294967296.0);
+    const double fh = floor(f * 2.3283064365386962890625e-10);  // f / 2^32
+    long long h = (long long)(unsigned int)fh * 4294967296LL +
+                  (long long)(unsigned int)(f - fh * 4294967296.0);
+    if (x < 0.0) {  // two's complement of (h, l)
+      h = ~h + (l == 0ull ? 1 : 0);
+      l = ~l + 1ull;
+    }
+    lo += l;
+    hi += h + (lo < l ? 1 : 0);
+  }
+  __host__ __device__ __forceinline__ void add(const Fix128 &o) {
+    lo += o.lo;
+    hi += o.hi + (lo < o.lo ? 1 : 0);
+    bad = bad || o.bad;
+  }
+  // The sum, rounded to a double: magnitude first, so small negative sums
+  // keep their low bits.
+  __host__ __device__ __forceinline__ double value() const {
+    if (bad) return __builtin_nan("");
+    long long h = hi;
+    unsigned long long l = lo;
+    const bool neg = h < 0;
+    if (neg) {
+      h = ~h + (l == 0ull ? 1 : 0);
+      l = ~l + 1ull;
+    }
+    const double v = (double)h + (double)l * 5.421010862427522170037264e-20;  // 2^-64
+    return neg ? -v : v;
+  }
+};
+
 // Per-agent constants the owner side of a pair needs.
 template <typename T>
 struct Owner {

@@ -424,14 +424,28 @@ __global__ void __launch_bounds__(256) k_walls(Agents<T> A, int model, const dou
 // K_e: forces (owner-computes over the 3x3 cell stencil)
 // ---------------------------------------------------------------------------
 //
-// Four lanes serve one agent (16, 32 or 64 agents per block, all of one
-// scenario, in agent index order so own-data loads are coalesced).
-// Lanes 0..2 sum the agent's side of every pair in stencil row dx = -1, 0, +1
-// (cells ly-1..ly+1 in slot order within each cell); lane 3 evaluates the
-// agent's own terms (fluctuation, navigation, orientation, adjusting), issuing
-// its loads and the noise arithmetic before the pair work; wall contacts are
-// split over the four lanes.  The partial sums meet on lane 3 through
-// wavefront shuffles in a fixed order (deterministic).
+// Four lanes serve one agent: lanes 0..2 sum the agent's side of every pair
+// in stencil rows dx = -1, 0, +1 (cells dy = -1, 0, +1); lane 3 evaluates the
+// agent's own terms (fluctuation, navigation, orientation, adjusting); wall
+// contacts are split over the four lanes in a fixed order.  Pair terms are
+// summed exactly (Fix128), so the result does not depend on the order in
+// which the binning inserted agents into cells or on how lanes and
+// workgroups split the work; it is the correctly rounded sum of the terms up
+// to the 2^-64 N grid.  Two kernels share this per-agent work:
+//
+//  * direct: workgroups take 16..64 agents in agent order; neighbour ids
+//    come from the cells' slot lists and neighbour data is read by id.
+//  * tiled: a workgroup takes a tile of tile_x * tile_y grid cells and
+//    stages the tile plus a one-cell halo in LDS (cell counts -> offsets by
+//    a block scan -> ids from the slot lists -> neighbour data by id), then
+//    serves the agents of the tile's inner cells from LDS.  Each cell's data
+//    is read from memory by at most four workgroups.  A region holding more
+//    agents than the LDS capacity is processed cell by cell; a cell whose
+//    3 x 3 region still does not fit is served from memory.
+//
+// Both give identical results; the tile shape is a tuning parameter.
+// Members of a cell holding more agents than slots are found by scanning
+// cell_of (exact, slow, flagged by SLOTS_FULL).
 
 struct FusedArgs {
   int flags, nav_mode;
@@ -440,6 +454,66 @@ struct FusedArgs {
   double nav_minx, nav_miny, nav_step;
   const double4 *segs;
 };
+
+constexpr int kTileThreads = 256;
+constexpr int kTileGroups = kTileThreads / 4;  // agents served at once
+constexpr int kTileMaxDim = 16;
+constexpr int kRegionMax = (kTileMaxDim + 2) * (kTileMaxDim + 2);
+constexpr int kWallTile = 256;
+
+// LDS capacity (agents of a region): nine cells of up to 128 (circular) or
+// 56 (three-circle) agents fit, so a 3 x 3 region fits whenever no cell
+// exceeds those counts.
+template <int MODEL>
+struct TileCap;
+template <>
+struct TileCap<0> {
+  static constexpr int v = 1152;
+};
+template <>
+struct TileCap<1> {
+  static constexpr int v = 512;
+};
+
+template <typename T, int MODEL>
+struct TileLds {
+  static constexpr int CAP = TileCap<MODEL>::v;
+  static constexpr int XC = MODEL ? CAP : 1;  // three-circle-only arrays
+  int cnt[kRegionMax];
+  int off[kRegionMax + 1];
+  int own[kTileMaxDim + 1];
+  int wsum[kTileThreads / 64];
+  int id[CAP];
+  int cell[CAP];  // region cell of each element
+  double2 pos[CAP];
+  V2<T> vel[CAP];
+  double r[CAP];
+  double2 ls[XC], rs[XC];
+  double phi[XC], rt[XC], rsr[XC], rts[XC];
+};
+
+// Exclusive scan of one int per thread over the block; *total gets the sum.
+__device__ __forceinline__ int block_scan_excl(int v, int *wsum, int &total) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int inc = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += u;
+  }
+  if (lane == 63) wsum[wid] = inc;
+  __syncthreads();
+  int off = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kTileThreads / 64; ++w) {
+    const int s = wsum[w];
+    if (w < wid) off += s;
+    tot += s;
+  }
+  __syncthreads();
+  total = tot;
+  return off + inc - v;
+}
 
 template <typename T>
 __device__ __forceinline__ Owner<T> owner_of(const Agents<T> &A, int i) {
@@ -450,25 +524,6 @@ __device__ __forceinline__ Owner<T> owner_of(const Agents<T> &A, int i) {
   o.kappa = (T)A.kappa[i];
   o.damping = (T)A.damping[i];
   return o;
-}
-
-__device__ __forceinline__ TCView tc_slot(const Work &W, size_t e) {
-  TCView t;
-  const double2 p = W.e_pos[e], l = W.e_ls[e], r = W.e_rs[e];
-  t.px[0] = p.x;
-  t.py[0] = p.y;
-  t.px[1] = l.x;
-  t.py[1] = l.y;
-  t.px[2] = r.x;
-  t.py[2] = r.y;
-  const V2<double> v = ((const V2<double> *)W.e_vel)[e];
-  t.vx = v.x;
-  t.vy = v.y;
-  t.r_t = W.e_rt[e];
-  t.r_s = W.e_rs_r[e];
-  t.r_ts = W.e_rts[e];
-  t.phi = W.e_phi[e];
-  return t;
 }
 
 template <typename T>
@@ -491,156 +546,235 @@ __device__ __forceinline__ TCView tc_agent(const Agents<T> &A, int j) {
   return t;
 }
 
+template <typename T, int MODEL>
+__device__ __forceinline__ TCView tc_lds(const TileLds<T, MODEL> &S, int e) {
+  TCView t;
+  const double2 p = S.pos[e], l = S.ls[e], r = S.rs[e];
+  t.px[0] = p.x;
+  t.py[0] = p.y;
+  t.px[1] = l.x;
+  t.py[1] = l.y;
+  t.px[2] = r.x;
+  t.py[2] = r.y;
+  const V2<T> v = S.vel[e];
+  t.vx = (double)v.x;
+  t.vy = (double)v.y;
+  t.r_t = S.rt[e];
+  t.r_s = S.rsr[e];
+  t.r_ts = S.rts[e];
+  t.phi = S.phi[e];
+  return t;
+}
+
+// Pair sums of one lane.
+struct PairSum {
+  Fix128 x, y, m;
+};
+
 // Circular: agent i's side of its pair with a neighbour at xj, vj, rj.
 template <typename T>
 __device__ __forceinline__ void pair_add_c(double2 xi, V2<T> vi, double ri, const Owner<T> &o,
-                                           double2 xj, V2<T> vj, double rj, T &fx, T &fy) {
+                                           double2 xj, V2<T> vj, double rj, PairSum &ps) {
   T px, py;
   if (pair_circular<T>((T)(xi.x - xj.x), (T)(xi.y - xj.y), vi.x - vj.x, vi.y - vj.y,
                        (T)(ri + rj), o, px, py)) {
-    fx += px;
-    fy += py;
+    ps.x.add((double)px);
+    ps.y.add((double)py);
   }
 }
 
 // Three-circle: the reference evaluates the pair as interaction(min, max).
 __device__ __forceinline__ void pair_add_tc(int i, int j, const TCView &me, const TCView &other,
-                                            const Owner<double> &o, double &fx, double &fy,
-                                            double &tq) {
+                                            const Owner<double> &o, PairSum &ps) {
   double px, py, m;
   const bool hit = (i < j) ? pair_three_circle(me, other, false, o, px, py, m)
                            : pair_three_circle(other, me, true, o, px, py, m);
   if (hit) {
-    fx += px;
-    fy += py;
-    tq += m;
+    ps.x.add(px);
+    ps.y.add(py);
+    ps.m.add(m);
   }
 }
 
-// Pairs of agent i with the agents of global cell c (scenario agents a0..a1).
+// Own (per-agent) terms of the fused step in the reference's post-order:
+// fluctuation, navigation (e0), orientation (phi0), adjusting.  Adds to
+// f / tq and writes e0 / phi0.  `local` is the agent's index within its
+// scenario (the noise counter).
 template <typename T, int MODEL>
-__device__ __forceinline__ void cell_pairs(const Agents<T> &A, const Work &W, int c, int a0,
-                                           int a1, int i, double2 xi, V2<T> vi, double ri,
-                                           const Owner<T> &o, const TCView &me, T &fx, T &fy,
-                                           double &tq) {
-  const int cnt = W.count[c];
-  if (cnt <= 0) return;
-  if (cnt <= W.K) {
-    const size_t e0 = (size_t)c * W.K;
-    const V2<T> *e_vel = (const V2<T> *)W.e_vel;
-    for (int s = 0; s < cnt; ++s) {
-      const size_t e = e0 + s;
-      const int j = W.ell_ids[e];
-      if (j == i) continue;
-      if constexpr (MODEL == 0) {
-        pair_add_c<T>(xi, vi, ri, o, W.e_pos[e], e_vel[e], W.e_r[e], fx, fy);
-      } else {
-        if constexpr (sizeof(T) == 8) pair_add_tc(i, j, me, tc_slot(W, e), o, fx, fy, tq);
+__device__ __forceinline__ void own_terms(const Agents<T> &A, const FusedArgs &fa, uint64_t seed,
+                                          uint64_t step, int i, int local, double2 pos, V2<T> vel,
+                                          V2<T> &f, double &tq) {
+  if (fa.flags & CD_USE_FLUCT) {
+    double nfx, nfy, nm;
+    fluct_noise(seed, step, local, A.mass[i], A.sig_f[i], MODEL != 0, MODEL ? A.inertia[i] : 0.0,
+                MODEL ? A.sig_t[i] : 0.0, nfx, nfy, nm);
+    f.x += (T)nfx;
+    f.y += (T)nfy;
+    if (MODEL) tq += nm;
+  }
+  V2<T> e = A.e0[i];
+  if (fa.flags & CD_USE_NAV) {
+    double ex, ey;
+    if (nav_lookup(pos.x, pos.y, A.target[i], fa.maps, fa.nav_T, fa.nav_rows, fa.nav_cols,
+                   fa.nav_minx, fa.nav_miny, fa.nav_step, fa.nav_mode, ex, ey)) {
+      e.x = (T)ex;
+      e.y = (T)ey;
+      A.e0[i] = e;
+    }
+  }
+  if (fa.flags & CD_USE_ADJUST) {
+    if (MODEL) A.phi0[i] = atan2((double)e.y, (double)e.x);
+    T ax, ay;
+    adjust_force(A, i, vel, e, ax, ay);
+    f.x += ax;
+    f.y += ay;
+    if (MODEL) tq += adjust_torque(A, i);
+  }
+}
+
+// Neighbour sources of one stencil row: the LDS elements of a staged
+// region ...
+template <typename T, int MODEL>
+struct LdsRow {
+  const TileLds<T, MODEL> &S;
+  int RH, kx, ky;  // region column length and the agent's region cell
+  template <class F>
+  __device__ __forceinline__ void each(int q, int self, F &&fn) const {
+    const int cx = kx + q - 1;
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int kk = cx * RH + (ky + dy);
+      const int e1 = S.off[kk + 1];
+      for (int e2 = S.off[kk]; e2 < e1; ++e2) {
+        const int j = S.id[e2];
+        if (j == self) continue;
+        if constexpr (MODEL == 0)
+          fn(j, S.pos[e2], S.vel[e2], S.r[e2]);
+        else
+          fn(j, tc_lds<T, MODEL>(S, e2));
       }
     }
-    return;
   }
-  // more agents than slots: visit the cell's members in agent index order
-  // (the contract's order) by scanning the cell of every agent of the scenario
-  for (int j = a0; j < a1; ++j) {
-    if (j == i || W.cell_of[j] != c) continue;
-    if constexpr (MODEL == 0) {
-      pair_add_c<T>(xi, vi, ri, o, A.pos[j], A.vel[j], A.radius[j], fx, fy);
-    } else {
-      if constexpr (sizeof(T) == 8) pair_add_tc(i, j, me, tc_agent(A, j), o, fx, fy, tq);
+};
+
+// ... or the grid cells in memory: slot lists, or a scan of cell_of for a
+// cell holding more agents than slots (always, when `scan`).
+template <typename T, int MODEL>
+struct MemRow {
+  const Agents<T> &A;
+  const Work &W;
+  int c0, nx, ny, lx, ly, a0, a1;
+  bool scan;
+  template <class F>
+  __device__ __forceinline__ void visit(int self, int j, F &fn) const {
+    if (j == self) return;
+    if constexpr (MODEL == 0)
+      fn(j, A.pos[j], A.vel[j], A.radius[j]);
+    else
+      fn(j, tc_agent(A, j));
+  }
+  template <class F>
+  __device__ __forceinline__ void each(int q, int self, F &&fn) const {
+    const int ox = lx + q - 1;
+    if (ox < 0 || ox >= nx) return;
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int oy = ly + dy;
+      if (oy < 0 || oy >= ny) continue;
+      const int gc = c0 + ox * ny + oy;
+      const int cnt = W.count[gc];
+      if (!scan && cnt <= W.K) {
+        const int *ids = W.ell_ids + (size_t)gc * W.K;
+        for (int s = 0; s < cnt; ++s) visit(self, ids[s], fn);
+      } else if (cnt > 0) {
+        for (int j = a0; j < a1; ++j)
+          if (W.cell_of[j] == gc) visit(self, j, fn);
+      }
     }
   }
+};
+
+// Stages the scenario's first kWallTile wall segments in LDS (before the
+// per-agent work, so the loads overlap it).  Returns the count staged.
+__device__ __forceinline__ int stage_walls(const FusedArgs &fa, const ScnDesc &d, double4 *sseg,
+                                           bool fused) {
+  if (!fused || !(fa.flags & CD_USE_OBSTACLE)) return 0;
+  const int cnt = min(kWallTile, d.w1 - d.w0);
+  for (int k = threadIdx.x; k < cnt; k += blockDim.x) sseg[k] = fa.segs[d.w0 + k];
+  return cnt;
 }
 
-// FUSED = false: the AgentAgentInteractions node, force (torque) += pairs.
-// FUSED = true: all per-agent forces of one step in the post-order of the
-// standard logic tree (fluctuation, navigation, orientation, adjusting,
-// pairs, walls), added to force (torque).
-template <typename T, int MODEL, bool FUSED>
-__global__ void __launch_bounds__(256) k_forces(Agents<T> A, Work W, Batch B, FusedArgs fa) {
-  __shared__ double4 sseg[kSegTile];
-  const int2 sk = block_e(B);
-  const ScnDesc d = desc_of(B, sk.x);
-  const ScnState *ss = &B.sst[sk.x];
-  const int i = d.a0 + sk.y * (int)(blockDim.x >> 2) + (threadIdx.x >> 2);
+// All force terms of agent i (four lanes; q = lane in the group of four).
+// Every lane of the block calls this (have = false for idle groups), so the
+// barriers of the wall staging and the shuffles are uniform.  The first
+// kWallTile segments are already in sseg (stage_walls + a barrier).
+template <typename T, int MODEL, bool FUSED, class Rows>
+__device__ __forceinline__ void agent_forces(const Agents<T> &A, const FusedArgs &fa,
+                                             const ScnDesc &d, uint64_t step, double4 *sseg,
+                                             bool have, int i, double2 pos, V2<T> vel,
+                                             const TCView &me, const Rows &rows) {
   const int q = threadIdx.x & 3;
   const int lane0 = (threadIdx.x & 63) & ~3;
-  const bool pairs_on = FUSED ? ((fa.flags & CD_USE_AGENT_AGENT) != 0) : true;
-  const bool binned = pairs_on && ss->valid && !ss->oob;
-  if (!FUSED && !binned) return;  // node mode: binning invalid, nothing to add
-  const bool have = i < d.a1;
-  double2 pos = make_double2(0.0, 0.0);
-  V2<T> vel;
-  vel.x = T(0);
-  vel.y = T(0);
-  if (have) {
-    pos = A.pos[i];
-    vel = A.vel[i];
-  }
-  // lane 3 loads the agent's own terms first, so their latency overlaps the
-  // pair work of the other lanes
   V2<T> f;
   f.x = T(0);
   f.y = T(0);
   double tq = 0.0;
-  V2<T> e;
-  e.x = T(0);
-  e.y = T(0);
-  bool nav_hit = false;
-  double nex = 0.0, ney = 0.0;
-  double nfx = 0.0, nfy = 0.0, nm = 0.0;
   if (have && q == 3) {
     f = A.force[i];
     if (MODEL) tq = A.torque[i];
-    if (FUSED) {
-      e = A.e0[i];
-      if (fa.flags & CD_USE_NAV)
-        nav_hit = nav_lookup(pos.x, pos.y, A.target[i], fa.maps, fa.nav_T, fa.nav_rows,
-                             fa.nav_cols, fa.nav_minx, fa.nav_miny, fa.nav_step, fa.nav_mode,
-                             nex, ney);
-      // the noise is pure arithmetic: evaluate it while the other lanes wait
-      // for their neighbour data
-      if (fa.flags & CD_USE_FLUCT)
-        fluct_noise(d.seed, W.st->step, i - d.a0, A.mass[i], A.sig_f[i], MODEL != 0,
-                    MODEL ? A.inertia[i] : 0.0, MODEL ? A.sig_t[i] : 0.0, nfx, nfy, nm);
+    if (FUSED) own_terms<T, MODEL>(A, fa, d.seed, step, i, i - d.a0, pos, vel, f, tq);
+  }
+  PairSum ps;
+  if (have && q < 3) {
+    const Owner<T> o = owner_of(A, i);
+    const double ri = A.radius[i];
+    if constexpr (MODEL == 0) {
+      rows.each(q, i, [&](int, double2 xj, V2<T> vj, double rj) {
+        pair_add_c<T>(pos, vel, ri, o, xj, vj, rj, ps);
+      });
+    } else {
+      rows.each(q, i, [&](int j, const TCView &other) {
+        if constexpr (sizeof(T) == 8) pair_add_tc(i, j, me, other, o, ps);
+      });
     }
   }
-  T px = T(0), py = T(0);  // this lane's pair partial sum
-  double ptq = 0.0;
-  if (have && binned && q < 3) {
-    const int c = W.cell_of[i] - d.c0;
-    const int ny = ss->ny, nx = ss->nx;
-    const int lx = c / ny, ly = c - lx * ny;
-    const int ox = lx + q - 1;
-    if (ox >= 0 && ox < nx) {
-      const Owner<T> o = owner_of(A, i);
-      TCView me;
-      if constexpr (MODEL == 1) me = tc_agent(A, i);
-      const double ri = A.radius[i];
-      const int ylo = max(ly - 1, 0), yhi = min(ly + 1, ny - 1);
-      for (int oy = ylo; oy <= yhi; ++oy)
-        cell_pairs<T, MODEL>(A, W, d.c0 + ox * ny + oy, d.a0, d.a1, i, pos, vel, ri, o, me, px,
-                             py, ptq);
-    }
-  }
-  T wx = T(0), wy = T(0);  // this lane's wall partial sum
+  T wx = T(0), wy = T(0);
   double wtq = 0.0;
   if (FUSED && (fa.flags & CD_USE_OBSTACLE)) {
     const int nw = d.w1 - d.w0;
-    for (int w0 = 0; w0 < nw; w0 += kSegTile) {
-      const int cnt = min(kSegTile, nw - w0);
+    if (have) walls_agent(A, MODEL, i, pos, vel, sseg, min(kWallTile, nw), wx, wy, wtq, q, 4);
+    for (int w0 = kWallTile; w0 < nw; w0 += kWallTile) {  // more segments than staged
+      const int cnt = min(kWallTile, nw - w0);
       __syncthreads();
       for (int k = threadIdx.x; k < cnt; k += blockDim.x) sseg[k] = fa.segs[d.w0 + w0 + k];
       __syncthreads();
       if (have) walls_agent(A, MODEL, i, pos, vel, sseg, cnt, wx, wy, wtq, q, 4);
     }
+    if (nw > kWallTile) {  // restore the first tile for the next agents
+      __syncthreads();
+      stage_walls(fa, d, sseg, true);
+      __syncthreads();
+    }
   }
-  // partial sums of lanes 0..3 to lane 3 (every lane takes part in the shuffles)
-  const T p0x = __shfl(px, lane0 + 0, 64), p0y = __shfl(py, lane0 + 0, 64);
-  const T p1x = __shfl(px, lane0 + 1, 64), p1y = __shfl(py, lane0 + 1, 64);
-  const T p2x = __shfl(px, lane0 + 2, 64), p2y = __shfl(py, lane0 + 2, 64);
-  const double t0 = __shfl(ptq, lane0 + 0, 64), t1 = __shfl(ptq, lane0 + 1, 64),
-               t2 = __shfl(ptq, lane0 + 2, 64);
+  // exact sums of lanes 0..2 on lane 3 (every lane takes part in the shuffles)
+  Fix128 sx, sy, sm;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    Fix128 t;
+    t.hi = __shfl(ps.x.hi, lane0 + k, 64);
+    t.lo = __shfl(ps.x.lo, lane0 + k, 64);
+    t.bad = __shfl((int)ps.x.bad, lane0 + k, 64) != 0;
+    sx.add(t);
+    t.hi = __shfl(ps.y.hi, lane0 + k, 64);
+    t.lo = __shfl(ps.y.lo, lane0 + k, 64);
+    t.bad = __shfl((int)ps.y.bad, lane0 + k, 64) != 0;
+    sy.add(t);
+    if (MODEL) {
+      t.hi = __shfl(ps.m.hi, lane0 + k, 64);
+      t.lo = __shfl(ps.m.lo, lane0 + k, 64);
+      t.bad = __shfl((int)ps.m.bad, lane0 + k, 64) != 0;
+      sm.add(t);
+    }
+  }
   T w0x = T(0), w0y = T(0), w1x = T(0), w1y = T(0), w2x = T(0), w2y = T(0);
   double u0 = 0.0, u1 = 0.0, u2 = 0.0;
   if (FUSED) {
@@ -655,29 +789,9 @@ __global__ void __launch_bounds__(256) k_forces(Agents<T> A, Work W, Batch B, Fu
     u2 = __shfl(wtq, lane0 + 2, 64);
   }
   if (!have || q != 3) return;
-  if (FUSED) {
-    if (fa.flags & CD_USE_FLUCT) {
-      f.x += (T)nfx;
-      f.y += (T)nfy;
-      if (MODEL) tq += nm;
-    }
-    if (nav_hit) {
-      e.x = (T)nex;
-      e.y = (T)ney;
-      A.e0[i] = e;
-    }
-    if (fa.flags & CD_USE_ADJUST) {
-      if (MODEL) A.phi0[i] = atan2((double)e.y, (double)e.x);
-      T ax, ay;
-      adjust_force(A, i, vel, e, ax, ay);
-      f.x += ax;
-      f.y += ay;
-      if (MODEL) tq += adjust_torque(A, i);
-    }
-  }
-  f.x += (p0x + p1x) + p2x;
-  f.y += (p0y + p1y) + p2y;
-  if (MODEL) tq += (t0 + t1) + t2;
+  f.x += (T)sx.value();
+  f.y += (T)sy.value();
+  if (MODEL) tq += sm.value();
   if (FUSED) {
     f.x += (w0x + w1x) + (w2x + wx);
     f.y += (w0y + w1y) + (w2y + wy);
@@ -685,6 +799,220 @@ __global__ void __launch_bounds__(256) k_forces(Agents<T> A, Work W, Batch B, Fu
   }
   A.force[i] = f;
   if (MODEL) A.torque[i] = tq;
+}
+
+// Direct kernel: blocks of blockDim / 4 agents of one scenario, in agent
+// index order.  FUSED = false: the AgentAgentInteractions node, force
+// (torque) += pairs.  FUSED = true: all per-agent forces of one step in the
+// post-order of the standard logic tree (fluctuation, navigation,
+// orientation, adjusting, pairs, walls), added to force (torque).  Nothing
+// runs when the scenario's binning is invalid (the integrator then adds the
+// per-agent terms itself).
+template <typename T, int MODEL, bool FUSED>
+__global__ void __launch_bounds__(256) k_forces_direct(Agents<T> A, Work W, Batch B,
+                                                       FusedArgs fa) {
+  __shared__ double4 sseg[kWallTile];
+  const int2 sk = block_e(B);
+  const ScnDesc d = desc_of(B, sk.x);
+  const ScnState *ss = &B.sst[sk.x];
+  const bool pairs_on = FUSED ? ((fa.flags & CD_USE_AGENT_AGENT) != 0) : true;
+  const bool ok = pairs_on && ss->valid && !ss->oob;
+  if (sk.y == 0 && threadIdx.x == 0) B.sst[sk.x].pairs_done = ok ? 1 : 0;
+  if (!ok) return;
+  stage_walls(fa, d, sseg, FUSED);
+  const int i = d.a0 + sk.y * (int)(blockDim.x >> 2) + (threadIdx.x >> 2);
+  const bool have = i < d.a1;
+  double2 pos = make_double2(0.0, 0.0);
+  V2<T> vel;
+  vel.x = T(0);
+  vel.y = T(0);
+  int c = d.c0;
+  TCView me;
+  if (have) {
+    pos = A.pos[i];
+    vel = A.vel[i];
+    c = W.cell_of[i];
+    if constexpr (MODEL == 1) me = tc_agent(A, i);
+  }
+  const int nx = ss->nx, ny = ss->ny;
+  const int lc = c - d.c0, lx = lc / ny, ly = lc - lx * ny;
+  const MemRow<T, MODEL> rows{A, W, d.c0, nx, ny, lx, ly, d.a0, d.a1, false};
+  __syncthreads();  // walls staged
+  agent_forces<T, MODEL, FUSED>(A, fa, d, W.st->step - 1, sseg, have, i, pos, vel, me, rows);
+}
+
+struct TileArgs {
+  int tx, ty;  // cells per tile
+};
+
+// Stages the region of w x h inner cells at grid cell (cx0, cy0) and serves
+// its agents.  Returns false, having done nothing, when the region holds
+// more agents than the LDS capacity.
+template <typename T, int MODEL, bool FUSED>
+__device__ bool tile_region(const Agents<T> &A, const Work &W, const FusedArgs &fa,
+                            const ScnDesc &d, int nx, int ny, uint64_t step, TileLds<T, MODEL> &S,
+                            double4 *sseg, int cx0, int cy0, int w, int h) {
+  constexpr int CAP = TileLds<T, MODEL>::CAP;
+  const int tid = threadIdx.x;
+  const int RH = h + 2, NC = (w + 2) * RH;
+  const int rx0 = cx0 - 1, ry0 = cy0 - 1;
+  auto gcell = [&](int k) { return d.c0 + (rx0 + k / RH) * ny + (ry0 + k % RH); };
+  auto count_of = [&](int k) -> int {
+    if (k >= NC) return 0;
+    const int lx = rx0 + k / RH, ly = ry0 + k % RH;
+    if (lx < 0 || lx >= nx || ly < 0 || ly >= ny) return 0;
+    return W.count[d.c0 + lx * ny + ly];
+  };
+  // 1. counts and offsets (two cells per thread: NC <= 2 * kTileThreads)
+  const int k0 = 2 * tid, k1 = 2 * tid + 1;
+  const int n0 = count_of(k0), n1 = count_of(k1);
+  const int over = __syncthreads_or(n0 > W.K || n1 > W.K);
+  int M;
+  const int x0 = block_scan_excl(n0 + n1, S.wsum, M);
+  if (M > CAP) return false;
+  if (k0 < NC) {
+    S.cnt[k0] = n0;
+    S.off[k0] = x0;
+  }
+  if (k1 < NC) {
+    S.cnt[k1] = n1;
+    S.off[k1] = x0 + n0;
+  }
+  if (tid == 0) S.off[NC] = M;
+  __syncthreads();
+  // 2. ids from the slot lists
+  for (int e = tid; e < M; e += kTileThreads) {
+    int lo = 0, hi = NC - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (S.off[mid] <= e)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    S.cell[e] = lo;
+    if (S.cnt[lo] <= W.K) S.id[e] = W.ell_ids[(size_t)gcell(lo) * W.K + (e - S.off[lo])];
+  }
+  if (over) {  // members of overfull cells, compacted by an ordered block scan
+    for (int k = 0; k < NC; ++k) {
+      if (S.cnt[k] <= W.K) continue;
+      const int gc = gcell(k);
+      int base = S.off[k];
+      for (int j0 = d.a0; j0 < d.a1; j0 += kTileThreads) {
+        const int j = j0 + tid;
+        const int pred = (j < d.a1 && W.cell_of[j] == gc) ? 1 : 0;
+        int tot;
+        const int r = block_scan_excl(pred, S.wsum, tot);
+        if (pred) S.id[base + r] = j;
+        base += tot;
+      }
+    }
+  }
+  __syncthreads();
+  // 3. neighbour data
+  for (int e = tid; e < M; e += kTileThreads) {
+    const int j = S.id[e];
+    S.pos[e] = A.pos[j];
+    S.vel[e] = A.vel[j];
+    S.r[e] = A.radius[j];
+    if constexpr (MODEL == 1) {
+      S.ls[e] = A.ls[j];
+      S.rs[e] = A.rs[j];
+      S.phi[e] = A.phi[j];
+      S.rt[e] = A.r_t[j];
+      S.rsr[e] = A.r_s[j];
+      S.rts[e] = A.r_ts[j];
+    }
+  }
+  if (tid == 0) {  // own agents: the inner cells, column by column
+    int acc = 0;
+    for (int c = 0; c < w; ++c) {
+      S.own[c] = acc;
+      acc += S.off[(c + 1) * RH + h + 1] - S.off[(c + 1) * RH + 1];
+    }
+    S.own[w] = acc;
+  }
+  __syncthreads();
+  // 4. the agents of the inner cells, four lanes each
+  const int Mo = S.own[w];
+  for (int base = 0; base < Mo; base += kTileGroups) {
+    const int o = base + (tid >> 2);
+    const bool have = o < Mo;
+    int e = 0;
+    if (have) {
+      int c = 0;
+      while (c + 1 < w && S.own[c + 1] <= o) ++c;
+      e = S.off[(c + 1) * RH + 1] + (o - S.own[c]);
+    }
+    const int k = S.cell[e];
+    const int kx = k / RH;
+    const LdsRow<T, MODEL> rows{S, RH, kx, k - kx * RH};
+    TCView me;
+    if constexpr (MODEL == 1) me = tc_lds<T, MODEL>(S, e);
+    agent_forces<T, MODEL, FUSED>(A, fa, d, step, sseg, have, S.id[e], S.pos[e], S.vel[e], me,
+                                  rows);
+  }
+  __syncthreads();
+  return true;
+}
+
+// Serves the agents of grid cell (lx, ly) from memory (exact slow path).
+template <typename T, int MODEL, bool FUSED>
+__device__ void cell_from_memory(const Agents<T> &A, const Work &W, const FusedArgs &fa,
+                                 const ScnDesc &d, int nx, int ny, uint64_t step, double4 *sseg,
+                                 int lx, int ly) {
+  const int gc = d.c0 + lx * ny + ly;
+  const MemRow<T, MODEL> rows{A, W, d.c0, nx, ny, lx, ly, d.a0, d.a1, false};
+  for (int j0 = d.a0; j0 < d.a1; j0 += kTileGroups) {
+    const int i = j0 + (threadIdx.x >> 2);
+    const bool have = i < d.a1 && W.cell_of[i] == gc;
+    double2 pos = make_double2(0.0, 0.0);
+    V2<T> vel;
+    vel.x = T(0);
+    vel.y = T(0);
+    TCView me;
+    if (have) {
+      pos = A.pos[i];
+      vel = A.vel[i];
+      if constexpr (MODEL == 1) me = tc_agent(A, i);
+    }
+    agent_forces<T, MODEL, FUSED>(A, fa, d, step, sseg, have, i, pos, vel, me, rows);
+  }
+}
+
+// Tiled kernel (same contract as k_forces_direct): blocks of a scenario
+// stride over its tiles.
+template <typename T, int MODEL, bool FUSED>
+__global__ void __launch_bounds__(kTileThreads) k_forces_tiled(Agents<T> A, Work W, Batch B,
+                                                                FusedArgs fa, TileArgs ta) {
+  __shared__ TileLds<T, MODEL> S;
+  __shared__ double4 sseg[kWallTile];
+  const int2 sk = B.map_t ? B.map_t[blockIdx.x] : make_int2(0, (int)blockIdx.x);
+  const ScnDesc d = desc_of(B, sk.x);
+  const int nb = B.map_t ? d.nt : (int)gridDim.x;
+  const ScnState *ss = &B.sst[sk.x];
+  const bool pairs_on = FUSED ? ((fa.flags & CD_USE_AGENT_AGENT) != 0) : true;
+  const bool ok = pairs_on && ss->valid && !ss->oob;
+  if (sk.y == 0 && threadIdx.x == 0) B.sst[sk.x].pairs_done = ok ? 1 : 0;
+  if (!ok) return;
+  const int nx = ss->nx, ny = ss->ny;
+  const int ntx = (nx + ta.tx - 1) / ta.tx, nty = (ny + ta.ty - 1) / ta.ty;
+  const int ntiles = ntx * nty;
+  if (sk.y >= ntiles) return;
+  stage_walls(fa, d, sseg, FUSED);
+  __syncthreads();
+  const uint64_t step = W.st->step - 1;  // counted by K_a of this step
+  for (int t = sk.y; t < ntiles; t += nb) {
+    const int cx0 = (t / nty) * ta.tx, cy0 = (t % nty) * ta.ty;
+    const int w = min(ta.tx, nx - cx0), h = min(ta.ty, ny - cy0);
+    if (tile_region<T, MODEL, FUSED>(A, W, fa, d, nx, ny, step, S, sseg, cx0, cy0, w, h))
+      continue;
+    for (int c = 0; c < w * h; ++c) {
+      const int lx = cx0 + c / h, ly = cy0 + c % h;
+      if (!tile_region<T, MODEL, FUSED>(A, W, fa, d, nx, ny, step, S, sseg, lx, ly, 1, 1))
+        cell_from_memory<T, MODEL, FUSED>(A, W, fa, d, nx, ny, step, sseg, lx, ly);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -776,11 +1104,14 @@ __global__ void __launch_bounds__(256) k_integrate_node(Agents<T> A, int model, 
   }
 }
 
-// Fused step: dt from the values K_a reduced (scenario state); Verlet and
-// Reset of the scenario's agents (grid-stride over its blocks); partials of
-// the new state for the next step; zero this step's cell counts.
-template <typename T>
-__global__ void __launch_bounds__(256) k_integrate(Agents<T> A, int model, Work W, Batch B,
+// Fused step: dt from the values K_a reduced (scenario state).  When the
+// force kernel did not serve the scenario (pairs off, or binning invalid),
+// the per-agent terms and walls are added here first, in the same order.
+// Then Verlet and Reset of the scenario's agents (grid-stride over its
+// blocks), partials of the new state for the next step, and zeroing of this
+// step's cell counts.
+template <typename T, int MODEL>
+__global__ void __launch_bounds__(256) k_integrate(Agents<T> A, Work W, Batch B, FusedArgs fa,
                                                    int do_reset) {
   const int2 sk = block_f(B);
   const ScnDesc d = desc_of(B, sk.x);
@@ -788,15 +1119,34 @@ __global__ void __launch_bounds__(256) k_integrate(Agents<T> A, int model, Work 
   unsigned long long prev[R_COUNT];
   for (int k = 0; k < R_COUNT; ++k) prev[k] = ss->red[k];
   const double dt = adaptive_dt(prev, d.dt_min, d.dt_max);
+  const bool own = !((fa.flags & CD_USE_AGENT_AGENT) && ss->pairs_done);
+  const uint64_t step = W.st->step - 1;  // counted by K_a of this step
   const int t0 = sk.y * blockDim.x + threadIdx.x, stride = d.nf * blockDim.x;
   const int clear = ss->C_clear;
   for (int c = t0; c < clear; c += stride) W.count[d.c0 + c] = 0;
   Red r;
   r.init();
   for (int i = d.a0 + t0; i < d.a1; i += stride) {
+    if (own) {
+      V2<T> f = A.force[i];
+      double tq = MODEL ? A.torque[i] : 0.0;
+      const double2 pos = A.pos[i];
+      const V2<T> vel = A.vel[i];
+      own_terms<T, MODEL>(A, fa, d.seed, step, i, i - d.a0, pos, vel, f, tq);
+      if (fa.flags & CD_USE_OBSTACLE) {
+        T wx = T(0), wy = T(0);
+        double wtq = 0.0;
+        walls_agent(A, MODEL, i, pos, vel, fa.segs + d.w0, d.w1 - d.w0, wx, wy, wtq);
+        f.x += wx;
+        f.y += wy;
+        if (MODEL) tq += wtq;
+      }
+      A.force[i] = f;
+      if (MODEL) A.torque[i] = tq;
+    }
     double2 x;
     V2<T> v;
-    verlet_agent(A, model, i, dt, do_reset != 0, x, v);
+    verlet_agent(A, MODEL, i, dt, do_reset != 0, x, v);
     r.add_agent(x, dhypot((double)v.x, (double)v.y), A.v0[i]);
   }
   r = block_reduce(r);
@@ -806,7 +1156,6 @@ __global__ void __launch_bounds__(256) k_integrate(Agents<T> A, int model, Work 
     ss->time_tot += dt;
     ss->oob = 0;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) W.st->step += 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -814,31 +1163,50 @@ __global__ void __launch_bounds__(256) k_integrate(Agents<T> A, int model, Work 
 // ---------------------------------------------------------------------------
 
 
+template <typename T, int MODEL, bool FUSED>
+void launch_forces_m(const Agents<T> &A, const Work &W, const Batch &B, const FusedArgs &fa,
+                     const Launch &L, const cd_workspace_t *w, hipStream_t s) {
+  if (w->tile_x > 0 && w->tile_y > 0) {
+    TileArgs ta;
+    ta.tx = (int)w->tile_x;
+    ta.ty = (int)w->tile_y;
+    hipLaunchKernelGGL((k_forces_tiled<T, MODEL, FUSED>), dim3(L.blocks_t), dim3(kTileThreads), 0,
+                       s, A, W, B, fa, ta);
+  } else {
+    hipLaunchKernelGGL((k_forces_direct<T, MODEL, FUSED>), dim3(L.blocks_e), dim3(L.threads_e), 0,
+                       s, A, W, B, fa);
+  }
+}
+
 template <typename T>
 int launch_forces(const cd_agents_t *a, const Work &W, const Batch &B, const FusedArgs &fa,
-                  bool fused, const Launch &L, hipStream_t s) {
+                  bool fused, const Launch &L, const cd_workspace_t *w, hipStream_t s) {
   const Agents<T> A = typed<T>(a);
-  const dim3 grid(L.blocks_e), block(L.threads_e);
   if (a->model == CD_THREE_CIRCLE) {
-    if (fused)
-      hipLaunchKernelGGL((k_forces<T, 1, true>), grid, block, 0, s, A, W, B, fa);
-    else
-      hipLaunchKernelGGL((k_forces<T, 1, false>), grid, block, 0, s, A, W, B, fa);
+    if constexpr (sizeof(T) == 8) {
+      if (fused)
+        launch_forces_m<T, 1, true>(A, W, B, fa, L, w, s);
+      else
+        launch_forces_m<T, 1, false>(A, W, B, fa, L, w, s);
+    }
   } else {
     if (fused)
-      hipLaunchKernelGGL((k_forces<T, 0, true>), grid, block, 0, s, A, W, B, fa);
+      launch_forces_m<T, 0, true>(A, W, B, fa, L, w, s);
     else
-      hipLaunchKernelGGL((k_forces<T, 0, false>), grid, block, 0, s, A, W, B, fa);
+      launch_forces_m<T, 0, false>(A, W, B, fa, L, w, s);
   }
   return check_launch();
 }
 
-// One step: K_a, K_b, K_e, K_f (events ev[0..4] around them when timed).
-template <typename T>
-int step_kernels(const cd_agents_t *a, const cd_step_params_t *p, const Work &W, const Batch &B,
-                 const Launch &L, hipStream_t s, hipEvent_t *ev) {
-  const Agents<T> A = typed<T>(a);
-  const bool pairs = (p->flags & CD_USE_AGENT_AGENT) != 0;
+Launch batch_launch(const cd_workspace_t *w) {
+  Launch L = single_launch(w);
+  L.blocks_f = w->blocks_f;
+  L.blocks_e = std::max(1, (int)w->blocks_e);
+  L.blocks_t = std::max(1, (int)w->blocks_t);
+  return L;
+}
+
+FusedArgs fused_args(const cd_step_params_t *p) {
   FusedArgs fa;
   fa.flags = p->flags;
   fa.nav_mode = p->nav_mode;
@@ -850,19 +1218,36 @@ int step_kernels(const cd_agents_t *a, const cd_step_params_t *p, const Work &W,
   fa.nav_miny = p->nav_miny;
   fa.nav_step = p->nav_step;
   fa.segs = (const double4 *)p->segments;
+  return fa;
+}
+
+// One step: K_a, K_e (when pairs are on), K_f; events ev[0..3] around them
+// when timed.
+template <typename T>
+int step_kernels(const cd_agents_t *a, const cd_step_params_t *p, const Work &W, const Batch &B,
+                 const Launch &L, const cd_workspace_t *w, hipStream_t s, hipEvent_t *ev) {
+  const Agents<T> A = typed<T>(a);
+  const bool pairs = (p->flags & CD_USE_AGENT_AGENT) != 0;
+  const FusedArgs fa = fused_args(p);
   mark(ev, 0, s);
   hipLaunchKernelGGL(k_bin_insert, dim3(L.blocks_f), dim3(kBS), 0, s, (const double2 *)A.pos,
-                     p->cell_size, W, B, pairs ? 1 : 0);
+                     p->cell_size, W, B, pairs ? 1 : 0, 1);
   mark(ev, 1, s);
-  if (pairs)
-    hipLaunchKernelGGL(k_cell_gather<T>, dim3(grid_for(W.cap, kBS)), dim3(kBS), 0, s, A, W);
+  if (pairs) {
+    const int rc = launch_forces<T>(a, W, B, fa, true, L, w, s);
+    if (rc) return rc;
+  }
   mark(ev, 2, s);
-  int rc = launch_forces<T>(a, W, B, fa, true, L, s);
-  if (rc) return rc;
+  const int reset = (p->flags & CD_USE_RESET) ? 1 : 0;
+  if (a->model == CD_THREE_CIRCLE) {
+    if constexpr (sizeof(T) == 8)
+      hipLaunchKernelGGL((k_integrate<T, 1>), dim3(L.blocks_f), dim3(kBS), 0, s, A, W, B, fa,
+                         reset);
+  } else {
+    hipLaunchKernelGGL((k_integrate<T, 0>), dim3(L.blocks_f), dim3(kBS), 0, s, A, W, B, fa,
+                       reset);
+  }
   mark(ev, 3, s);
-  hipLaunchKernelGGL(k_integrate<T>, dim3(L.blocks_f), dim3(kBS), 0, s, A, (int)a->model, W, B,
-                     (p->flags & CD_USE_RESET) ? 1 : 0);
-  mark(ev, 4, s);
   return check_launch();
 }
 
@@ -875,12 +1260,11 @@ int do_agent_agent(const cd_agents_t *a, double cell, const cd_workspace_t *w, h
   int rc;
   if ((rc = launch_reduce(A, W, B, L.blocks_f, true, s))) return rc;
   hipLaunchKernelGGL(k_bin_insert, dim3(L.blocks_f), dim3(kBS), 0, s, (const double2 *)A.pos,
-                     cell, W, B, 1);
-  hipLaunchKernelGGL(k_cell_gather<T>, dim3(grid_for(W.cap, kBS)), dim3(kBS), 0, s, A, W);
+                     cell, W, B, 1, 0);
   if ((rc = check_launch())) return rc;
   FusedArgs fa;
   memset(&fa, 0, sizeof(fa));
-  return launch_forces<T>(a, W, B, fa, false, L, s);
+  return launch_forces<T>(a, W, B, fa, false, L, w, s);
 }
 
 template <typename T>
@@ -905,7 +1289,7 @@ int do_add_to_cells(const cd_agents_t *a, double cell, const cd_workspace_t *w, 
 // Scenario table and block maps of a batch (host side).
 struct HostBatch {
   std::vector<ScnDesc> desc;
-  std::vector<int2> map_f, map_e;
+  std::vector<int2> map_f, map_e, map_t;
 };
 
 int build_batch(const cd_scenario_t *sc, int S, const cd_workspace_t *w, int64_t n_segments,
@@ -939,10 +1323,13 @@ int build_batch(const cd_scenario_t *sc, int S, const cd_workspace_t *w, int64_t
     f += d.nf;
     const int ne = grid_for(ns, force_threads(w) / 4);
     for (int k = 0; k < ne; ++k) H.map_e.push_back(make_int2(s, k));
+    d.nt = tile_blocks(c.cell_capacity, w);
+    for (int k = 0; k < d.nt; ++k) H.map_t.push_back(make_int2(s, k));
   }
   const int64_t Smax = std::max<int64_t>(1, w->max_scenarios);
   if ((int64_t)H.map_f.size() > max_blocks_f(w->n, Smax) ||
-      (int64_t)H.map_e.size() > max_blocks_e(w->n, Smax))
+      (int64_t)H.map_e.size() > max_blocks_e(w->n, Smax) ||
+      (int64_t)H.map_t.size() > w->cell_capacity + Smax + 1)
     return CD_ERR_WORKSPACE;
   return CD_OK;
 }
@@ -1199,8 +1586,8 @@ int cd_step(const cd_agents_t *a, const cd_step_params_t *p, const cd_workspace_
                                p->dt_min, p->dt_max);
   const Launch L = single_launch(w);
   if (a->real == CD_F64)
-    return step_kernels<double>(a, p, W, B, L, (hipStream_t)stream, nullptr);
-  return step_kernels<float>(a, p, W, B, L, (hipStream_t)stream, nullptr);
+    return step_kernels<double>(a, p, W, B, L, w, (hipStream_t)stream, nullptr);
+  return step_kernels<float>(a, p, W, B, L, w, (hipStream_t)stream, nullptr);
 }
 
 int cd_batch_prepare(const cd_agents_t *a, const cd_step_params_t *p,
@@ -1221,11 +1608,14 @@ int cd_batch_prepare(const cd_agents_t *a, const cd_step_params_t *p,
       (!H.map_e.empty() &&
        hipMemcpyAsync(b + Lw.map_e, H.map_e.data(), sizeof(int2) * H.map_e.size(),
                       hipMemcpyHostToDevice, s) != hipSuccess) ||
+      hipMemcpyAsync(b + Lw.map_t, H.map_t.data(), sizeof(int2) * H.map_t.size(),
+                     hipMemcpyHostToDevice, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess)
     return CD_ERR_HIP;
   w->n_scenarios = n_scenarios;
   w->blocks_f = (int32_t)H.map_f.size();
   w->blocks_e = (int32_t)H.map_e.size();
+  w->blocks_t = (int32_t)H.map_t.size();
   const Work W = carve(w);
   const Batch B = batch_of(w);
   hipLaunchKernelGGL(k_set_step, dim3(1), dim3(1), 0, s, W.st, (unsigned long long)first_step);
@@ -1241,13 +1631,11 @@ int cd_batch_step(const cd_agents_t *a, const cd_step_params_t *p, const cd_work
   if (w->n_scenarios < 1 || w->blocks_f < 1) return CD_ERR_ARG;
   const Work W = carve(w);
   const Batch B = batch_of(w);
-  Launch L = single_launch(w);
-  L.blocks_f = w->blocks_f;
-  L.blocks_e = std::max(1, (int)w->blocks_e);
-  if (w->blocks_e < 1) return CD_OK;  // no agents
+  if (a->n == 0) return CD_OK;
+  const Launch L = batch_launch(w);
   if (a->real == CD_F64)
-    return step_kernels<double>(a, p, W, B, L, (hipStream_t)stream, nullptr);
-  return step_kernels<float>(a, p, W, B, L, (hipStream_t)stream, nullptr);
+    return step_kernels<double>(a, p, W, B, L, w, (hipStream_t)stream, nullptr);
+  return step_kernels<float>(a, p, W, B, L, w, (hipStream_t)stream, nullptr);
 }
 
 int cd_step_timed(const cd_agents_t *a, const cd_step_params_t *p, const cd_workspace_t *w,
@@ -1263,29 +1651,26 @@ int cd_step_timed(const cd_agents_t *a, const cd_step_params_t *p, const cd_work
   const Batch B = batched ? batch_of(w)
                           : single_batch(w, (p->flags & CD_USE_OBSTACLE) ? p->n_segments : 0,
                                          p->seed, p->dt_min, p->dt_max);
-  Launch L = single_launch(w);
-  if (batched) {
-    L.blocks_f = w->blocks_f;
-    L.blocks_e = w->blocks_e;
-  }
-  hipEvent_t ev[5];
-  for (int k = 0; k < 5; ++k)
+  const Launch L = batched ? batch_launch(w) : single_launch(w);
+  hipEvent_t ev[4];
+  for (int k = 0; k < 4; ++k)
     if (hipEventCreate(&ev[k]) != hipSuccess) return CD_ERR_HIP;
   for (int64_t it = 0; it < steps && rc == CD_OK; ++it) {
-    rc = a->real == CD_F64 ? step_kernels<double>(a, p, W, B, L, s, ev)
-                           : step_kernels<float>(a, p, W, B, L, s, ev);
+    rc = a->real == CD_F64 ? step_kernels<double>(a, p, W, B, L, w, s, ev)
+                           : step_kernels<float>(a, p, W, B, L, w, s, ev);
     if (rc) break;
-    if (hipEventSynchronize(ev[4]) != hipSuccess) {
+    if (hipEventSynchronize(ev[3]) != hipSuccess) {
       rc = CD_ERR_HIP;
       break;
     }
+    const int from[4] = {0, 1, 2, 0}, to[4] = {1, 2, 3, 3};
     for (int k = 0; k < 4; ++k) {
       float ms = 0.f;
-      if (hipEventElapsedTime(&ms, ev[k], ev[k + 1]) != hipSuccess) rc = CD_ERR_HIP;
+      if (hipEventElapsedTime(&ms, ev[from[k]], ev[to[k]]) != hipSuccess) rc = CD_ERR_HIP;
       kernel_ms[k] += ms;
     }
   }
-  for (int k = 0; k < 5; ++k) (void)hipEventDestroy(ev[k]);
+  for (int k = 0; k < 4; ++k) (void)hipEventDestroy(ev[k]);
   return rc;
 }
 

@@ -1,5 +1,5 @@
 """Batched ensemble: independent scenarios of uneven agent counts advanced
-together, four kernel launches per step over all of them (capturable in a
+together, three kernel launches per step over all of them (capturable in a
 HIP graph).
 
 The reference runs one simulation per process
@@ -47,7 +47,7 @@ class Ensemble(Stepper):
     _entry = 'cd_batch_step'
 
     def __init__(self, scenarios, precision='f32', flags=ALL_NODES, cell_size=None,
-                 cell_slots=None, force_threads=0, first_step=0, device=None, margin=0.5):
+                 cell_slots=None, tile=None, first_step=0, device=None, margin=0.5):
         scenarios = list(scenarios)
         if not scenarios:
             raise InvalidValue('an ensemble needs at least one scenario')
@@ -100,7 +100,7 @@ class Ensemble(Stepper):
                              for s in scenarios)
         self.ws = Workspace(self.n, int(self.cell_offsets[-1]), self.agents.model,
                             cell_slots=cell_slots, device=dev, max_scenarios=self.S,
-                            force_threads=force_threads)
+                            tile=tile)
         self.table = (_lib.CDScenario * self.S)()
         for k, s in enumerate(scenarios):
             t = self.table[k]

@@ -1,4 +1,4 @@
-"""Fused multi-step driver over cd_step (four kernels per step) with HIP
+"""Fused multi-step driver over cd_step (three kernels per step) with HIP
 graph capture.
 
 ``Stepper`` runs one MultiAgentSimulation.update() of the standard logic
@@ -56,7 +56,7 @@ class Stepper:
     def __init__(self, array, segments=None, nav=None, cell_size=3.6, dt_min=0.01,
                  dt_max=0.01, seed=1, flags=ALL_NODES, precision='f64', cell_capacity=None,
                  bounds=None, first_step=0, device=None, device_agents=None,
-                 cell_slots=None, force_threads=0):
+                 cell_slots=None, force_threads=0, tile=None):
         if device_agents is not None:
             self.agents = device_agents
         else:
@@ -86,7 +86,7 @@ class Stepper:
         if cell_slots is None:
             cell_slots = cell_slots_for(cell_size, min_body_radius(array))
         self.ws = Workspace(self.n, cell_capacity, self.agents.model, cell_slots=cell_slots,
-                            device=dev, force_threads=force_threads)
+                            device=dev, force_threads=force_threads, tile=tile)
         self.params = make_params(cell_size, dt_min, dt_max, flags, seed, self.segments,
                                   self.maps, nav)
         self.flags = flags
@@ -130,8 +130,9 @@ class Stepper:
         return self.ws.state()
 
     def timed(self, steps):
-        """Per-kernel GPU milliseconds (bin/reduce, gather, forces, integrate)
-        summed over `steps` steps, from HIP events on the current stream."""
+        """GPU milliseconds of the binning, force and integrate kernels and of
+        the whole step, summed over `steps` steps, from HIP events on the
+        current stream."""
         ms = (C.c_double * 4)()
         _lib.check(_lib.lib().cd_step_timed(self.agents.ref(), C.byref(self.params),
                                             self.ws.ref(), int(steps), ms, stream_handle()),

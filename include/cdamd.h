@@ -5,7 +5,8 @@
  * owned by the caller.  Every call is stream-ordered on `stream` (a
  * hipStream_t), performs no host synchronisation and no allocation, and may
  * be captured into a HIP graph, except the calls documented as synchronous
- * (cd_read_state, cd_step_timed, cd_graph_*).  Calls return 0 on success or
+ * (cd_read_state, cd_batch_prepare, cd_batch_read_states, cd_step_timed,
+ * cd_graph_*).  Calls return 0 on success or
  * a negative CD_ERR_* code; kernels record data-dependent conditions (grid
  * overflow, non-finite positions, full cell slot lists) as sticky bits in
  * the workspace state, read back by cd_read_state.
@@ -79,7 +80,12 @@ typedef struct cd_workspace {
   int32_t max_scenarios; /* scenario table size (0 or 1: single scenario) */
   /* set by cd_batch_prepare: */
   int32_t n_scenarios, blocks_f, blocks_e;
-  int32_t force_threads; /* threads per force block, 4 per agent: 64, 128 or 256 (0: 256) */
+  int32_t force_threads; /* threads per block of the adjoint's pair kernel, 4 per agent:
+                            64, 128 or 256 (0: 256) */
+  int32_t tile_x, tile_y; /* force kernel: 0 = agent-parallel; else grid cells per
+                             LDS tile, each 1..16 (same results either way) */
+  int32_t blocks_t;       /* set by cd_batch_prepare */
+  int32_t _pad;
 } cd_workspace_t;
 
 /* One scenario of a batch: its agents are [agent_begin, agent_end) of the
@@ -214,7 +220,8 @@ typedef struct cd_step_params {
 /* One MultiAgentSimulation.update() (crowddynamics/simulation/multiagent.py:51-55)
  * of the tree Reset << (Integrator << (Fluctuation, Adjusting << (Navigation,
  * Orientation), AgentAgentInteractions, AgentObstacleInteractions))
- * (crowddynamics/examples/simulations.py:123-136) in four kernels.  The
+ * (crowddynamics/examples/simulations.py:123-136) in three kernels (binning,
+ * LDS-tiled forces, integration).  The
  * fluctuation step counter and every reduction stay on the device.
  * cd_step_prepare must run once before the first cd_step (and after any
  * host-side change of positions or velocities). */
@@ -224,7 +231,7 @@ int cd_step(const cd_agents_t *agents, const cd_step_params_t *params, const cd_
             cd_stream_t stream);
 
 /* Batched ensemble: independent scenarios advanced together, one launch per
- * step kernel over all scenarios (four launches per step, graph-capturable).
+ * step kernel over all scenarios (three launches per step, graph-capturable).
  * Synchronous: uploads the scenario table and block maps into the
  * workspace (which needs max_scenarios >= n_scenarios) and fills ws->
  * n_scenarios / blocks_f / blocks_e.  params supplies the cell size, flags,
@@ -260,9 +267,10 @@ int cd_adjoint_step(const cd_agents_t *agents, const cd_step_params_t *params,
                     const double *grad_x, const cd_adjoint_t *adj, const cd_workspace_t *ws,
                     cd_stream_t stream);
 
-/* Synchronous: runs `steps` cd_step calls and returns in kernel_ms[4] the
- * summed GPU time of each step kernel (bin/reduce, sort+gather, forces,
- * integrate), measured with HIP events recorded on `stream` around each
+/* Synchronous: runs `steps` steps (cd_step, or cd_batch_step on a workspace
+ * prepared by cd_batch_prepare) and returns in kernel_ms[4] the summed GPU
+ * time of the binning kernel, the force kernel, the integrate kernel and the
+ * whole step, measured with HIP events recorded on `stream` around each
  * launch.  No reference counterpart (measurement). */
 int cd_step_timed(const cd_agents_t *agents, const cd_step_params_t *params,
                   const cd_workspace_t *ws, int64_t steps, double *kernel_ms,

@@ -79,16 +79,20 @@ def test_ensemble_equals_standalone(precision):
     _compare(outs, refs, FIELDS)
 
 
-def test_ensemble_graph_replay_and_force_blocks():
-    """Graph replays equal eager batched steps; the force block size
-    (64/128/256 threads) changes nothing."""
+@pytest.mark.parametrize('cell_size', [3.6, 8.0])
+def test_ensemble_graph_replay_and_tiles(cell_size):
+    """Graph replays equal eager batched steps, and the force kernel changes
+    nothing, bit for bit: the agent-parallel kernel and LDS tiles of 1 x 1
+    up to 16 x 16 cells (at 8 m cells the large tiles exceed the LDS
+    capacity and fall back to 3 x 3 regions)."""
     scns = _small_ensemble(S=5, seed=11)
     stream = torch.cuda.Stream()
     res = []
-    for ft, graph in ((256, False), (256, True), (64, False), (128, True)):
-        say('force_threads', ft, 'graph', graph)
+    for tile, graph in (((0, 0), False), ((0, 0), True), ((1, 1), False), ((3, 5), True),
+                        ((16, 16), False)):
+        say('tile', tile, 'graph', graph, 'cell', cell_size)
         with torch.cuda.stream(stream):
-            ens = Ensemble(scns, precision='f32', force_threads=ft)
+            ens = Ensemble(scns, precision='f32', tile=tile, cell_size=cell_size)
             if graph:
                 g = ens.capture(10)
                 for _ in range(3):
@@ -147,7 +151,7 @@ def test_autotuned_cell_sizes_are_physics_neutral():
             assert np.max(np.abs(a['velocity'] - b['velocity'])) <= 1e-9
     with torch.cuda.stream(stream):
         best = tune_ensemble(scns, steps=3, warmup=1, cell_sizes=sizes[:2],
-                             force_threads=(64, 256))
-    say('autotune', best['cell_size'], best['force_threads'], best['ms_per_step'])
-    assert best['cell_size'] in sizes[:2] and best['force_threads'] in (64, 256)
+                             tiles=((2, 2), (4, 4)))
+    say('autotune', best['cell_size'], best['tile'], best['ms_per_step'])
+    assert best['cell_size'] in sizes[:2] and best['tile'] in ((2, 2), (4, 4))
     assert len(best['table']) == 4

@@ -162,6 +162,20 @@ def test_pairs_slot_overflow_exact_path():
     _check_pairs(a, 1.2, slots=2)
 
 
+def test_pairs_large_cells_global_path():
+    """Cells of 10 m holding ~170 agents each: tiles exceed the LDS capacity,
+    3 x 3 regions around inner cells still do, and the centre cell's region
+    does not (served from memory).  Same forces on every path."""
+    rng = np.random.default_rng(17)
+    n = 1500
+    a = empty_agents(n)
+    a['radius'] = rng.uniform(0.22, 0.28, n)
+    a['mass'] = rng.normal(80, 8, n)
+    a['position'] = rng.uniform(0, 29.9, (n, 2))
+    a['velocity'] = rng.normal(0, 1, (n, 2))
+    _check_pairs(a, 10.0)
+
+
 def test_pairs_contact_heavy():
     """Overlapping agents (contact + friction + damping terms active)."""
     rng = np.random.default_rng(11)
