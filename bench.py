@@ -93,7 +93,7 @@ def cpu_baseline(scn, seconds, threads=None):
     return scn.n * steps / el, steps, el
 
 
-def bench_ensemble(n_scenarios, steps, loop_steps, stream):
+def bench_ensemble(n_scenarios, steps, loop_steps, stream, tune=True):
     """Batched ensemble of uneven rooms (fp32, fp64 positions): autotuned
     batched steps replayed from a HIP graph versus a per-scenario loop of
     plain GPU steps (one Stepper per scenario, same cell size and
@@ -108,10 +108,15 @@ def bench_ensemble(n_scenarios, steps, loop_steps, stream):
     n = sum(s.n for s in scns)
     with torch.cuda.stream(stream):
         sizes = neutral_cell_sizes(max_body_radius(scns))[:3]
-        tuned = tune_ensemble(scns, precision='f32', cell_sizes=sizes, steps=5, warmup=2)
-        log('autotune: cell %.3f, tile %s (%.3f ms/step)' % (
-            tuned['cell_size'], tuned['tile'], tuned['ms_per_step']))
-        ens = Ensemble(scns, precision='f32', cell_size=tuned['cell_size'], tile=tuned['tile'])
+        if tune:
+            tuned = tune_ensemble(scns, precision='f32', cell_sizes=sizes, steps=5, warmup=2)
+        else:
+            tuned = dict(cell_size=sizes[0], tile=(0, 0), lanes=4, ms_per_step=float('nan'),
+                         table=[])
+        log('autotune: cell %.3f, tile %s, lanes %d (%.3f ms/step)' % (
+            tuned['cell_size'], tuned['tile'], tuned['lanes'], tuned['ms_per_step']))
+        ens = Ensemble(scns, precision='f32', cell_size=tuned['cell_size'], tile=tuned['tile'],
+                       lanes=tuned['lanes'])
         ens.step(10)
         G = 50
         graph = ens.capture(G)
@@ -150,8 +155,8 @@ def bench_ensemble(n_scenarios, steps, loop_steps, stream):
                 ms_per_step=1e3 * el / (reps * G),
                 kernel_ms={k: kms[i] / 20 for i, k in enumerate(KERNELS)},
                 autotuned=dict(cell_size=tuned['cell_size'],
-                               tile=tuned['tile'],
-                               candidates=[dict(ms=t[0], cell_size=t[1], tile=t[2])
+                               tile=tuned['tile'], lanes=tuned['lanes'],
+                               candidates=[dict(ms=t[0], cell_size=t[1], kernel=t[2])
                                            for t in tuned['table']]),
                 per_scenario_loop=dict(value=loop, steps=loop_steps,
                                        ms_per_step=1e3 * el_loop / loop_steps),
@@ -201,8 +206,6 @@ def bench_adjoint(n, steps, checkpoint):
     log('adjoint: n=%d, %d steps, checkpoint %d' % (n, steps, checkpoint))
     scn = E.rollout(n=n, steps=steps)
     arr = scn.agents
-    R = Rollout(arr, steps=steps, dt=scn.dt_max, cell_size=scn.cell_size, checkpoint=checkpoint,
-                bounds=scn.bounds)
     dev = 'cuda'
     f64 = torch.float64
 
@@ -210,6 +213,25 @@ def bench_adjoint(n, steps, checkpoint):
         ps = [torch.tensor(3.0), torch.tensor(1.2e5), torch.tensor(4e4), torch.tensor(500.0),
               torch.as_tensor(np.array(arr['target_velocity'], dtype=np.float64))]
         return [p.to(device=dev, dtype=f64).requires_grad_(grad) for p in ps]
+
+    def rollout(k, lanes):
+        return Rollout(arr, steps=k, dt=scn.dt_max, cell_size=scn.cell_size,
+                       checkpoint=checkpoint, bounds=scn.bounds, lanes=lanes)
+
+    # lanes per agent: timed on a short rollout (forward + backward)
+    lane_ms = {}
+    for lanes in (4, 8, 16):
+        Rt = rollout(2 * checkpoint, lanes)
+        p = params(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        (rollout_positions(Rt, *p) ** 2).sum().backward()
+        torch.cuda.synchronize()
+        lane_ms[lanes] = 1e3 * (time.perf_counter() - t0) / (2 * checkpoint)
+        del Rt
+    lanes = min(lane_ms, key=lane_ms.get)
+    log('adjoint lanes per agent %d (%s ms/step)' % (lanes, lane_ms))
+    R = rollout(steps, lanes)
 
     rng = np.random.default_rng(0)
     q = params()
@@ -232,7 +254,7 @@ def bench_adjoint(n, steps, checkpoint):
     n_par = 4 + n
     log('adjoint: forward %.3f ms/step, backward %.3f ms/step' % (1e3 * fwd / steps,
                                                                    1e3 * bwd / steps))
-    return dict(agents=n, steps=steps, checkpoint=checkpoint, dt=scn.dt_max,
+    return dict(agents=n, steps=steps, checkpoint=checkpoint, dt=scn.dt_max, lanes=lanes,
                 loss=float(L.detach()), forward_ms_per_step=1e3 * fwd / steps,
                 backward_ms_per_step=1e3 * bwd / steps,
                 adjoint_step_ms=1e3 * (fwd + bwd) / steps,
@@ -263,6 +285,9 @@ def main():
     ap.add_argument('--no-corridor', action='store_true')
     ap.add_argument('--no-adjoint', action='store_true')
     ap.add_argument('--adjoint-steps', type=int, default=500)
+    ap.add_argument('--no-tune', action='store_true',
+                    help='skip the autotuner (agent-parallel force kernel, smallest '
+                         'physics-neutral ensemble cell)')
     ap.add_argument('--calibrate-copy', action='store_true',
                     help='first run a plain 256 MiB device copy five times (a kernel of '
                          'known traffic for scaling hardware byte counters)')
@@ -290,20 +315,23 @@ def main():
     scn = E.room(n=args.n)
     results = {}
     kernel_ms = {}
+    kernel_b2b = {}
     stream = torch.cuda.Stream()
-    from crowddynamics_amd.autotune import tune_tile
+    from crowddynamics_amd.autotune import tune_kernel
 
-    def make(prec, tile):
+    def make(prec, tile, lanes):
         return Stepper(scn.agents.copy(), segments=scn.segments, nav=scn.nav,
                        cell_size=scn.cell_size, dt_min=scn.dt_min, dt_max=scn.dt_max,
-                       seed=scn.seed, precision=prec, bounds=scn.bounds, tile=tile)
+                       seed=scn.seed, precision=prec, bounds=scn.bounds, tile=tile, lanes=lanes)
 
-    with torch.cuda.stream(stream):
-        tile, tile_table = tune_tile(lambda t: make(args.precision, t))
-    log('force tile %s (candidates %s)' % (tile, [(t[1], round(t[0], 4)) for t in tile_table]))
+    kern, kern_table = (0, 0, 4), []
+    if not args.no_tune:
+        with torch.cuda.stream(stream):
+            kern, kern_table = tune_kernel(lambda t, l: make(args.precision, t, l))
+    log('force kernel %s (candidates %s)' % (kern, [(t[1], round(t[0], 4)) for t in kern_table]))
     for prec in ('f64', 'f32'):
         with torch.cuda.stream(stream):
-            st = make(prec, tile)
+            st = make(prec, kern[:2], kern[2])
             st.step(args.warmup)
             G = max(1, min(args.graph_steps, args.steps))
             graph = st.capture(G)
@@ -319,10 +347,18 @@ def main():
             el = time.perf_counter() - t0
             state = st.check()
             out = st.download(scn.agents.copy())
-            # per-kernel GPU time, HIP events on the launch stream
+            # per-kernel GPU time, HIP events on the launch stream: inside the
+            # step sequence, and over 20 back-to-back launches of each kernel
+            # (the step's state is discarded afterwards)
             nt = 50
             ms = st.timed(nt)
             kernel_ms[prec] = {k: ms[i] / nt for i, k in enumerate(KERNELS)}
+            b2b = {}
+            for k, name in (('forces', 'forces'), ('integrate', 'integrate'),
+                            ('bin', 'bin_insert')):
+                st.prepare()
+                b2b[name] = st.kernel_timed(k, 20)
+            kernel_b2b[prec] = b2b
             results[prec] = dict(value=scn.n * args.steps / el, ms_per_step=1e3 * el / args.steps,
                                  time_tot=state.time_tot, cells=int(state.shape[0] * state.shape[1]),
                                  out=out)
@@ -332,7 +368,7 @@ def main():
             {k: round(v * 1e3, 2) for k, v in kernel_ms[prec].items()}))
 
     main_p = args.precision
-    km = kernel_ms[main_p]
+    km = kernel_b2b[main_p]
     cells = results[main_p]['cells']
     t0_tally = pair_tally(scn.agents, scn.cell_size, scn.segments)
     t1_tally = pair_tally(results[main_p]['out'], scn.cell_size, scn.segments)
@@ -369,7 +405,7 @@ def main():
     ensemble = None
     if not args.no_ensemble:
         ensemble = bench_ensemble(args.ensemble_scenarios, args.ensemble_steps,
-                                  args.ensemble_loop_steps, stream)
+                                  args.ensemble_loop_steps, stream, tune=not args.no_tune)
     corridor = None
     if not args.no_corridor:
         corridor = bench_corridor(300, stream)
@@ -409,11 +445,12 @@ def main():
         'config': {'workload': 'room: 20k Circular agents, 200x100 m, 4 exits of 1.2 m, '
                                'cell 0.6 m, 0.1 m potential grid, adaptive dt',
                    'agents': scn.n, 'cell_size': scn.cell_size, 'precision': main_p,
-                   'graph_steps': args.graph_steps, 'force_tile': list(tile)},
+                   'graph_steps': args.graph_steps, 'force_kernel': list(kern)},
         'roofline': roofline,
         'cpu_baseline': cpu,
         'fp32_value': results['f32']['value'] if main_p == 'f64' else results['f64']['value'],
-        'kernel_ms': {p: {k: v for k, v in d.items()} for p, d in kernel_ms.items()},
+        'kernel_ms': kernel_b2b,
+        'kernel_ms_in_step': kernel_ms,
         'rooflines': per_kernel,
         'pair_tally': t0_tally,
         'ensemble': ensemble,

@@ -55,7 +55,7 @@ class CDWorkspace(C.Structure):
                 ('max_scenarios', C.c_int32), ('n_scenarios', C.c_int32),
                 ('blocks_f', C.c_int32), ('blocks_e', C.c_int32),
                 ('force_threads', C.c_int32), ('tile_x', C.c_int32), ('tile_y', C.c_int32),
-                ('blocks_t', C.c_int32), ('_pad', C.c_int32)]
+                ('blocks_t', C.c_int32), ('agent_lanes', C.c_int32)]
 
 
 class CDScenario(C.Structure):
@@ -118,8 +118,14 @@ SIGNATURES = {
     'cd_batch_read_states': (C.c_int, [_W, _P(CDState), C.c_int32, _vp]),
     'cd_adjoint_step': (C.c_int, [_P(CDAgents), _P(CDStepParams), _vp, _P(CDAdjoint), _W,
                                   _vp]),
+    'cd_rollout_record': (C.c_int, [_P(CDAgents), _P(CDStepParams), _W, C.c_int64, C.c_int64,
+                                    _vp, _vp, _vp, _vp, _vp]),
+    'cd_adjoint_segment': (C.c_int, [_P(CDAgents), _P(CDStepParams), _W, C.c_int64, C.c_int64,
+                                     _vp, _vp, _vp, _vp, _vp, _vp, _P(CDAdjoint), _vp]),
     'cd_step_timed': (C.c_int, [_P(CDAgents), _P(CDStepParams), _W, C.c_int64,
                                 _P(C.c_double), _vp]),
+    'cd_kernel_timed': (C.c_int, [_P(CDAgents), _P(CDStepParams), _W, C.c_int32, C.c_int64,
+                                  _P(C.c_double), _vp]),
     'cd_graph_begin': (C.c_int, [_vp]),
     'cd_graph_end': (C.c_int, [_vp, _P(_vp)]),
     'cd_graph_launch': (C.c_int, [_vp, _vp]),

@@ -33,7 +33,7 @@ class Rollout:
     """Fixed-dt rollout of `steps` steps from the records `array`."""
 
     def __init__(self, array, steps, dt=0.005, cell_size=3.6, checkpoint=25, seed=1,
-                 fluctuation=False, cell_capacity=None, bounds=None, device=None):
+                 fluctuation=False, cell_capacity=None, bounds=None, device=None, lanes=0):
         if 'position_ls' in array.dtype.names:
             raise InvalidValue('the adjoint covers circular agents')
         if steps < 1 or checkpoint < 1:
@@ -45,7 +45,7 @@ class Rollout:
         self.dt = float(dt)
         self.st = Stepper(array.copy(), cell_size=cell_size, dt_min=dt, dt_max=dt, seed=seed,
                           flags=flags, precision='f64', cell_capacity=cell_capacity,
-                          bounds=bounds, device=device)
+                          bounds=bounds, device=device, lanes=lanes)
         self.agents = self.st.agents
         self.n = self.agents.n
         dev = self.agents.device
@@ -64,7 +64,6 @@ class Rollout:
             setattr(a, k, C.c_void_p(t.data_ptr()))
         a.grad = C.c_void_p(self.grad.data_ptr())
         self.adj_struct = a
-        self.view = _lib.CDAgents.from_buffer_copy(self.agents.struct)
 
     # -- parameters and state -------------------------------------------
     def set_parameters(self, tau_0=None, mu=None, kappa=None, damping=None, v0=None):
@@ -85,29 +84,20 @@ class Rollout:
         self.agents.upload(self.array0, fields=('position', 'velocity', 'force', 'force_prev',
                                                 'target_direction'))
 
-    def _restore(self, s):
-        f = self.agents.fields
-        for k, t in self.ck.items():
-            f[k].copy_(t[s])
-        f['force'].zero_()
-
     # -- passes ------------------------------------------------------------
     def forward(self):
         """Run the rollout from the initial records; returns positions
         [steps + 1, n, 2] (fp64, on the device) and stores checkpoints."""
         self._restore_initial()
         self.st.prepare(0)
-        f = self.agents.fields
         X = torch.empty((self.steps + 1, self.n, 2), dtype=torch.float64,
                         device=self.agents.device)
-        X[0].copy_(f['position'])
-        for k in range(self.steps):
-            if k % self.checkpoint == 0:
-                s = k // self.checkpoint
-                for name, t in self.ck.items():
-                    t[s].copy_(f[name])
-            self.st.step(1)
-            X[k + 1].copy_(f['position'])
+        ck = self.ck
+        _lib.check(_lib.lib().cd_rollout_record(
+            self.agents.ref(), C.byref(self.st.params), self.st.ws.ref(), self.steps,
+            self.checkpoint, C.c_void_p(X.data_ptr()), C.c_void_p(ck['position'].data_ptr()),
+            C.c_void_p(ck['velocity'].data_ptr()), C.c_void_p(ck['force_prev'].data_ptr()),
+            stream_handle()), 'rollout_record')
         self.st.check()
         return X
 
@@ -122,27 +112,17 @@ class Rollout:
         self.adj['lam_x'].copy_(grad_X[self.steps])
         L = _lib.lib()
         s_h = stream_handle()
-        f = self.agents.fields
         n_seg = (self.steps + self.checkpoint - 1) // self.checkpoint
-        view = self.view
+        ck = self.ck
         for s in reversed(range(n_seg)):
             k0 = s * self.checkpoint
             k1 = min(k0 + self.checkpoint, self.steps)
-            self._restore(s)
-            self.st.prepare(k0)
-            self.tape_x[0].copy_(f['position'])
-            self.tape_v[0].copy_(f['velocity'])
-            for k in range(k0, k1 - 1):  # states k0 .. k1 - 1 are needed
-                self.st.step(1)
-                self.tape_x[k + 1 - k0].copy_(f['position'])
-                self.tape_v[k + 1 - k0].copy_(f['velocity'])
-            for k in reversed(range(k0, k1)):
-                view.position = self.tape_x[k - k0].data_ptr()
-                view.velocity = self.tape_v[k - k0].data_ptr()
-                _lib.check(L.cd_adjoint_step(C.byref(view), C.byref(self.st.params),
-                                             C.c_void_p(grad_X[k].data_ptr()),
-                                             C.byref(self.adj_struct), self.st.ws.ref(), s_h),
-                           'adjoint_step')
+            _lib.check(L.cd_adjoint_segment(
+                self.agents.ref(), C.byref(self.st.params), self.st.ws.ref(), k0, k1,
+                C.c_void_p(ck['position'][s].data_ptr()), C.c_void_p(ck['velocity'][s].data_ptr()),
+                C.c_void_p(ck['force_prev'][s].data_ptr()), C.c_void_p(self.tape_x.data_ptr()),
+                C.c_void_p(self.tape_v.data_ptr()), C.c_void_p(grad_X.data_ptr()),
+                C.byref(self.adj_struct), s_h), 'adjoint_segment')
         self.st.check()
         return self.grad.clone(), self.adj['lam_x'].clone(), self.adj['lam_v'].clone()
 

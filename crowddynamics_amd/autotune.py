@@ -33,41 +33,47 @@ def neutral_cell_sizes(r_max, factors=(1.0, 1.15, 1.35, 1.6), sight=SIGHT_SOC):
 
 
 def tune_ensemble(scenarios, precision='f32', cell_sizes=None,
-                  tiles=((0, 0), (1, 1), (2, 2), (4, 4)),
-                  steps=10, warmup=3, flags=None):
-    """Time every (cell size, force tile) pair over `steps` batched steps
+                  kernels=((0, 0, 4), (0, 0, 8), (1, 1, 4), (2, 2, 4)), steps=10, warmup=3,
+                  flags=None):
+    """Time every (cell size, force kernel) pair over `steps` batched steps
     (after `warmup` steps) and return the fastest.
 
-    The force kernel choice (tile (0, 0): agent-parallel; else grid cells
-    per LDS tile) never changes results: pair terms are summed exactly.  Returns
-    dict(cell_size, tile, ms_per_step, table) where table lists
-    (ms_per_step, cell_size, tile, per-kernel ms)."""
+    A force kernel is (tile_x, tile_y, lanes): tile (0, 0) selects the
+    agent-parallel kernel with `lanes` lanes per agent, otherwise LDS tiles of
+    tile_x * tile_y cells.  The choice never changes results (pair and wall
+    terms are summed exactly).  Returns dict(cell_size, tile, lanes,
+    ms_per_step, table) where table lists (ms_per_step, cell_size, kernel,
+    per-kernel ms)."""
     if cell_sizes is None:
         cell_sizes = neutral_cell_sizes(max_body_radius(scenarios))
     kw = {} if flags is None else dict(flags=flags)
     table = []
     for cs in cell_sizes:
-        for tile in tiles:
-            ens = Ensemble(scenarios, precision=precision, cell_size=cs, tile=tile, **kw)
+        for kern in kernels:
+            ens = Ensemble(scenarios, precision=precision, cell_size=cs, tile=kern[:2],
+                           lanes=kern[2], **kw)
             ens.step(warmup)
             ms = ens.timed(steps)
             ens.check()
-            table.append((ms[3] / steps, cs, tuple(tile), [m / steps for m in ms]))
+            table.append((ms[3] / steps, cs, tuple(kern), [m / steps for m in ms]))
             del ens
     best = min(table, key=lambda t: t[0])
-    return dict(cell_size=best[1], tile=best[2], ms_per_step=best[0], table=table)
+    return dict(cell_size=best[1], tile=best[2][:2], lanes=best[2][2], ms_per_step=best[0],
+                table=table)
 
 
-def tune_tile(make_stepper, tiles=((0, 0), (4, 4), (8, 8), (16, 16)), steps=20, warmup=5):
-    """Fastest force tile for one simulation: make_stepper(tile) builds a
-    Stepper on a fresh copy of the state.  Returns (tile, table)."""
+def tune_kernel(make_stepper, kernels=((0, 0, 4), (0, 0, 8), (0, 0, 16), (8, 8, 4),
+                                       (16, 16, 4)), steps=20, warmup=5):
+    """Fastest force kernel (tile_x, tile_y, lanes) for one simulation:
+    make_stepper(tile, lanes) builds a Stepper on a fresh copy of the state.
+    Returns (kernel, table)."""
     table = []
-    for tile in tiles:
-        st = make_stepper(tile)
+    for kern in kernels:
+        st = make_stepper(kern[:2], kern[2])
         st.step(warmup)
         ms = st.timed(steps)
         st.check()
-        table.append((ms[3] / steps, tuple(tile), [m / steps for m in ms]))
+        table.append((ms[3] / steps, tuple(kern), [m / steps for m in ms]))
         del st
     best = min(table, key=lambda t: t[0])
     return best[1], table

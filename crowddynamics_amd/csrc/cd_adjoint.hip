@@ -94,46 +94,61 @@ __device__ __forceinline__ Owner<double> owner_at(const Agents<double> &A, int j
   return o;
 }
 
-// Pair adjoints of agent i with the agents of global cell c.
-__device__ __forceinline__ void cell_vjp(const Agents<double> &A, const Work &W, const AdjView &J,
-                                         int c, int a0, int a1, int i, double2 xi, double2 vi,
-                                         double ri, const Owner<double> &oi, double2 lfi,
-                                         double acc[8]) {
-  const int cnt = W.count[c];
-  if (cnt <= 0) return;
-  if (cnt <= W.K) {
-    const size_t e0 = (size_t)c * W.K;
-    const double2 *e_vel = (const double2 *)W.e_vel;
-    for (int s = 0; s < cnt; ++s) {
-      const size_t e = e0 + s;
-      const int j = W.ell_ids[e];
-      if (j == i) continue;
-      const double2 xj = W.e_pos[e], vj = e_vel[e];
-      pair_vjp(xi.x - xj.x, xi.y - xj.y, vi.x - vj.x, vi.y - vj.y, ri + W.e_r[e], oi,
-               owner_at(A, j), lfi, J.lam_f[j], acc);
+// Pair adjoints of agent i with part `part` (of m) of the candidates in its
+// stencil row through flat cell g0 (cells g0 .. g0 + 2 with counts n[]),
+// numbered cell by cell.
+__device__ __forceinline__ void row_vjp(const Agents<double> &A, const Work &W, const AdjView &J,
+                                        int g0, const int n[3], int part, int m, int a0, int a1,
+                                        int i, double2 xi, double2 vi, double ri,
+                                        const Owner<double> &oi, double2 lfi, double acc[8]) {
+  const int tot = n[0] + n[1] + n[2];
+  const int b0 = (tot * part) / m, b1 = (tot * (part + 1)) / m;
+  const double2 *e_vel = (const double2 *)W.e_vel;
+  int idx = 0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int cnt = n[k];
+    const int s0 = max(0, b0 - idx), s1 = min(cnt, b1 - idx);
+    idx += cnt;
+    if (s0 >= s1) continue;
+    const int c = g0 + k;
+    if (cnt <= W.K) {
+      const size_t e0 = (size_t)c * W.K;
+      for (int s = s0; s < s1; ++s) {
+        const size_t e = e0 + s;
+        const int j = W.ell_ids[e];
+        if (j == i) continue;
+        const double2 xj = W.e_pos[e], vj = e_vel[e];
+        pair_vjp(xi.x - xj.x, xi.y - xj.y, vi.x - vj.x, vi.y - vj.y, ri + W.e_r[e], oi,
+                 owner_at(A, j), lfi, J.lam_f[j], acc);
+      }
+    } else {  // more agents than slots (exact slow path), members in agent order
+      int r = 0;
+      for (int j = a0; j < a1 && r < s1; ++j) {
+        if (W.cell_of[j] != c) continue;
+        if (r >= s0 && j != i) {
+          const double2 xj = A.pos[j];
+          const V2<double> vj = A.vel[j];
+          pair_vjp(xi.x - xj.x, xi.y - xj.y, vi.x - vj.x, vi.y - vj.y, ri + A.radius[j], oi,
+                   owner_at(A, j), lfi, J.lam_f[j], acc);
+        }
+        ++r;
+      }
     }
-    return;
-  }
-  for (int j = a0; j < a1; ++j) {  // more agents than slots (exact slow path)
-    if (j == i || W.cell_of[j] != c) continue;
-    const double2 xj = A.pos[j];
-    const V2<double> vj = A.vel[j];
-    pair_vjp(xi.x - xj.x, xi.y - xj.y, vi.x - vj.x, vi.y - vj.y, ri + A.radius[j], oi,
-             owner_at(A, j), lfi, J.lam_f[j], acc);
   }
 }
 
-// Four lanes per agent as in the forward force kernel: lanes 0..2 take the
-// stencil rows dx = -1, 0, +1, lane 3 the agent's own (adjusting) terms and
-// the combination.
+// L lanes per agent (L = 4 m) as in the forward force kernel: lanes
+// 0 .. 3m-1 take part q / 3 of stencil row dx = q % 3 - 1, lane 3m the
+// agent's own (adjusting) terms and the combination.
 __global__ void __launch_bounds__(256) k_adjoint(Agents<double> A, Work W, Batch B, AdjView J,
-                                                 int flags) {
+                                                 int flags, int L) {
   const int2 sk = block_e(B);
   const ScnDesc d = desc_of(B, sk.x);
   const ScnState *ss = &B.sst[sk.x];
-  const int i = d.a0 + sk.y * (int)(blockDim.x >> 2) + (threadIdx.x >> 2);
-  const int q = threadIdx.x & 3;
-  const int lane0 = (threadIdx.x & 63) & ~3;
+  const int m = L >> 2;
+  const int i = d.a0 + sk.y * ((int)blockDim.x / L) + (int)threadIdx.x / L;
+  const int q = threadIdx.x & (L - 1);
   const bool have = i < d.a1;
   const bool binned = (flags & CD_USE_AGENT_AGENT) && ss->valid && !ss->oob;
   double acc[8];
@@ -141,30 +156,32 @@ __global__ void __launch_bounds__(256) k_adjoint(Agents<double> A, Work W, Batch
   for (int k = 0; k < 8; ++k) acc[k] = 0.0;
   double2 lfi = make_double2(0.0, 0.0);
   if (have) lfi = J.lam_f[i];
-  if (have && binned && q < 3) {
+  if (have && binned && q < 3 * m) {
     const double2 xi = A.pos[i];
     const V2<double> v = A.vel[i];
     const double2 vi = make_double2(v.x, v.y);
     const int c = W.cell_of[i] - d.c0;
     const int ny = ss->ny, nx = ss->nx;
     const int lx = c / ny, ly = c - lx * ny;
-    const int ox = lx + q - 1;
+    const int ox = lx + q % 3 - 1;
     if (ox >= 0 && ox < nx) {
       const Owner<double> oi = owner_at(A, i);
       const double ri = A.radius[i];
-      const int ylo = max(ly - 1, 0), yhi = min(ly + 1, ny - 1);
-      for (int oy = ylo; oy <= yhi; ++oy)
-        cell_vjp(A, W, J, d.c0 + ox * ny + oy, d.a0, d.a1, i, xi, vi, ri, oi, lfi, acc);
+      const int g0 = d.c0 + ox * ny + ly - 1;
+      const int n[3] = {ly - 1 >= 0 ? W.count[g0] : 0, W.count[g0 + 1],
+                        ly + 1 < ny ? W.count[g0 + 2] : 0};
+      row_vjp(A, W, J, g0, n, q / 3, m, d.a0, d.a1, i, xi, vi, ri, oi, lfi, acc);
     }
+  }
+  // sums over the L lanes (butterfly in a fixed order: deterministic)
+  for (int mask = 1; mask < L; mask <<= 1) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc[k] += __shfl_xor(acc[k], mask, 64);
   }
   double sum[8];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    const double a0 = __shfl(acc[k], lane0 + 0, 64), a1 = __shfl(acc[k], lane0 + 1, 64),
-                 a2 = __shfl(acc[k], lane0 + 2, 64);
-    sum[k] = (a0 + a1) + a2;
-  }
-  if (!have || q != 3) return;
+  for (int k = 0; k < 8; ++k) sum[k] = acc[k];
+  if (!have || q != 3 * m) return;
   double2 lv = J.lam_vt[i];
   double g_v0 = 0.0;
   if (flags & CD_USE_ADJUST) {
@@ -236,8 +253,75 @@ int cd_adjoint_step(const cd_agents_t *a, const cd_step_params_t *p, const doubl
     if ((rc = check_launch())) return rc;
   }
   hipLaunchKernelGGL(k_adjoint, dim3(L.blocks_e), dim3(L.threads_e), 0, s, A, W, B, J,
-                     (int)p->flags);
+                     (int)p->flags, L.lanes);
   return check_launch();
+}
+
+int cd_rollout_record(const cd_agents_t *a, const cd_step_params_t *p, const cd_workspace_t *w,
+                      int64_t steps, int64_t checkpoint, double *x_out, double *ck_pos,
+                      double *ck_vel, double *ck_fprev, cd_stream_t stream) {
+  int rc = check_step(a, p, w);
+  if (rc) return rc;
+  if (a->model != CD_CIRCULAR || a->real != CD_F64) return CD_ERR_UNSUPPORTED;
+  if (steps < 0 || checkpoint < 1 || !x_out || !ck_pos || !ck_vel || !ck_fprev)
+    return CD_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  const size_t bytes = (size_t)a->n * 16;
+  auto copy = [&](void *dst, const void *src) {
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s) == hipSuccess;
+  };
+  if (!copy(x_out, a->position)) return CD_ERR_HIP;
+  for (int64_t k = 0; k < steps; ++k) {
+    if (k % checkpoint == 0) {
+      const size_t c = (size_t)(k / checkpoint) * a->n * 2;
+      if (!copy(ck_pos + c, a->position) || !copy(ck_vel + c, a->velocity) ||
+          !copy(ck_fprev + c, a->force_prev))
+        return CD_ERR_HIP;
+    }
+    if ((rc = cd_step(a, p, w, stream))) return rc;
+    if (!copy(x_out + (size_t)(k + 1) * a->n * 2, a->position)) return CD_ERR_HIP;
+  }
+  return CD_OK;
+}
+
+int cd_adjoint_segment(const cd_agents_t *a, const cd_step_params_t *p, const cd_workspace_t *w,
+                       int64_t k0, int64_t k1, const double *ck_pos, const double *ck_vel,
+                       const double *ck_fprev, double *tape_x, double *tape_v,
+                       const double *grad_x, const cd_adjoint_t *adj, cd_stream_t stream) {
+  int rc = check_step(a, p, w);
+  if (rc) return rc;
+  if (a->model != CD_CIRCULAR || a->real != CD_F64) return CD_ERR_UNSUPPORTED;
+  if (k0 < 0 || k1 <= k0 || !ck_pos || !ck_vel || !ck_fprev || !tape_x || !tape_v || !adj)
+    return CD_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  const size_t n2 = (size_t)a->n * 2, bytes = n2 * 8;
+  auto copy = [&](void *dst, const void *src) {
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s) == hipSuccess;
+  };
+  // restore the checkpoint state (force is zero after Reset)
+  if (!copy(a->position, ck_pos) || !copy(a->velocity, ck_vel) ||
+      !copy(a->force_prev, ck_fprev))
+    return CD_ERR_HIP;
+  if ((rc = cd_reset(a, stream))) return rc;
+  if ((rc = cd_step_prepare(a, p, w, (uint64_t)k0, stream))) return rc;
+  // the states k0 .. k1 - 1 of the segment
+  if (!copy(tape_x, a->position) || !copy(tape_v, a->velocity)) return CD_ERR_HIP;
+  for (int64_t k = k0; k < k1 - 1; ++k) {
+    if ((rc = cd_step(a, p, w, stream))) return rc;
+    if (!copy(tape_x + (size_t)(k + 1 - k0) * n2, a->position) ||
+        !copy(tape_v + (size_t)(k + 1 - k0) * n2, a->velocity))
+      return CD_ERR_HIP;
+  }
+  // reverse sweep over the segment
+  cd_agents_t view = *a;
+  for (int64_t k = k1 - 1; k >= k0; --k) {
+    view.position = tape_x + (size_t)(k - k0) * n2;
+    view.velocity = tape_v + (size_t)(k - k0) * n2;
+    if ((rc = cd_adjoint_step(&view, p, grad_x ? grad_x + (size_t)k * n2 : nullptr, adj, w,
+                              stream)))
+      return rc;
+  }
+  return CD_OK;
 }
 
 }  // extern "C"

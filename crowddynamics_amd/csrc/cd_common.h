@@ -74,7 +74,7 @@ constexpr int kBS = 256;              // threads per block of per-agent kernels
 constexpr int kMaxPartsPerScn = 128;  // blocks of the partial-producing kernels per scenario
 constexpr int kSingleParts = 512;     // the same for a single (unbatched) scenario
 constexpr int kAgentsPerForceBlock = 64;  // default agents per force block (256 threads)
-constexpr int kMinForceAgents = 16;
+constexpr int kMinForceAgents = 4;  // agents per force block: 64 threads / 16 lanes
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 16;
 constexpr int kScanTile = kScanThreads * kScanItems;
@@ -87,6 +87,8 @@ inline int64_t max_blocks_f(int64_t n, int64_t S) {
   return std::max<int64_t>(kSingleParts, n / kBS + S + 1);
 }
 inline int64_t max_blocks_e(int64_t n, int64_t S) { return n / kMinForceAgents + S + 1; }
+// lanes per agent of the agent-parallel force kernel (4, 8 or 16; 0: 4)
+inline int agent_lanes(const cd_workspace_t *w) { return w->agent_lanes > 0 ? (int)w->agent_lanes : 4; }
 
 struct Layout {
   int64_t state, scn_desc, scn_state, map_f, map_e, map_t, partials, tiles;
@@ -615,6 +617,8 @@ bool ws_sizes_ok(const cd_workspace_t *w) {
          (w->force_threads == 0 || w->force_threads == 64 || w->force_threads == 128 ||
           w->force_threads == 256) &&
          w->tile_x >= 0 && w->tile_x <= 16 && w->tile_y >= 0 && w->tile_y <= 16 &&
+         (w->agent_lanes == 0 || w->agent_lanes == 4 || w->agent_lanes == 8 ||
+          w->agent_lanes == 16) &&
          (w->model == CD_CIRCULAR || w->model == CD_THREE_CIRCLE);
 }
 
@@ -643,13 +647,14 @@ inline void mark(hipEvent_t *ev, int k, hipStream_t s) {
 
 // Grid sizes of one step: per-scenario kernels (f) and force kernel (e).
 struct Launch {
-  int blocks_f, blocks_e, threads_e, blocks_t;
+  int blocks_f, blocks_e, threads_e, blocks_t, lanes;
 };
 
 Launch single_launch(const cd_workspace_t *w) {
   Launch L;
   L.blocks_f = single_parts(w->n);
-  L.blocks_e = std::max(1, grid_for(w->n, force_threads(w) / 4));
+  L.lanes = agent_lanes(w);
+  L.blocks_e = std::max(1, grid_for(w->n, force_threads(w) / L.lanes));
   L.threads_e = force_threads(w);
   L.blocks_t = tile_blocks(w->cell_capacity, w);
   return L;

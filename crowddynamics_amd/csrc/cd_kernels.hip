@@ -333,11 +333,12 @@ __global__ void __launch_bounds__(256) k_reset(Agents<T> A, int model) {
 }
 
 // Wall contacts of agent i over segments w_begin, w_begin + w_stride, ...
-// (core/interactions.py:107-141, 169-186).
-template <typename T>
-__device__ __forceinline__ void walls_agent(const Agents<T> &A, int model, int i, double2 p,
-                                            V2<T> v, const double4 *seg, int W, T &fx, T &fy,
-                                            double &tq, int w_begin = 0, int w_stride = 1) {
+// (core/interactions.py:107-141, 169-186): add(force x, force y, torque)
+// for each contact.
+template <typename T, class Add>
+__device__ __forceinline__ void walls_each(const Agents<T> &A, int model, int i, double2 p,
+                                           V2<T> v, const double4 *seg, int W, Add &&add,
+                                           int w_begin = 0, int w_stride = 1) {
   const double r = A.radius[i];
   double2 ls = p, rs = p;
   double rt = 0.0, rsr = 0.0;
@@ -382,11 +383,21 @@ __device__ __forceinline__ void walls_agent(const Agents<T> &A, int model, int i
       T cx, cy;
       const T tnx = (T)nx, tny = (T)ny;
       force_contact((T)h, tnx, tny, v.x, v.y, tny, -tnx, mu, kappa, damping, cx, cy);
-      fx += cx;
-      fy += cy;
-      if (model) tq += rmx * (double)cy - rmy * (double)cx;
+      add(cx, cy, model ? rmx * (double)cy - rmy * (double)cx : 0.0);
     }
   }
+}
+
+// The same, summed in segment order into fx, fy, tq.
+template <typename T>
+__device__ __forceinline__ void walls_agent(const Agents<T> &A, int model, int i, double2 p,
+                                            V2<T> v, const double4 *seg, int W, T &fx, T &fy,
+                                            double &tq) {
+  walls_each(A, model, i, p, v, seg, W, [&](T cx, T cy, double m) {
+    fx += cx;
+    fy += cy;
+    if (model) tq += m;
+  });
 }
 
 constexpr int kSegTile = 512;
@@ -632,39 +643,39 @@ __device__ __forceinline__ void own_terms(const Agents<T> &A, const FusedArgs &f
   }
 }
 
-// Neighbour sources of one stencil row: the LDS elements of a staged
-// region ...
+// Neighbour sources.  each(row, part, m, self, fn) visits part `part` (of
+// m equal parts) of the candidates in stencil row dx = row - 1: the agents
+// of the cells (lx + dx, ly - 1 .. ly + 1), as one sequence.  Parts may
+// visit candidates in any order (the pair sums are exact).  From the LDS
+// elements of a staged region (the row's three cells are consecutive region
+// cells, so their elements are one contiguous range) ...
 template <typename T, int MODEL>
 struct LdsRow {
   const TileLds<T, MODEL> &S;
   int RH, kx, ky;  // region column length and the agent's region cell
   template <class F>
-  __device__ __forceinline__ void each(int q, int self, F &&fn) const {
-    const int cx = kx + q - 1;
-#pragma unroll
-    for (int dy = -1; dy <= 1; ++dy) {
-      const int kk = cx * RH + (ky + dy);
-      const int e1 = S.off[kk + 1];
-      for (int e2 = S.off[kk]; e2 < e1; ++e2) {
-        const int j = S.id[e2];
-        if (j == self) continue;
-        if constexpr (MODEL == 0)
-          fn(j, S.pos[e2], S.vel[e2], S.r[e2]);
-        else
-          fn(j, tc_lds<T, MODEL>(S, e2));
-      }
+  __device__ __forceinline__ void each(int row, int part, int m, int self, F &&fn) const {
+    const int kk = (kx + row - 1) * RH + (ky - 1);
+    const int e0 = S.off[kk], tot = S.off[kk + 3] - e0;
+    const int b = e0 + (tot * part) / m, e = e0 + (tot * (part + 1)) / m;
+    for (int e2 = b; e2 < e; ++e2) {
+      const int j = S.id[e2];
+      if (j == self) continue;
+      if constexpr (MODEL == 0)
+        fn(j, S.pos[e2], S.vel[e2], S.r[e2]);
+      else
+        fn(j, tc_lds<T, MODEL>(S, e2));
     }
   }
 };
 
-// ... or the grid cells in memory: slot lists, or a scan of cell_of for a
-// cell holding more agents than slots (always, when `scan`).
+// ... or from the grid cells in memory: slot lists, or a scan of cell_of
+// for a cell holding more agents than slots.
 template <typename T, int MODEL>
 struct MemRow {
   const Agents<T> &A;
   const Work &W;
   int c0, nx, ny, lx, ly, a0, a1;
-  bool scan;
   template <class F>
   __device__ __forceinline__ void visit(int self, int j, F &fn) const {
     if (j == self) return;
@@ -673,81 +684,149 @@ struct MemRow {
     else
       fn(j, tc_agent(A, j));
   }
+  // The row's cells have consecutive flat indices.  Their slot lists are
+  // walked as one sequence in batches of four candidates whose id and data
+  // loads are issued together, so a lane waits for memory once per batch
+  // rather than once per candidate.
   template <class F>
-  __device__ __forceinline__ void each(int q, int self, F &&fn) const {
-    const int ox = lx + q - 1;
+  __device__ __forceinline__ void each(int row, int part, int m, int self, F &&fn) const {
+    const int ox = lx + row - 1;
     if (ox < 0 || ox >= nx) return;
-    for (int dy = -1; dy <= 1; ++dy) {
-      const int oy = ly + dy;
-      if (oy < 0 || oy >= ny) continue;
-      const int gc = c0 + ox * ny + oy;
-      const int cnt = W.count[gc];
-      if (!scan && cnt <= W.K) {
+    const int g0 = c0 + ox * ny + ly - 1;  // flat index of cell dy = -1
+    const int n0 = (ly - 1 >= 0) ? W.count[g0] : 0;
+    const int n1 = W.count[g0 + 1];
+    const int n2 = (ly + 1 < ny) ? W.count[g0 + 2] : 0;
+    const int p1 = n0, p2 = n0 + n1, tot = p2 + n2;
+    const int b0 = (tot * part) / m, b1 = (tot * (part + 1)) / m;
+    if (n0 <= W.K && n1 <= W.K && n2 <= W.K) {
+      for (int b = b0; b < b1; b += 4) {
+        int j[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int idx = b + u;
+          const int k = (idx >= p1 ? 1 : 0) + (idx >= p2 ? 1 : 0);
+          const int slot = idx - (k == 0 ? 0 : (k == 1 ? p1 : p2));
+          j[u] = idx < b1 ? W.ell_ids[(size_t)(g0 + k) * W.K + slot] : -1;
+        }
+        if constexpr (MODEL == 0) {
+          double2 xj[4];
+          V2<T> vj[4];
+          double rj[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int jj = j[u] < 0 ? self : j[u];
+            xj[u] = A.pos[jj];
+            vj[u] = A.vel[jj];
+            rj[u] = A.radius[jj];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (j[u] >= 0 && j[u] != self) fn(j[u], xj[u], vj[u], rj[u]);
+        } else {
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (j[u] >= 0) visit(self, j[u], fn);
+        }
+      }
+      return;
+    }
+    // a cell holding more agents than slots: members found by scanning
+    // cell_of, the row's candidates numbered cell by cell
+    const int nn[3] = {n0, n1, n2};
+    int idx = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int cnt = nn[k];
+      const int s0 = max(0, b0 - idx), s1 = min(cnt, b1 - idx);
+      idx += cnt;
+      if (s0 >= s1) continue;
+      const int gc = g0 + k;
+      if (cnt <= W.K) {
         const int *ids = W.ell_ids + (size_t)gc * W.K;
-        for (int s = 0; s < cnt; ++s) visit(self, ids[s], fn);
-      } else if (cnt > 0) {
-        for (int j = a0; j < a1; ++j)
-          if (W.cell_of[j] == gc) visit(self, j, fn);
+        for (int s = s0; s < s1; ++s) visit(self, ids[s], fn);
+      } else {
+        int r = 0;
+        for (int jx = a0; jx < a1 && r < s1; ++jx) {
+          if (W.cell_of[jx] != gc) continue;
+          if (r >= s0) visit(self, jx, fn);
+          ++r;
+        }
       }
     }
   }
 };
 
 // Stages the scenario's first kWallTile wall segments in LDS (before the
-// per-agent work, so the loads overlap it).  Returns the count staged.
-__device__ __forceinline__ int stage_walls(const FusedArgs &fa, const ScnDesc &d, double4 *sseg,
-                                           bool fused) {
-  if (!fused || !(fa.flags & CD_USE_OBSTACLE)) return 0;
+// per-agent work, so the loads overlap it).
+__device__ __forceinline__ void stage_walls(const FusedArgs &fa, const ScnDesc &d, double4 *sseg,
+                                            bool fused) {
+  if (!fused || !(fa.flags & CD_USE_OBSTACLE)) return;
   const int cnt = min(kWallTile, d.w1 - d.w0);
   for (int k = threadIdx.x; k < cnt; k += blockDim.x) sseg[k] = fa.segs[d.w0 + k];
-  return cnt;
 }
 
-// All force terms of agent i (four lanes; q = lane in the group of four).
-// Every lane of the block calls this (have = false for idle groups), so the
-// barriers of the wall staging and the shuffles are uniform.  The first
-// kWallTile segments are already in sseg (stage_walls + a barrier).
+__device__ __forceinline__ Fix128 shfl_xor(const Fix128 &v, int mask) {
+  Fix128 t;
+  t.hi = __shfl_xor(v.hi, mask, 64);
+  t.lo = __shfl_xor(v.lo, mask, 64);
+  t.bad = __shfl_xor((int)v.bad, mask, 64) != 0;
+  return t;
+}
+
+// All force terms of agent i, served by L lanes (L = 4 m, m = 1, 2 or 4;
+// q = lane in the group): lanes 0 .. 3m-1 take part q / 3 of stencil row
+// q % 3 of the pairs, lane 3m the agent's own terms (fluctuation, navigation,
+// orientation, adjusting), and all L lanes share the walls.  Pair and wall
+// terms are summed exactly, so results do not depend on L.  Every lane of
+// the block calls this (have = false for idle groups), so the wall staging
+// barriers and the shuffles are uniform; the first kWallTile segments are
+// already in sseg.
 template <typename T, int MODEL, bool FUSED, class Rows>
 __device__ __forceinline__ void agent_forces(const Agents<T> &A, const FusedArgs &fa,
                                              const ScnDesc &d, uint64_t step, double4 *sseg,
-                                             bool have, int i, double2 pos, V2<T> vel,
+                                             int L, bool have, int i, double2 pos, V2<T> vel,
                                              const TCView &me, const Rows &rows) {
-  const int q = threadIdx.x & 3;
-  const int lane0 = (threadIdx.x & 63) & ~3;
+  const int q = threadIdx.x & (L - 1);
+  const int m = L >> 2;
   V2<T> f;
   f.x = T(0);
   f.y = T(0);
   double tq = 0.0;
-  if (have && q == 3) {
+  if (have && q == 3 * m) {
     f = A.force[i];
     if (MODEL) tq = A.torque[i];
     if (FUSED) own_terms<T, MODEL>(A, fa, d.seed, step, i, i - d.a0, pos, vel, f, tq);
   }
-  PairSum ps;
-  if (have && q < 3) {
+  PairSum ps;  // pairs
+  if (have && q < 3 * m) {
     const Owner<T> o = owner_of(A, i);
     const double ri = A.radius[i];
     if constexpr (MODEL == 0) {
-      rows.each(q, i, [&](int, double2 xj, V2<T> vj, double rj) {
+      rows.each(q % 3, q / 3, m, i, [&](int, double2 xj, V2<T> vj, double rj) {
         pair_add_c<T>(pos, vel, ri, o, xj, vj, rj, ps);
       });
     } else {
-      rows.each(q, i, [&](int j, const TCView &other) {
+      rows.each(q % 3, q / 3, m, i, [&](int j, const TCView &other) {
         if constexpr (sizeof(T) == 8) pair_add_tc(i, j, me, other, o, ps);
       });
     }
   }
-  T wx = T(0), wy = T(0);
-  double wtq = 0.0;
-  if (FUSED && (fa.flags & CD_USE_OBSTACLE)) {
+  PairSum ws;  // walls
+  const bool walls = FUSED && (fa.flags & CD_USE_OBSTACLE);
+  if (walls) {
+    auto add = [&](T cx, T cy, double mm) {
+      ws.x.add((double)cx);
+      ws.y.add((double)cy);
+      if (MODEL) ws.m.add(mm);
+    };
     const int nw = d.w1 - d.w0;
-    if (have) walls_agent(A, MODEL, i, pos, vel, sseg, min(kWallTile, nw), wx, wy, wtq, q, 4);
+    if (have) walls_each(A, MODEL, i, pos, vel, sseg, min(kWallTile, nw), add, q, L);
     for (int w0 = kWallTile; w0 < nw; w0 += kWallTile) {  // more segments than staged
       const int cnt = min(kWallTile, nw - w0);
       __syncthreads();
       for (int k = threadIdx.x; k < cnt; k += blockDim.x) sseg[k] = fa.segs[d.w0 + w0 + k];
       __syncthreads();
-      if (have) walls_agent(A, MODEL, i, pos, vel, sseg, cnt, wx, wy, wtq, q, 4);
+      if (have) walls_each(A, MODEL, i, pos, vel, sseg, cnt, add, q, L);
     }
     if (nw > kWallTile) {  // restore the first tile for the next agents
       __syncthreads();
@@ -755,62 +834,40 @@ __device__ __forceinline__ void agent_forces(const Agents<T> &A, const FusedArgs
       __syncthreads();
     }
   }
-  // exact sums of lanes 0..2 on lane 3 (every lane takes part in the shuffles)
-  Fix128 sx, sy, sm;
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    Fix128 t;
-    t.hi = __shfl(ps.x.hi, lane0 + k, 64);
-    t.lo = __shfl(ps.x.lo, lane0 + k, 64);
-    t.bad = __shfl((int)ps.x.bad, lane0 + k, 64) != 0;
-    sx.add(t);
-    t.hi = __shfl(ps.y.hi, lane0 + k, 64);
-    t.lo = __shfl(ps.y.lo, lane0 + k, 64);
-    t.bad = __shfl((int)ps.y.bad, lane0 + k, 64) != 0;
-    sy.add(t);
-    if (MODEL) {
-      t.hi = __shfl(ps.m.hi, lane0 + k, 64);
-      t.lo = __shfl(ps.m.lo, lane0 + k, 64);
-      t.bad = __shfl((int)ps.m.bad, lane0 + k, 64) != 0;
-      sm.add(t);
+  // exact sums over the L lanes (butterfly: every lane ends with the totals)
+  for (int mask = 1; mask < L; mask <<= 1) {
+    ps.x.add(shfl_xor(ps.x, mask));
+    ps.y.add(shfl_xor(ps.y, mask));
+    if (MODEL) ps.m.add(shfl_xor(ps.m, mask));
+    if (walls) {
+      ws.x.add(shfl_xor(ws.x, mask));
+      ws.y.add(shfl_xor(ws.y, mask));
+      if (MODEL) ws.m.add(shfl_xor(ws.m, mask));
     }
   }
-  T w0x = T(0), w0y = T(0), w1x = T(0), w1y = T(0), w2x = T(0), w2y = T(0);
-  double u0 = 0.0, u1 = 0.0, u2 = 0.0;
-  if (FUSED) {
-    w0x = __shfl(wx, lane0 + 0, 64);
-    w0y = __shfl(wy, lane0 + 0, 64);
-    w1x = __shfl(wx, lane0 + 1, 64);
-    w1y = __shfl(wy, lane0 + 1, 64);
-    w2x = __shfl(wx, lane0 + 2, 64);
-    w2y = __shfl(wy, lane0 + 2, 64);
-    u0 = __shfl(wtq, lane0 + 0, 64);
-    u1 = __shfl(wtq, lane0 + 1, 64);
-    u2 = __shfl(wtq, lane0 + 2, 64);
-  }
-  if (!have || q != 3) return;
-  f.x += (T)sx.value();
-  f.y += (T)sy.value();
-  if (MODEL) tq += sm.value();
-  if (FUSED) {
-    f.x += (w0x + w1x) + (w2x + wx);
-    f.y += (w0y + w1y) + (w2y + wy);
-    if (MODEL) tq += (u0 + u1) + (u2 + wtq);
+  if (!have || q != 3 * m) return;
+  f.x += (T)ps.x.value();
+  f.y += (T)ps.y.value();
+  if (MODEL) tq += ps.m.value();
+  if (walls) {
+    f.x += (T)ws.x.value();
+    f.y += (T)ws.y.value();
+    if (MODEL) tq += ws.m.value();
   }
   A.force[i] = f;
   if (MODEL) A.torque[i] = tq;
 }
 
-// Direct kernel: blocks of blockDim / 4 agents of one scenario, in agent
-// index order.  FUSED = false: the AgentAgentInteractions node, force
-// (torque) += pairs.  FUSED = true: all per-agent forces of one step in the
-// post-order of the standard logic tree (fluctuation, navigation,
-// orientation, adjusting, pairs, walls), added to force (torque).  Nothing
-// runs when the scenario's binning is invalid (the integrator then adds the
-// per-agent terms itself).
+// Direct kernel: blocks of blockDim / L agents of one scenario, in agent
+// index order, L lanes each.  FUSED = false: the AgentAgentInteractions
+// node, force (torque) += pairs.  FUSED = true: all per-agent forces of one
+// step in the post-order of the standard logic tree (fluctuation,
+// navigation, orientation, adjusting, pairs, walls), added to force
+// (torque).  Nothing runs when the scenario's binning is invalid (the
+// integrator then adds the per-agent terms itself).
 template <typename T, int MODEL, bool FUSED>
 __global__ void __launch_bounds__(256) k_forces_direct(Agents<T> A, Work W, Batch B,
-                                                       FusedArgs fa) {
+                                                       FusedArgs fa, int L) {
   __shared__ double4 sseg[kWallTile];
   const int2 sk = block_e(B);
   const ScnDesc d = desc_of(B, sk.x);
@@ -820,7 +877,8 @@ __global__ void __launch_bounds__(256) k_forces_direct(Agents<T> A, Work W, Batc
   if (sk.y == 0 && threadIdx.x == 0) B.sst[sk.x].pairs_done = ok ? 1 : 0;
   if (!ok) return;
   stage_walls(fa, d, sseg, FUSED);
-  const int i = d.a0 + sk.y * (int)(blockDim.x >> 2) + (threadIdx.x >> 2);
+  const int per = (int)blockDim.x / L;
+  const int i = d.a0 + sk.y * per + (int)threadIdx.x / L;
   const bool have = i < d.a1;
   double2 pos = make_double2(0.0, 0.0);
   V2<T> vel;
@@ -836,9 +894,9 @@ __global__ void __launch_bounds__(256) k_forces_direct(Agents<T> A, Work W, Batc
   }
   const int nx = ss->nx, ny = ss->ny;
   const int lc = c - d.c0, lx = lc / ny, ly = lc - lx * ny;
-  const MemRow<T, MODEL> rows{A, W, d.c0, nx, ny, lx, ly, d.a0, d.a1, false};
+  const MemRow<T, MODEL> rows{A, W, d.c0, nx, ny, lx, ly, d.a0, d.a1};
   __syncthreads();  // walls staged
-  agent_forces<T, MODEL, FUSED>(A, fa, d, W.st->step - 1, sseg, have, i, pos, vel, me, rows);
+  agent_forces<T, MODEL, FUSED>(A, fa, d, W.st->step - 1, sseg, L, have, i, pos, vel, me, rows);
 }
 
 struct TileArgs {
@@ -949,8 +1007,8 @@ __device__ bool tile_region(const Agents<T> &A, const Work &W, const FusedArgs &
     const LdsRow<T, MODEL> rows{S, RH, kx, k - kx * RH};
     TCView me;
     if constexpr (MODEL == 1) me = tc_lds<T, MODEL>(S, e);
-    agent_forces<T, MODEL, FUSED>(A, fa, d, step, sseg, have, S.id[e], S.pos[e], S.vel[e], me,
-                                  rows);
+    agent_forces<T, MODEL, FUSED>(A, fa, d, step, sseg, 4, have, S.id[e], S.pos[e], S.vel[e],
+                                  me, rows);
   }
   __syncthreads();
   return true;
@@ -962,7 +1020,7 @@ __device__ void cell_from_memory(const Agents<T> &A, const Work &W, const FusedA
                                  const ScnDesc &d, int nx, int ny, uint64_t step, double4 *sseg,
                                  int lx, int ly) {
   const int gc = d.c0 + lx * ny + ly;
-  const MemRow<T, MODEL> rows{A, W, d.c0, nx, ny, lx, ly, d.a0, d.a1, false};
+  const MemRow<T, MODEL> rows{A, W, d.c0, nx, ny, lx, ly, d.a0, d.a1};
   for (int j0 = d.a0; j0 < d.a1; j0 += kTileGroups) {
     const int i = j0 + (threadIdx.x >> 2);
     const bool have = i < d.a1 && W.cell_of[i] == gc;
@@ -976,7 +1034,7 @@ __device__ void cell_from_memory(const Agents<T> &A, const Work &W, const FusedA
       vel = A.vel[i];
       if constexpr (MODEL == 1) me = tc_agent(A, i);
     }
-    agent_forces<T, MODEL, FUSED>(A, fa, d, step, sseg, have, i, pos, vel, me, rows);
+    agent_forces<T, MODEL, FUSED>(A, fa, d, step, sseg, 4, have, i, pos, vel, me, rows);
   }
 }
 
@@ -1174,7 +1232,7 @@ void launch_forces_m(const Agents<T> &A, const Work &W, const Batch &B, const Fu
                        s, A, W, B, fa, ta);
   } else {
     hipLaunchKernelGGL((k_forces_direct<T, MODEL, FUSED>), dim3(L.blocks_e), dim3(L.threads_e), 0,
-                       s, A, W, B, fa);
+                       s, A, W, B, fa, L.lanes);
   }
 }
 
@@ -1321,7 +1379,7 @@ int build_batch(const cd_scenario_t *sc, int S, const cd_workspace_t *w, int64_t
     d.f0 = f;
     for (int k = 0; k < d.nf; ++k) H.map_f.push_back(make_int2(s, k));
     f += d.nf;
-    const int ne = grid_for(ns, force_threads(w) / 4);
+    const int ne = grid_for(ns, force_threads(w) / agent_lanes(w));
     for (int k = 0; k < ne; ++k) H.map_e.push_back(make_int2(s, k));
     d.nt = tile_blocks(c.cell_capacity, w);
     for (int k = 0; k < d.nt; ++k) H.map_t.push_back(make_int2(s, k));
@@ -1671,6 +1729,64 @@ int cd_step_timed(const cd_agents_t *a, const cd_step_params_t *p, const cd_work
     }
   }
   for (int k = 0; k < 4; ++k) (void)hipEventDestroy(ev[k]);
+  return rc;
+}
+
+int cd_kernel_timed(const cd_agents_t *a, const cd_step_params_t *p, const cd_workspace_t *w,
+                    int32_t kernel, int64_t reps, double *avg_ms, cd_stream_t stream) {
+  int rc = check_step(a, p, w);
+  if (rc) return rc;
+  if (!avg_ms || reps < 1 || kernel < 0 || kernel > 2) return CD_ERR_ARG;
+  *avg_ms = 0.0;
+  if (a->n == 0) return CD_OK;
+  hipStream_t s = (hipStream_t)stream;
+  const Work W = carve(w);
+  const bool batched = w->n_scenarios >= 1 && w->blocks_f >= 1;
+  const Batch B = batched ? batch_of(w)
+                          : single_batch(w, (p->flags & CD_USE_OBSTACLE) ? p->n_segments : 0,
+                                         p->seed, p->dt_min, p->dt_max);
+  const Launch L = batched ? batch_launch(w) : single_launch(w);
+  const FusedArgs fa = fused_args(p);
+  const bool pairs = (p->flags & CD_USE_AGENT_AGENT) != 0;
+  const int reset = (p->flags & CD_USE_RESET) ? 1 : 0;
+  auto bin = [&]() {
+    hipLaunchKernelGGL(k_bin_insert, dim3(L.blocks_f), dim3(kBS), 0, s, (const double2 *)a->position,
+                       p->cell_size, W, B, pairs ? 1 : 0, 1);
+  };
+  auto integrate = [&]() {
+    if (a->model == CD_THREE_CIRCLE)
+      hipLaunchKernelGGL((k_integrate<double, 1>), dim3(L.blocks_f), dim3(kBS), 0, s,
+                         typed<double>(a), W, B, fa, reset);
+    else if (a->real == CD_F64)
+      hipLaunchKernelGGL((k_integrate<double, 0>), dim3(L.blocks_f), dim3(kBS), 0, s,
+                         typed<double>(a), W, B, fa, reset);
+    else
+      hipLaunchKernelGGL((k_integrate<float, 0>), dim3(L.blocks_f), dim3(kBS), 0, s,
+                         typed<float>(a), W, B, fa, reset);
+  };
+  bin();
+  if ((rc = check_launch())) return rc;
+  hipEvent_t ev[2];
+  for (int k = 0; k < 2; ++k)
+    if (hipEventCreate(&ev[k]) != hipSuccess) return CD_ERR_HIP;
+  (void)hipEventRecord(ev[0], s);
+  for (int64_t r = 0; r < reps && rc == CD_OK; ++r) {
+    if (kernel == 0)
+      bin();
+    else if (kernel == 1)
+      rc = a->real == CD_F64 ? launch_forces<double>(a, W, B, fa, true, L, w, s)
+                             : launch_forces<float>(a, W, B, fa, true, L, w, s);
+    else
+      integrate();
+    if (rc == CD_OK) rc = check_launch();
+  }
+  (void)hipEventRecord(ev[1], s);
+  float ms = 0.f;
+  if (rc == CD_OK && (hipEventSynchronize(ev[1]) != hipSuccess ||
+                      hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess))
+    rc = CD_ERR_HIP;
+  for (int k = 0; k < 2; ++k) (void)hipEventDestroy(ev[k]);
+  *avg_ms = (double)ms / (double)reps;
   return rc;
 }
 

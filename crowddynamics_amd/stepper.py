@@ -56,7 +56,7 @@ class Stepper:
     def __init__(self, array, segments=None, nav=None, cell_size=3.6, dt_min=0.01,
                  dt_max=0.01, seed=1, flags=ALL_NODES, precision='f64', cell_capacity=None,
                  bounds=None, first_step=0, device=None, device_agents=None,
-                 cell_slots=None, force_threads=0, tile=None):
+                 cell_slots=None, force_threads=0, tile=None, lanes=0):
         if device_agents is not None:
             self.agents = device_agents
         else:
@@ -86,7 +86,7 @@ class Stepper:
         if cell_slots is None:
             cell_slots = cell_slots_for(cell_size, min_body_radius(array))
         self.ws = Workspace(self.n, cell_capacity, self.agents.model, cell_slots=cell_slots,
-                            device=dev, force_threads=force_threads, tile=tile)
+                            device=dev, force_threads=force_threads, tile=tile, lanes=lanes)
         self.params = make_params(cell_size, dt_min, dt_max, flags, seed, self.segments,
                                   self.maps, nav)
         self.flags = flags
@@ -138,6 +138,17 @@ class Stepper:
                                             self.ws.ref(), int(steps), ms, stream_handle()),
                    'step_timed')
         return list(ms)
+
+    def kernel_timed(self, kernel, reps=20):
+        """Average GPU milliseconds of `reps` back-to-back launches of one
+        step kernel ('bin', 'forces' or 'integrate').  Leaves the state
+        meaningless: time a copy, or prepare() again afterwards."""
+        k = {'bin': 0, 'forces': 1, 'integrate': 2}[kernel]
+        ms = C.c_double()
+        _lib.check(_lib.lib().cd_kernel_timed(self.agents.ref(), C.byref(self.params),
+                                              self.ws.ref(), k, int(reps), C.byref(ms),
+                                              stream_handle()), 'kernel_timed')
+        return ms.value
 
     def check(self):
         """Raise if the device recorded a grid overflow or non-finite positions."""

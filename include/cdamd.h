@@ -80,12 +80,13 @@ typedef struct cd_workspace {
   int32_t max_scenarios; /* scenario table size (0 or 1: single scenario) */
   /* set by cd_batch_prepare: */
   int32_t n_scenarios, blocks_f, blocks_e;
-  int32_t force_threads; /* threads per block of the adjoint's pair kernel, 4 per agent:
-                            64, 128 or 256 (0: 256) */
+  int32_t force_threads; /* threads per block of the agent-parallel force kernel and the
+                            adjoint: 64, 128 or 256 (0: 256) */
   int32_t tile_x, tile_y; /* force kernel: 0 = agent-parallel; else grid cells per
                              LDS tile, each 1..16 (same results either way) */
   int32_t blocks_t;       /* set by cd_batch_prepare */
-  int32_t _pad;
+  int32_t agent_lanes;    /* lanes per agent of the agent-parallel force kernel and the
+                             adjoint: 4, 8 or 16 (0: 4); same results either way */
 } cd_workspace_t;
 
 /* One scenario of a batch: its agents are [agent_begin, agent_end) of the
@@ -267,6 +268,27 @@ int cd_adjoint_step(const cd_agents_t *agents, const cd_step_params_t *params,
                     const double *grad_x, const cd_adjoint_t *adj, const cd_workspace_t *ws,
                     cd_stream_t stream);
 
+/* Differentiable rollout drivers (circular, fp64, the cd_adjoint_step scope).
+ * cd_rollout_record: from the agents' current state (after cd_step_prepare),
+ * `steps` fused steps; positions to x_out [steps + 1][n][2] (x_out[0] the
+ * start) and, before steps k = 0, checkpoint, 2 * checkpoint, ..., the
+ * state (position, velocity, force_prev) to ck_* [k / checkpoint][n][2].
+ * cd_adjoint_segment: restores checkpoint state ck_*[k0] (the caller passes
+ * the pointers of that checkpoint), re-runs steps k0 .. k1 - 2 into
+ * tape_x / tape_v [k1 - k0][n][2] (bit-identical to the recorded run), then
+ * applies cd_adjoint_step for k = k1 - 1 down to k0 with grad_x + k * n * 2
+ * (dL/dx_k of the whole rollout, may be null).  Stream-ordered; no reference
+ * counterpart (reverse-mode differentiation of the update). */
+int cd_rollout_record(const cd_agents_t *agents, const cd_step_params_t *params,
+                      const cd_workspace_t *ws, int64_t steps, int64_t checkpoint,
+                      double *x_out, double *ck_pos, double *ck_vel, double *ck_fprev,
+                      cd_stream_t stream);
+int cd_adjoint_segment(const cd_agents_t *agents, const cd_step_params_t *params,
+                       const cd_workspace_t *ws, int64_t k0, int64_t k1, const double *ck_pos,
+                       const double *ck_vel, const double *ck_fprev, double *tape_x,
+                       double *tape_v, const double *grad_x, const cd_adjoint_t *adj,
+                       cd_stream_t stream);
+
 /* Synchronous: runs `steps` steps (cd_step, or cd_batch_step on a workspace
  * prepared by cd_batch_prepare) and returns in kernel_ms[4] the summed GPU
  * time of the binning kernel, the force kernel, the integrate kernel and the
@@ -277,6 +299,16 @@ int cd_step_timed(const cd_agents_t *agents, const cd_step_params_t *params,
                   cd_stream_t stream);
 
 /* ---- HIP graphs (synchronous helpers) ------------------------------------- */
+
+/* Synchronous measurement of one step kernel (0: binning, 1: forces,
+ * 2: integrate): after one binning pass, `reps` back-to-back launches of it
+ * between two HIP events on `stream`; *avg_ms = elapsed / reps.  Repeating
+ * a kernel leaves the agent state and workspace meaningless (forces
+ * accumulate, agents move, cell counts grow): use a copy, then discard it
+ * or prepare again.  No reference counterpart (measurement). */
+int cd_kernel_timed(const cd_agents_t *agents, const cd_step_params_t *params,
+                    const cd_workspace_t *ws, int32_t kernel, int64_t reps, double *avg_ms,
+                    cd_stream_t stream);
 
 int cd_graph_begin(cd_stream_t stream);
 int cd_graph_end(cd_stream_t stream, void **graph_exec);

@@ -197,6 +197,20 @@ def test_adjoint_central_fd_persistent_contacts():
         _fd_checked(R, arr, Xhat, k, h, L, float(g[k]), rtol=1e-5, rtol_rich=1e-5)
 
 
+def test_adjoint_lanes_per_agent():
+    """16 lanes per agent (rows split in four parts) give the gradients of
+    4 lanes up to summation order."""
+    arr = AC.packed(40, seed=8)
+    out = []
+    for lanes in (4, 16):
+        R = Rollout(arr, steps=20, dt=DT, cell_size=CELL, checkpoint=6, lanes=lanes)
+        Xhat = _targets(R, arr, seed=4)
+        out.append(_adjoint_grads(R, arr, Xhat)[1])
+    for a, b in zip(*out):
+        scale = float(torch.max(torch.abs(a))) + 1e-300
+        assert float(torch.max(torch.abs(a - b))) <= 1e-12 * scale
+
+
 def test_adjoint_checkpoint_interval_is_exact():
     arr = AC.packed(40, seed=6)
     out = []
