@@ -61,16 +61,43 @@ def pair_tally(arr, cell, segments):
                 wall_pairs=w_all, wall_contacts=w_con, flops=flops)
 
 
-def binning_bytes(n, cells):
-    """24 B per agent (read position, write its id into its cell's list and
-    its cell index) + 16 B per cell (count written and cleared)."""
-    return 24 * n + 16 * cells
+def binning_bytes(n):
+    """Binning kernel: per agent read the position (16 B), write its cell
+    index and its id into the cell's slot list (4 + 4 B), and read-modify-
+    write its cell's count (8 B): 32 B/agent."""
+    return 32 * n
 
 
-def integrate_bytes(n, real_bytes=8):
-    """Circular adjust+integrate traffic: reads x, v, f, f_prev, e0 and m,
-    tau_adj, v0; writes x, v, f_prev (152 B/agent in fp64)."""
-    return n * (16 + 4 * 2 * real_bytes + 3 * 8 + 16 + 2 * 2 * real_bytes)
+def forces_bytes(n, real_bytes=8):
+    """Force kernel (circular, fused): per agent read position 16, velocity,
+    force and target direction (2 reals each), target 8, cell index 4, ten
+    fp64 parameters 80, one slot-list id 4 and the four potential values of
+    the bilinear navigation lookup 32; write force and target direction
+    (2 reals each): 144 + 10 reals = 224 B/agent in fp64, 184 B in fp32."""
+    return n * (16 + 6 * real_bytes + 8 + 4 + 80 + 4 + 32 + 4 * real_bytes)
+
+
+def integrate_bytes(n, cells, real_bytes=8):
+    """Integrate kernel: per agent read position 16, velocity, force,
+    force_prev (2 reals each), mass and target speed 16; write position 16,
+    velocity, force_prev and the reset force (2 reals each): 48 + 12 reals
+    = 144 B/agent in fp64; clear the grid's cell counts (4 B per cell)."""
+    return n * (48 + 12 * real_bytes) + 4 * cells
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` (e.g. 'k_forces_direct<double, 0, true>') from
+    the committed summary of the two rocprofv3 counter runs (FETCH_SIZE and
+    WRITE_SIZE scaled by a 256 MiB copy), or None when it is not listed."""
+    path = os.path.join(ROOT, 'profiles', 'room20k_hbm_traffic.md')
+    try:
+        for line in open(path):
+            cells = [c.strip() for c in line.strip().strip('|').split('|')]
+            if len(cells) == 5 and cells[0] == '`%s`' % kernel:
+                return int(cells[4])
+    except OSError:
+        return None
+    return None
 
 
 def cpu_baseline(scn, seconds, threads=None):
@@ -285,6 +312,10 @@ def main():
     ap.add_argument('--no-corridor', action='store_true')
     ap.add_argument('--no-adjoint', action='store_true')
     ap.add_argument('--adjoint-steps', type=int, default=500)
+    ap.add_argument('--tune-room', action='store_true',
+                    help='autotune the force kernel of the room workload (results are '
+                         'identical for every choice; the default is the agent-parallel '
+                         'kernel with 4 lanes per agent)')
     ap.add_argument('--no-tune', action='store_true',
                     help='skip the autotuner (agent-parallel force kernel, smallest '
                          'physics-neutral ensemble cell)')
@@ -324,8 +355,8 @@ def main():
                        cell_size=scn.cell_size, dt_min=scn.dt_min, dt_max=scn.dt_max,
                        seed=scn.seed, precision=prec, bounds=scn.bounds, tile=tile, lanes=lanes)
 
-    kern, kern_table = (0, 0, 4), []
-    if not args.no_tune:
+    kern, kern_table = (0, 0, 4), []  # agent-parallel, 4 lanes per agent
+    if args.tune_room:
         with torch.cuda.stream(stream):
             kern, kern_table = tune_kernel(lambda t, l: make(args.precision, t, l))
     log('force kernel %s (candidates %s)' % (kern, [(t[1], round(t[0], 4)) for t in kern_table]))
@@ -377,30 +408,35 @@ def main():
     rb = 8 if main_p == 'f64' else 4
     per_kernel = {}
     bin_ms = km['bin_insert']
+    tname = 'double' if main_p == 'f64' else 'float'
+    knames = {'binning': 'k_bin_insert', 'forces': 'k_forces_direct<%s, 0, true>' % tname,
+              'integrate': 'k_integrate<%s, 0>' % tname}
     per_kernel['binning'] = dict(bound='hbm', unit='GB/s', ms=bin_ms,
-                                 achieved=binning_bytes(scn.n, cells) / (bin_ms * 1e-3) / 1e9,
+                                 bytes=binning_bytes(scn.n),
+                                 achieved=binning_bytes(scn.n) / (bin_ms * 1e-3) / 1e9,
                                  peak=HBM_PEAK_GBS)
-    per_kernel['integrate'] = dict(bound='hbm', unit='GB/s', ms=km['integrate'],
-                                   achieved=integrate_bytes(scn.n, rb) / (km['integrate'] * 1e-3) / 1e9,
+    ib = integrate_bytes(scn.n, cells, rb)
+    per_kernel['integrate'] = dict(bound='hbm', unit='GB/s', ms=km['integrate'], bytes=ib,
+                                   achieved=ib / (km['integrate'] * 1e-3) / 1e9,
                                    peak=HBM_PEAK_GBS)
-    per_kernel['forces'] = dict(bound='fp64-valu', unit='TFLOP/s', ms=km['forces'],
-                                achieved=flops_step / (km['forces'] * 1e-3) / 1e12,
-                                peak=FP64_PEAK_TFLOPS,
+    # the force kernel's arithmetic intensity (pair tally flops / bytes) is far
+    # below the ridge point (78.6 TFLOP/s / 8 TB/s): memory-side roofline
+    fb = forces_bytes(scn.n, rb)
+    per_kernel['forces'] = dict(bound='hbm', unit='GB/s', ms=km['forces'], bytes=fb,
+                                achieved=fb / (km['forces'] * 1e-3) / 1e9, peak=HBM_PEAK_GBS,
+                                flops=flops_step,
+                                flops_achieved_tflops=flops_step / (km['forces'] * 1e-3) / 1e12,
+                                flops_frac=flops_step / (km['forces'] * 1e-3) / 1e12 /
+                                FP64_PEAK_TFLOPS,
                                 candidate_pairs_per_s=cand_step / (km['forces'] * 1e-3))
-    for v in per_kernel.values():
+    for k, v in per_kernel.items():
         v['frac'] = v['achieved'] / v['peak']
+        v['traffic'] = measured_traffic(knames[k]) if kern == (0, 0, 4) else None
     dom = max(STEP_KERNELS, key=lambda k: km[k])
-    if dom == 'forces':
-        rk = per_kernel['forces']
-        traffic = None
-    elif dom == 'integrate':
-        rk = per_kernel['integrate']
-        traffic = None
-    else:
-        rk = dict(per_kernel['binning'])
-        traffic = None
-    roofline = dict(kernel=dom, bound=rk['bound'], achieved=rk['achieved'], peak=rk['peak'],
-                    unit=rk['unit'], frac=rk['frac'], traffic=traffic)
+    rk = per_kernel['binning' if dom == 'bin_insert' else dom]
+    roofline = dict(kernel=knames['binning' if dom == 'bin_insert' else dom], bound=rk['bound'],
+                    achieved=rk['achieved'], peak=rk['peak'], unit=rk['unit'], frac=rk['frac'],
+                    traffic=rk['traffic'], bytes_per_launch=rk['bytes'], ms_per_launch=rk['ms'])
 
     ensemble = None
     if not args.no_ensemble:
