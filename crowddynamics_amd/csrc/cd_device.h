@@ -20,12 +20,25 @@ struct __attribute__((aligned(2 * sizeof(T)))) V2 {
   T x, y;
 };
 
+// Arithmetic of the pair terms by precision.  fp64 (and the dual numbers of
+// the adjoint): IEEE operations in the reference's order.  fp32 mode: the
+// hardware's approximate reciprocal, square root and exponential (a few
+// ulp each), well inside the mode's stated error bound against the fp64
+// oracle.
 __device__ __forceinline__ double dhypot(double a, double b) { return hypot(a, b); }
-__device__ __forceinline__ float dhypot(float a, float b) { return hypotf(a, b); }
+__device__ __forceinline__ float dhypot(float a, float b) { return __builtin_amdgcn_sqrtf(a * a + b * b); }
 __device__ __forceinline__ double dsqrt(double a) { return sqrt(a); }
-__device__ __forceinline__ float dsqrt(float a) { return sqrtf(a); }
+__device__ __forceinline__ float dsqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
 __device__ __forceinline__ double dexp(double a) { return exp(a); }
-__device__ __forceinline__ float dexp(float a) { return expf(a); }
+__device__ __forceinline__ float dexp(float a) { return __expf(a); }
+template <typename T>
+__device__ __forceinline__ T ddiv(T a, T b) {
+  return a / b;
+}
+template <>
+__device__ __forceinline__ float ddiv<float>(float a, float b) {
+  return __fdividef(a, b);
+}
 __device__ __forceinline__ bool is_nan(double a) { return __builtin_isnan(a); }
 __device__ __forceinline__ bool is_nan(float a) { return __builtin_isnan(a); }
 
@@ -68,7 +81,7 @@ __device__ __forceinline__ void force_contact(T h, T nx, T ny, T vx, T vy, T tx,
 // magnitude, core/motion/power_law.py:83-102
 template <typename T>
 __device__ __forceinline__ T magnitude(T tau, T tau_0) {
-  return ((T(2) / tau + T(1) / tau_0) * dexp(-tau / tau_0)) / (tau * tau);
+  return ddiv((ddiv(T(2), tau) + ddiv(T(1), tau_0)) * dexp(ddiv(-tau, tau_0)), tau * tau);
 }
 
 // truncate, core/vector2D.py:167-187
@@ -76,7 +89,7 @@ template <typename T>
 __device__ __forceinline__ void truncate(T &x, T &y, T l) {
   T vlen = dhypot(x, y);
   if (vlen > l) {
-    T s = l / vlen;
+    T s = ddiv(l, vlen);
     x *= s;
     y *= s;
   }
@@ -161,10 +174,10 @@ __device__ __forceinline__ bool pair_circular(T x, T y, T vx, T vy, T r_tot, con
   T c = (x * x + y * y) - r_tot * r_tot;
   T ds = dsqrt(b * b - a * c);
   if (!(is_nan(ds) || ds == T(0) || a == T(0))) {
-    T tau = (b - ds) / a;
+    T tau = ddiv(b - ds, a);
     if (!(tau <= T(0) || tau > T(kTauMax))) {
-      T gx = (vx - (vx * b + x * a) / ds) / a;
-      T gy = (vy - (vy * b + y * a) / ds) / a;
+      T gx = ddiv(vx - ddiv(vx * b + x * a, ds), a);
+      T gy = ddiv(vy - ddiv(vy * b + y * a, ds), a);
       T m = magnitude(tau, o.tau_0);
       sx = T(0) + (o.neg_mk * gx) * m;
       sy = T(0) + (o.neg_mk * gy) * m;
@@ -174,8 +187,8 @@ __device__ __forceinline__ bool pair_circular(T x, T y, T vx, T vy, T r_tot, con
   if (h < T(0)) {
     T nx = T(0), ny = T(0);
     if (d != T(0)) {
-      nx = x / d;
-      ny = y / d;
+      nx = ddiv(x, d);
+      ny = ddiv(y, d);
     }
     T cx, cy;
     force_contact(h, nx, ny, vx, vy, ny, -nx, o.mu, o.kappa, o.damping, cx, cy);
