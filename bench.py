@@ -26,7 +26,7 @@ sys.path.insert(0, ROOT)
 METRIC = ('agent-steps/s @20k Circular (GPU fp64/fp32 vs host numba); adjoint step time')
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
-KERNELS = ('bin_insert', 'forces', 'integrate', 'step')
+KERNELS = ('step_begin', 'forces', 'integrate', 'step')
 STEP_KERNELS = KERNELS[:3]
 
 
@@ -61,20 +61,24 @@ def pair_tally(arr, cell, segments):
                 wall_pairs=w_all, wall_contacts=w_con, flops=flops)
 
 
-def binning_bytes(n):
-    """Binning kernel: per agent read the position (16 B), write its cell
-    index and its id into the cell's slot list (4 + 4 B), and read-modify-
-    write its cell's count (8 B): 32 B/agent."""
-    return 32 * n
+def step_begin_bytes(n, real_bytes=8):
+    """First kernel of the step (binning and the own terms), per agent.
+    Binning: read the position 16, write the cell index and the slot id
+    (4 + 4), read-modify-write the cell's count 8.  Own terms: read mass,
+    fluctuation sigma, tau_adj, target speed and target 40, the four
+    potential values of the bilinear navigation lookup 32, force, velocity
+    and target direction (2 reals each); write force and target direction
+    (2 reals each).  104 + 10 reals = 184 B/agent in fp64, 144 B in fp32."""
+    return n * (32 + 40 + 32 + 10 * real_bytes)
 
 
 def forces_bytes(n, real_bytes=8):
-    """Force kernel (circular, fused): per agent read position 16, velocity,
-    force and target direction (2 reals each), target 8, cell index 4, ten
-    fp64 parameters 80, one slot-list id 4 and the four potential values of
-    the bilinear navigation lookup 32; write force and target direction
-    (2 reals each): 144 + 10 reals = 224 B/agent in fp64, 184 B in fp32."""
-    return n * (16 + 6 * real_bytes + 8 + 4 + 80 + 4 + 32 + 4 * real_bytes)
+    """Force kernel (circular, fused), per agent: read position 16, cell
+    index 4, radius and the six pair and contact parameters 56, its id in a
+    slot list 4, velocity and force (2 reals each); write force (2 reals).
+    80 + 6 reals = 128 B/agent in fp64, 104 B in fp32 (neighbour data is
+    counted once: every agent is read as a neighbour from cache)."""
+    return n * (16 + 4 + 56 + 4 + 6 * real_bytes)
 
 
 def integrate_bytes(n, cells, real_bytes=8):
@@ -85,15 +89,16 @@ def integrate_bytes(n, cells, real_bytes=8):
     return n * (48 + 12 * real_bytes) + 4 * cells
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` (e.g. 'k_forces_direct<double, 0, true>') from
-    the committed summary of the two rocprofv3 counter runs (FETCH_SIZE and
-    WRITE_SIZE scaled by a 256 MiB copy), or None when it is not listed."""
+def measured_traffic(kernel, threads):
+    """HBM bytes per launch of `kernel` (e.g. 'k_forces_direct<double, 0, true>')
+    at a grid of `threads` threads, from the committed summary of the two
+    rocprofv3 counter runs (FETCH_SIZE and WRITE_SIZE scaled by a 256 MiB
+    copy), or None when it is not listed."""
     path = os.path.join(ROOT, 'profiles', 'room20k_hbm_traffic.md')
     try:
         for line in open(path):
             cells = [c.strip() for c in line.strip().strip('|').split('|')]
-            if len(cells) == 5 and cells[0] == '`%s`' % kernel:
+            if len(cells) == 5 and cells[0] == '`%s`' % kernel and cells[1] == str(threads):
                 return int(cells[4])
     except OSError:
         return None
@@ -312,13 +317,10 @@ def main():
     ap.add_argument('--no-corridor', action='store_true')
     ap.add_argument('--no-adjoint', action='store_true')
     ap.add_argument('--adjoint-steps', type=int, default=500)
-    ap.add_argument('--tune-room', action='store_true',
-                    help='autotune the force kernel of the room workload (results are '
-                         'identical for every choice; the default is the agent-parallel '
-                         'kernel with 4 lanes per agent)')
     ap.add_argument('--no-tune', action='store_true',
-                    help='skip the autotuner (agent-parallel force kernel, smallest '
-                         'physics-neutral ensemble cell)')
+                    help='skip the autotuners (room: agent-parallel force kernel with 8 '
+                         'lanes per agent; ensemble: 4 lanes and the smallest '
+                         'physics-neutral cell; results are identical for every choice)')
     ap.add_argument('--calibrate-copy', action='store_true',
                     help='first run a plain 256 MiB device copy five times (a kernel of '
                          'known traffic for scaling hardware byte counters)')
@@ -355,10 +357,11 @@ def main():
                        cell_size=scn.cell_size, dt_min=scn.dt_min, dt_max=scn.dt_max,
                        seed=scn.seed, precision=prec, bounds=scn.bounds, tile=tile, lanes=lanes)
 
-    kern, kern_table = (0, 0, 4), []  # agent-parallel, 4 lanes per agent
-    if args.tune_room:
+    kern, kern_table = (0, 0, 8), []  # agent-parallel, 8 lanes per agent
+    if not args.no_tune:  # lanes per agent of the agent-parallel kernel
         with torch.cuda.stream(stream):
-            kern, kern_table = tune_kernel(lambda t, l: make(args.precision, t, l))
+            kern, kern_table = tune_kernel(lambda t, l: make(args.precision, t, l),
+                                           kernels=((0, 0, 4), (0, 0, 8), (0, 0, 16)))
     log('force kernel %s (candidates %s)' % (kern, [(t[1], round(t[0], 4)) for t in kern_table]))
     for prec in ('f64', 'f32'):
         with torch.cuda.stream(stream):
@@ -386,7 +389,7 @@ def main():
             kernel_ms[prec] = {k: ms[i] / nt for i, k in enumerate(KERNELS)}
             b2b = {}
             for k, name in (('forces', 'forces'), ('integrate', 'integrate'),
-                            ('bin', 'bin_insert')):
+                            ('bin', 'step_begin')):
                 st.prepare()
                 b2b[name] = st.kernel_timed(k, 20)
             kernel_b2b[prec] = b2b
@@ -407,14 +410,18 @@ def main():
     cand_step = 0.5 * (t0_tally['candidate_pairs'] + t1_tally['candidate_pairs'])
     rb = 8 if main_p == 'f64' else 4
     per_kernel = {}
-    bin_ms = km['bin_insert']
     tname = 'double' if main_p == 'f64' else 'float'
-    knames = {'binning': 'k_bin_insert', 'forces': 'k_forces_direct<%s, 0, true>' % tname,
+    knames = {'step_begin': 'k_step_begin<%s, 0>' % tname,
+              'forces': 'k_forces_direct<%s, 0, true>' % tname,
               'integrate': 'k_integrate<%s, 0>' % tname}
-    per_kernel['binning'] = dict(bound='hbm', unit='GB/s', ms=bin_ms,
-                                 bytes=binning_bytes(scn.n),
-                                 achieved=binning_bytes(scn.n) / (bin_ms * 1e-3) / 1e9,
-                                 peak=HBM_PEAK_GBS)
+    # grid threads of each launch (the traffic summary is keyed by them)
+    f_threads = min(-(-scn.n // 256), 128) * 256  # blocks_f: one partial per block
+    grid = {'step_begin': f_threads, 'integrate': f_threads,
+            'forces': -(-scn.n // (256 // kern[2])) * 256}
+    sb = step_begin_bytes(scn.n, rb)
+    per_kernel['step_begin'] = dict(bound='hbm', unit='GB/s', ms=km['step_begin'], bytes=sb,
+                                    achieved=sb / (km['step_begin'] * 1e-3) / 1e9,
+                                    peak=HBM_PEAK_GBS)
     ib = integrate_bytes(scn.n, cells, rb)
     per_kernel['integrate'] = dict(bound='hbm', unit='GB/s', ms=km['integrate'], bytes=ib,
                                    achieved=ib / (km['integrate'] * 1e-3) / 1e9,
@@ -430,13 +437,15 @@ def main():
                                 FP64_PEAK_TFLOPS,
                                 candidate_pairs_per_s=cand_step / (km['forces'] * 1e-3))
     for k, v in per_kernel.items():
+        v['kernel'] = knames[k]
+        v['grid_threads'] = grid[k]
         v['frac'] = v['achieved'] / v['peak']
-        v['traffic'] = measured_traffic(knames[k]) if kern == (0, 0, 4) else None
+        v['traffic'] = measured_traffic(knames[k], grid[k])
     dom = max(STEP_KERNELS, key=lambda k: km[k])
-    rk = per_kernel['binning' if dom == 'bin_insert' else dom]
-    roofline = dict(kernel=knames['binning' if dom == 'bin_insert' else dom], bound=rk['bound'],
-                    achieved=rk['achieved'], peak=rk['peak'], unit=rk['unit'], frac=rk['frac'],
-                    traffic=rk['traffic'], bytes_per_launch=rk['bytes'], ms_per_launch=rk['ms'])
+    rk = per_kernel[dom]
+    roofline = dict(kernel=knames[dom], bound=rk['bound'], achieved=rk['achieved'],
+                    peak=rk['peak'], unit=rk['unit'], frac=rk['frac'], traffic=rk['traffic'],
+                    bytes_per_launch=rk['bytes'], ms_per_launch=rk['ms'])
 
     ensemble = None
     if not args.no_ensemble:

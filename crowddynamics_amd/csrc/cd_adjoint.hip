@@ -248,7 +248,7 @@ int cd_adjoint_step(const cd_agents_t *a, const cd_step_params_t *p, const doubl
   if (p->flags & CD_USE_AGENT_AGENT) {
     if ((rc = launch_reduce(A, W, B, L.blocks_f, true, s))) return rc;
     hipLaunchKernelGGL(k_bin_insert, dim3(L.blocks_f), dim3(kBS), 0, s, (const double2 *)A.pos,
-                       p->cell_size, W, B, 1, 0);
+                       p->cell_size, W, B, 1);
     hipLaunchKernelGGL(k_cell_gather<double>, dim3(grid_for(W.cap, kBS)), dim3(kBS), 0, s, A, W);
     if ((rc = check_launch())) return rc;
   }

@@ -72,6 +72,7 @@ struct ScnState {
 
 constexpr int kBS = 256;              // threads per block of per-agent kernels
 constexpr int kMaxPartsPerScn = 128;  // blocks of the partial-producing kernels per scenario
+static_assert(kMaxPartsPerScn <= kBS, "one partial per thread of a consumer block");
 constexpr int kSingleParts = 512;     // the same for a single (unbatched) scenario
 constexpr int kAgentsPerForceBlock = 64;  // default agents per force block (256 threads)
 constexpr int kMinForceAgents = 4;  // agents per force block: 64 threads / 16 lanes
@@ -397,11 +398,19 @@ __device__ __forceinline__ void store_partial(const Work &W, const Red &r) {
 }
 
 // Consumer side: every block combines its scenario's partials.
-__device__ Red reduce_partials(const Work &W, const ScnDesc &d) {
+// This thread's partial of the scenario (the identity past the scenario's
+// nf <= kMaxPartsPerScn <= blockDim partials): loaded, not yet combined, so
+// the loads stay in flight until block_reduce uses them.
+__device__ __forceinline__ Red load_partial(const Work &W, const ScnDesc &d) {
   Red r;
   r.init();
-  for (int b = threadIdx.x; b < d.nf; b += blockDim.x) r.combine(&W.partials[(d.f0 + b) * 8]);
-  return block_reduce(r);
+  if ((int)threadIdx.x < d.nf)
+    for (int k = 0; k < R_COUNT; ++k) r.v[k] = W.partials[(d.f0 + threadIdx.x) * 8 + k];
+  return r;
+}
+
+__device__ Red reduce_partials(const Work &W, const ScnDesc &d) {
+  return block_reduce(load_partial(W, d));
 }
 
 __device__ __forceinline__ double red_vmax(const unsigned long long *v) {
@@ -486,24 +495,18 @@ __device__ __forceinline__ int cell_of_pos(double2 p, const Grid &g, double cell
 // K_a / K_b: per-cell slot lists
 // ---------------------------------------------------------------------------
 
-// K_a: combine the scenario's partials; when `binning`, bin its agents into
-// their cells' slot lists (slots beyond cell_slots are not stored; the cell
-// count tells).  Cell indices are global: scenario base + local cell.
-// `count_step`: the fused step counts itself here (no kernel of the step
-// reads the counter concurrently; the force and integrate kernels of the
-// step read it as step + 1).
-__global__ void __launch_bounds__(256) k_bin_insert(const double2 *pos, double cell, Work W,
-                                                    Batch B, int binning, int count_step) {
-  const int2 sk = block_f(B);
-  const ScnDesc d = desc_of(B, sk.x);
-  ScnState *ss = &B.sst[sk.x];
-  const int t0 = sk.y * blockDim.x + threadIdx.x, stride = d.nf * blockDim.x;
-  const int i0 = d.a0 + t0;
-  const double2 p0 = (binning && i0 < d.a1) ? pos[i0] : make_double2(0.0, 0.0);
-  const Red r = reduce_partials(W, d);
+// Binning of a scenario's agents with the grid of its reduced partials:
+// publishes the grid (one thread per scenario) and, when `binning`, inserts
+// the agents into their cells' slot lists (slots beyond cell_slots are not
+// stored; the cell count tells).  Cell indices are global: scenario base +
+// local cell.  p0: the position of the block's first agent, loaded early.
+__device__ __forceinline__ void bin_scenario(const double2 *pos, double2 p0, double cell,
+                                             const Work &W, const ScnDesc &d, ScnState *ss,
+                                             int2 sk, const Red &r, int binning) {
+  const int stride = d.nf * blockDim.x;
+  const int i0 = d.a0 + sk.y * blockDim.x + threadIdx.x;
   const Grid g = grid_of(r, cell, d.ccap);
   if (sk.y == 0 && threadIdx.x == 0) publish_grid(W, ss, r, g, binning != 0, d.a1 > d.a0);
-  if (count_step && blockIdx.x == 0 && threadIdx.x == 0) W.st->step += 1;
   if (!binning || !g.ok) return;
   for (int i = i0; i < d.a1; i += stride) {
     const int lc = cell_of_pos(i == i0 ? p0 : pos[i], g, cell);
@@ -520,6 +523,19 @@ __global__ void __launch_bounds__(256) k_bin_insert(const double2 *pos, double c
     else if (k == W.K)  // the cell's list is full: its members are found by scanning cell_of
       atomicOr((unsigned long long *)&W.st->status, (unsigned long long)CD_STATUS_SLOTS_FULL);
   }
+}
+
+// K_a of the operator paths: combine the scenario's partials, then
+// bin_scenario.
+__global__ void __launch_bounds__(256) k_bin_insert(const double2 *pos, double cell, Work W,
+                                                    Batch B, int binning) {
+  const int2 sk = block_f(B);
+  const ScnDesc d = desc_of(B, sk.x);
+  ScnState *ss = &B.sst[sk.x];
+  const int i0 = d.a0 + sk.y * blockDim.x + threadIdx.x;
+  const double2 p0 = (binning && i0 < d.a1) ? pos[i0] : make_double2(0.0, 0.0);
+  const Red r = reduce_partials(W, d);
+  bin_scenario(pos, p0, cell, W, d, ss, sk, r, binning);
 }
 
 // K_b: one thread per cell of the workspace grid.  Orders the cell's slot

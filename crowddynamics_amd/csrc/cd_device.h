@@ -208,14 +208,23 @@ __device__ __forceinline__ double np_sign(double v) {
   return v;
 }
 
-// distance_circle_line, core/distance.py:108-146.  Absolute coordinates in
-// fp64 (positions are fp64 in both precision modes).
+// Per-segment part of distance_circle_line: unit tangent (tx, ty) and
+// length l_w of the segment p0 -> p1 (returned as {tx, ty, l_w, 0}).
+__device__ __forceinline__ double4 seg_frame(double p0x, double p0y, double p1x, double p1y) {
+  const double dx = p1x - p0x, dy = p1y - p0y;
+  const double l_w = hypot(dx, dy);
+  return make_double4(dx / l_w, dy / l_w, l_w, 0.0);
+}
+
+// distance_circle_line, core/distance.py:108-146, given the segment's frame
+// fr = seg_frame(p0, p1) (the same operations on the same values as
+// computing it here).  Absolute coordinates in fp64 (positions are fp64 in
+// both precision modes).
 __device__ __forceinline__ void distance_circle_line(double x, double y, double r, double p0x,
                                                      double p0y, double p1x, double p1y,
-                                                     double &h, double &nx, double &ny) {
-  double dx = p1x - p0x, dy = p1y - p0y;
-  double l_w = hypot(dx, dy);
-  double tx = dx / l_w, ty = dy / l_w;
+                                                     const double4 &fr, double &h, double &nx,
+                                                     double &ny) {
+  const double tx = fr.x, ty = fr.y, l_w = fr.z;
   double nwx = -ty, nwy = tx;
   double q0x = x - p0x, q0y = y - p0y;
   double q1x = x - p1x, q1y = y - p1y;

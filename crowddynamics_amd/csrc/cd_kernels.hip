@@ -4,20 +4,22 @@
 // Workspace layout, binning contract, reductions and the binning kernels
 // K_a / K_b are in cd_common.h.
 //
-// One step is four kernels, each launched once over all scenarios of a
+// One step is three kernels, each launched once over all scenarios of a
 // batch (a single simulation is a batch of one):
-//   K_a  k_bin_insert     combine the previous step's partials per scenario
-//                         (bounding box, max |v|, max v0); bin every agent
-//                         into its cell's slot list
-//   K_b  k_cell_gather    per cell: order the slot list by agent index and
-//                         gather the neighbour data the force kernel reads
-//   K_e  k_forces         four lanes per agent: stencil rows, walls, own terms
+//   K_a  k_step_begin     per agent: the step's own terms (fluctuation,
+//                         navigation, orientation, adjusting); per scenario:
+//                         combine the previous step's partials (bounding
+//                         box, max |v|, max v0) and bin every agent into its
+//                         cell's slot list
+//   K_e  k_forces_direct  L = 4, 8 or 16 lanes per agent: stencil rows of
+//        (k_forces_tiled) the pairs and the walls, summed exactly
 //   K_f  k_integrate      Verlet update, Reset, per-block partials for the
-//                         next step, clear the cell counts
+//                         next step, clear the cell counts, count the step
 // Every block of K_a, K_e and K_f serves a single scenario (block maps built
 // by cd_batch_prepare).  Pair forces are owner-computes: each agent sums its
-// own side of every pair, so forces need no atomics and results are
-// run-to-run deterministic.
+// own side of every pair, so forces need no atomics, and the sums are exact,
+// so results do not depend on the lanes, tiles or insertion order.
+// K_b (k_cell_gather, cd_common.h) serves the adjoint only.
 
 #include "cd_common.h"
 
@@ -207,10 +209,12 @@ __device__ __forceinline__ void fluct_noise(uint64_t seed, uint64_t step, int i,
                                             double sig_t, double &fx, double &fy, double &m) {
   double ua, ub;
   draw2(seed, i, step, 0, 0, ua, ub);
-  const double phi = 2.0 * kPi * ua;
+  // (cos, sin)(2 pi ua) as sincospi(2 ua): exact argument, short reduction
+  double sp, cp;
+  sincospi(2.0 * ua, &sp, &cp);
   const double mm = mass * (sig_f * fabs(truncnorm3(seed, i, step, 1)));
-  fx = mm * cos(phi);
-  fy = mm * sin(phi);
+  fx = mm * cp;
+  fy = mm * sp;
   m = torque ? inertia * (sig_t * truncnorm3(seed, i, step, 2)) : 0.0;
 }
 
@@ -332,38 +336,58 @@ __global__ void __launch_bounds__(256) k_reset(Agents<T> A, int model) {
   if (model) A.torque[i] = 0.0;
 }
 
-// Wall contacts of agent i over segments w_begin, w_begin + w_stride, ...
-// (core/interactions.py:107-141, 169-186): add(force x, force y, torque)
-// for each contact.
-template <typename T, class Add>
-__device__ __forceinline__ void walls_each(const Agents<T> &A, int model, int i, double2 p,
-                                           V2<T> v, const double4 *seg, int W, Add &&add,
-                                           int w_begin = 0, int w_stride = 1) {
-  const double r = A.radius[i];
-  double2 ls = p, rs = p;
-  double rt = 0.0, rsr = 0.0;
+// Agent-side inputs of the wall contacts (loaded once per agent).
+template <typename T>
+struct WallSelf {
+  double r, rt, rsr;
+  double2 ls, rs;
+  T mu, kappa, damping;
+};
+
+template <typename T>
+__device__ __forceinline__ WallSelf<T> wall_self(const Agents<T> &A, int model, int i, double2 p) {
+  WallSelf<T> w;
+  w.r = A.radius[i];
+  w.ls = p;
+  w.rs = p;
+  w.rt = 0.0;
+  w.rsr = 0.0;
   if (model) {
-    ls = A.ls[i];
-    rs = A.rs[i];
-    rt = A.r_t[i];
-    rsr = A.r_s[i];
+    w.ls = A.ls[i];
+    w.rs = A.rs[i];
+    w.rt = A.r_t[i];
+    w.rsr = A.r_s[i];
   }
-  const T mu = (T)A.mu[i], kappa = (T)A.kappa[i], damping = (T)A.damping[i];
+  w.mu = (T)A.mu[i];
+  w.kappa = (T)A.kappa[i];
+  w.damping = (T)A.damping[i];
+  return w;
+}
+
+// Wall contacts of an agent at p with velocity v over segments w_begin,
+// w_begin + w_stride, ... (core/interactions.py:107-141, 169-186):
+// add(force x, force y, torque) for each contact.
+// `frm`: the segments' frames (seg_frame), or null to compute them here.
+template <typename T, class Add>
+__device__ __forceinline__ void walls_each(const WallSelf<T> &me, int model, double2 p, V2<T> v,
+                                           const double4 *seg, const double4 *frm, int W,
+                                           Add &&add, int w_begin = 0, int w_stride = 1) {
   for (int w = w_begin; w < W; w += w_stride) {
     const double4 s = seg[w];
+    const double4 fr = frm ? frm[w] : seg_frame(s.x, s.y, s.z, s.w);
     double h, nx, ny, rmx = 0.0, rmy = 0.0;
     if (!model) {
-      distance_circle_line(p.x, p.y, r, s.x, s.y, s.z, s.w, h, nx, ny);
+      distance_circle_line(p.x, p.y, me.r, s.x, s.y, s.z, s.w, fr, h, nx, ny);
     } else {
       // distance_three_circle_line, core/distance.py:149-180
-      const double px[3] = {p.x, ls.x, rs.x}, py[3] = {p.y, ls.y, rs.y};
-      const double rr[3] = {rt, rsr, rsr};
+      const double px[3] = {p.x, me.ls.x, me.rs.x}, py[3] = {p.y, me.ls.y, me.rs.y};
+      const double rr[3] = {me.rt, me.rsr, me.rsr};
       double h_min = __builtin_nan("");
       double bx = 0.0, by = 0.0, sx = 0.0, sy = 0.0, sr_ = 0.0;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         double hk, ex, ey;
-        distance_circle_line(px[k], py[k], rr[k], s.x, s.y, s.z, s.w, hk, ex, ey);
+        distance_circle_line(px[k], py[k], rr[k], s.x, s.y, s.z, s.w, fr, hk, ex, ey);
         if (hk < h_min || __builtin_isnan(h_min)) {
           h_min = hk;
           bx = ex;
@@ -382,18 +406,18 @@ __device__ __forceinline__ void walls_each(const Agents<T> &A, int model, int i,
     if (h < 0.0) {
       T cx, cy;
       const T tnx = (T)nx, tny = (T)ny;
-      force_contact((T)h, tnx, tny, v.x, v.y, tny, -tnx, mu, kappa, damping, cx, cy);
+      force_contact((T)h, tnx, tny, v.x, v.y, tny, -tnx, me.mu, me.kappa, me.damping, cx, cy);
       add(cx, cy, model ? rmx * (double)cy - rmy * (double)cx : 0.0);
     }
   }
 }
 
-// The same, summed in segment order into fx, fy, tq.
+// The same for agent i, summed in segment order into fx, fy, tq.
 template <typename T>
 __device__ __forceinline__ void walls_agent(const Agents<T> &A, int model, int i, double2 p,
                                             V2<T> v, const double4 *seg, int W, T &fx, T &fy,
                                             double &tq) {
-  walls_each(A, model, i, p, v, seg, W, [&](T cx, T cy, double m) {
+  walls_each(wall_self(A, model, i, p), model, p, v, seg, nullptr, W, [&](T cx, T cy, double m) {
     fx += cx;
     fy += cy;
     if (model) tq += m;
@@ -435,10 +459,10 @@ __global__ void __launch_bounds__(256) k_walls(Agents<T> A, int model, const dou
 // K_e: forces (owner-computes over the 3x3 cell stencil)
 // ---------------------------------------------------------------------------
 //
-// Four lanes serve one agent: lanes 0..2 sum the agent's side of every pair
-// in stencil rows dx = -1, 0, +1 (cells dy = -1, 0, +1); lane 3 evaluates the
-// agent's own terms (fluctuation, navigation, orientation, adjusting); wall
-// contacts are split over the four lanes in a fixed order.  Pair terms are
+// L = 4, 8 or 16 lanes serve one agent: the candidates of its 3 x 3 cell
+// stencil, taken as one sequence, are split evenly over the lanes, and so
+// are the wall segments.  (The agent's own terms were added by the step's
+// first kernel.)  Pair and wall terms are
 // summed exactly (Fix128), so the result does not depend on the order in
 // which the binning inserted agents into cells or on how lanes and
 // workgroups split the work; it is the correctly rounded sum of the terms up
@@ -608,63 +632,162 @@ __device__ __forceinline__ void pair_add_tc(int i, int j, const TCView &me, cons
 }
 
 // Own (per-agent) terms of the fused step in the reference's post-order:
-// fluctuation, navigation (e0), orientation (phi0), adjusting.  Adds to
-// f / tq and writes e0 / phi0.  `local` is the agent's index within its
-// scenario (the noise counter).
+// fluctuation, navigation (e0), orientation (phi0), adjusting; force
+// (torque) += fluctuation + adjusting.  own_load issues every load they need
+// (including the navigation map's), own_apply computes and stores, so a
+// kernel can put independent work between the two.
+template <typename T>
+struct OwnIn {
+  V2<T> f, vel, e;
+  double tq, mass, sig_f, inertia, sig_t, tau_adj, v0, ex, ey;
+  bool nav;
+};
+
 template <typename T, int MODEL>
-__device__ __forceinline__ void own_terms(const Agents<T> &A, const FusedArgs &fa, uint64_t seed,
-                                          uint64_t step, int i, int local, double2 pos, V2<T> vel,
-                                          V2<T> &f, double &tq) {
+__device__ __forceinline__ OwnIn<T> own_load(const Agents<T> &A, const FusedArgs &fa, int i,
+                                             double2 pos) {
+  OwnIn<T> o;
+  o.f = A.force[i];
+  o.tq = MODEL ? A.torque[i] : 0.0;
+  o.vel = A.vel[i];
+  o.mass = A.mass[i];
+  o.sig_f = o.inertia = o.sig_t = o.tau_adj = o.v0 = o.ex = o.ey = 0.0;
+  if (fa.flags & CD_USE_FLUCT) {
+    o.sig_f = A.sig_f[i];
+    if (MODEL) {
+      o.inertia = A.inertia[i];
+      o.sig_t = A.sig_t[i];
+    }
+  }
+  o.e = A.e0[i];
+  o.nav = (fa.flags & CD_USE_NAV) &&
+          nav_lookup(pos.x, pos.y, A.target[i], fa.maps, fa.nav_T, fa.nav_rows, fa.nav_cols,
+                     fa.nav_minx, fa.nav_miny, fa.nav_step, fa.nav_mode, o.ex, o.ey);
+  if (fa.flags & CD_USE_ADJUST) {
+    o.tau_adj = A.tau_adj[i];
+    o.v0 = A.v0[i];
+  }
+  return o;
+}
+
+// `local` is the agent's index within its scenario (the noise counter).
+template <typename T, int MODEL>
+__device__ __forceinline__ void own_apply(const Agents<T> &A, const FusedArgs &fa, uint64_t seed,
+                                          uint64_t step, int i, int local, const OwnIn<T> &o) {
+  V2<T> f = o.f;
+  double tq = o.tq;
   if (fa.flags & CD_USE_FLUCT) {
     double nfx, nfy, nm;
-    fluct_noise(seed, step, local, A.mass[i], A.sig_f[i], MODEL != 0, MODEL ? A.inertia[i] : 0.0,
-                MODEL ? A.sig_t[i] : 0.0, nfx, nfy, nm);
+    fluct_noise(seed, step, local, o.mass, o.sig_f, MODEL != 0, o.inertia, o.sig_t, nfx, nfy, nm);
     f.x += (T)nfx;
     f.y += (T)nfy;
     if (MODEL) tq += nm;
   }
-  V2<T> e = A.e0[i];
-  if (fa.flags & CD_USE_NAV) {
-    double ex, ey;
-    if (nav_lookup(pos.x, pos.y, A.target[i], fa.maps, fa.nav_T, fa.nav_rows, fa.nav_cols,
-                   fa.nav_minx, fa.nav_miny, fa.nav_step, fa.nav_mode, ex, ey)) {
-      e.x = (T)ex;
-      e.y = (T)ey;
-      A.e0[i] = e;
-    }
+  V2<T> e = o.e;
+  if (o.nav) {
+    e.x = (T)o.ex;
+    e.y = (T)o.ey;
+    A.e0[i] = e;
   }
   if (fa.flags & CD_USE_ADJUST) {
     if (MODEL) A.phi0[i] = atan2((double)e.y, (double)e.x);
-    T ax, ay;
-    adjust_force(A, i, vel, e, ax, ay);
-    f.x += ax;
-    f.y += ay;
+    // force_adjust (core/motion/adjusting.py:16-51), as adjust_force
+    const T s = (T)(o.mass / o.tau_adj);
+    const T v0 = (T)o.v0;
+    f.x += s * (v0 * e.x - o.vel.x);
+    f.y += s * (v0 * e.y - o.vel.y);
     if (MODEL) tq += adjust_torque(A, i);
+  }
+  A.force[i] = f;
+  if (MODEL) A.torque[i] = tq;
+}
+
+// K_a of the fused step: per scenario, combine the previous step's partials
+// and publish the grid; per agent, bin it (bin_scenario's contract) and add
+// the step's own terms.  The own terms need neither grid nor neighbours:
+// the first agent's inputs (and its navigation map entry) are loaded before
+// the partials are reduced, and its noise is computed while its slot atomic
+// is in flight.  The step's index is the counter's value: no other kernel
+// of the step has advanced it yet (K_f does, last).
+template <typename T, int MODEL>
+__global__ void __launch_bounds__(256) k_step_begin(Agents<T> A, Work W, Batch B, FusedArgs fa,
+                                                    double cell, int binning) {
+  const int2 sk = block_f(B);
+  const ScnDesc d = desc_of(B, sk.x);
+  ScnState *ss = &B.sst[sk.x];
+  const uint64_t step = W.st->step;
+  const int stride = d.nf * blockDim.x;
+  const int i0 = d.a0 + sk.y * blockDim.x + threadIdx.x;
+  const Red part = load_partial(W, d);
+  double2 p0 = make_double2(0.0, 0.0);
+  OwnIn<T> o0;
+  if (i0 < d.a1) {
+    p0 = A.pos[i0];
+    o0 = own_load<T, MODEL>(A, fa, i0, p0);
+  }
+  const Red r = block_reduce(part);
+  const Grid g = grid_of(r, cell, d.ccap);
+  if (sk.y == 0 && threadIdx.x == 0) publish_grid(W, ss, r, g, binning != 0, d.a1 > d.a0);
+  const bool bin = binning && g.ok;
+  auto serve = [&](int i, double2 pos, const OwnIn<T> &o) {
+    int c = -1, k = 0;
+    if (bin) {
+      const int lc = cell_of_pos(pos, g, cell);
+      if (lc < 0) {  // positions changed after the bounding box was reduced
+        atomicOr(&ss->oob, 1);
+        W.cell_of[i] = -1;
+      } else {
+        c = d.c0 + lc;
+        W.cell_of[i] = c;
+        k = atomicAdd(&W.count[c], 1);
+      }
+    }
+    own_apply<T, MODEL>(A, fa, d.seed, step, i, i - d.a0, o);
+    if (c >= 0) {
+      if (k < W.K)
+        W.ell_ids[c * W.K + k] = i;
+      else if (k == W.K)  // the cell's list is full: its members are found by scanning cell_of
+        atomicOr((unsigned long long *)&W.st->status, (unsigned long long)CD_STATUS_SLOTS_FULL);
+    }
+  };
+  if (i0 < d.a1) serve(i0, p0, o0);
+  for (int i = i0 + stride; i < d.a1; i += stride) {
+    const double2 pos = A.pos[i];
+    serve(i, pos, own_load<T, MODEL>(A, fa, i, pos));
   }
 }
 
-// Neighbour sources.  each(row, part, m, self, fn) visits part `part` (of
-// m equal parts) of the candidates in stencil row dx = row - 1: the agents
-// of the cells (lx + dx, ly - 1 .. ly + 1), as one sequence.  Parts may
-// visit candidates in any order (the pair sums are exact).  From the LDS
-// elements of a staged region (the row's three cells are consecutive region
-// cells, so their elements are one contiguous range) ...
+// Neighbour sources.  each(q, L, self, fn) visits part q (of L equal
+// parts) of the candidates of the agent's 3 x 3 stencil, taken as one
+// sequence: the cells of rows dx = -1, 0, +1, each row's cells dy = -1, 0,
+// +1.  Parts may visit candidates in any order (the pair sums are exact).
+// From the LDS elements of a staged region (each row's three cells are
+// consecutive region cells, so their elements are one contiguous range) ...
 template <typename T, int MODEL>
 struct LdsRow {
   const TileLds<T, MODEL> &S;
   int RH, kx, ky;  // region column length and the agent's region cell
   template <class F>
-  __device__ __forceinline__ void each(int row, int part, int m, int self, F &&fn) const {
-    const int kk = (kx + row - 1) * RH + (ky - 1);
-    const int e0 = S.off[kk], tot = S.off[kk + 3] - e0;
-    const int b = e0 + (tot * part) / m, e = e0 + (tot * (part + 1)) / m;
-    for (int e2 = b; e2 < e; ++e2) {
-      const int j = S.id[e2];
+  __device__ __forceinline__ void each(int q, int L, int self, F &&fn) const {
+    int st[3], pre[4];
+    pre[0] = 0;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const int kk = (kx + r - 1) * RH + (ky - 1);
+      st[r] = S.off[kk];
+      pre[r + 1] = pre[r] + (S.off[kk + 3] - st[r]);
+    }
+    const int tot = pre[3];
+    const int b0 = (tot * q) / L, b1 = (tot * (q + 1)) / L;
+    for (int idx = b0; idx < b1; ++idx) {
+      const int r = (idx >= pre[1] ? 1 : 0) + (idx >= pre[2] ? 1 : 0);
+      const int e = (r == 0 ? st[0] : (r == 1 ? st[1] - pre[1] : st[2] - pre[2])) + idx;
+      const int j = S.id[e];
       if (j == self) continue;
       if constexpr (MODEL == 0)
-        fn(j, S.pos[e2], S.vel[e2], S.r[e2]);
+        fn(j, S.pos[e], S.vel[e], S.r[e]);
       else
-        fn(j, tc_lds<T, MODEL>(S, e2));
+        fn(j, tc_lds<T, MODEL>(S, e));
     }
   }
 };
@@ -684,29 +807,47 @@ struct MemRow {
     else
       fn(j, tc_agent(A, j));
   }
-  // The row's cells have consecutive flat indices.  Their slot lists are
-  // walked as one sequence in batches of four candidates whose id and data
-  // loads are issued together, so a lane waits for memory once per batch
-  // rather than once per candidate.
+  // The nine counts are loaded together.  A part's candidates are walked
+  // in batches of four whose id and data loads are issued together, so a
+  // lane waits for memory once per batch rather than once per candidate.
   template <class F>
-  __device__ __forceinline__ void each(int row, int part, int m, int self, F &&fn) const {
-    const int ox = lx + row - 1;
-    if (ox < 0 || ox >= nx) return;
-    const int g0 = c0 + ox * ny + ly - 1;  // flat index of cell dy = -1
-    const int n0 = (ly - 1 >= 0) ? W.count[g0] : 0;
-    const int n1 = W.count[g0 + 1];
-    const int n2 = (ly + 1 < ny) ? W.count[g0 + 2] : 0;
-    const int p1 = n0, p2 = n0 + n1, tot = p2 + n2;
-    const int b0 = (tot * part) / m, b1 = (tot * (part + 1)) / m;
-    if (n0 <= W.K && n1 <= W.K && n2 <= W.K) {
+  __device__ __forceinline__ void each(int q, int L, int self, F &&fn) const {
+    int gr[3], n[9];  // flat index of each row's cell dy = -1; counts
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const int ox = lx + r - 1;
+      const bool okx = ox >= 0 && ox < nx;
+      gr[r] = c0 + ox * ny + ly - 1;
+      n[3 * r] = (okx && ly - 1 >= 0) ? W.count[gr[r]] : 0;
+      n[3 * r + 1] = okx ? W.count[gr[r] + 1] : 0;
+      n[3 * r + 2] = (okx && ly + 1 < ny) ? W.count[gr[r] + 2] : 0;
+    }
+    int pre[10];
+    pre[0] = 0;
+    bool over = false;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      pre[k + 1] = pre[k] + n[k];
+      over = over || n[k] > W.K;
+    }
+    const int tot = pre[9];
+    const int b0 = (int)(((long long)tot * q) / L), b1 = (int)(((long long)tot * (q + 1)) / L);
+    if (!over) {
       for (int b = b0; b < b1; b += 4) {
         int j[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           const int idx = b + u;
-          const int k = (idx >= p1 ? 1 : 0) + (idx >= p2 ? 1 : 0);
-          const int slot = idx - (k == 0 ? 0 : (k == 1 ? p1 : p2));
-          j[u] = idx < b1 ? W.ell_ids[(size_t)(g0 + k) * W.K + slot] : -1;
+          // the cell holding candidate idx: the last k with pre[k] <= idx
+          int base = 0, cell = gr[0];
+#pragma unroll
+          for (int k = 1; k < 9; ++k) {
+            if (idx >= pre[k]) {
+              base = pre[k];
+              cell = gr[k / 3] + k % 3;
+            }
+          }
+          j[u] = idx < b1 ? W.ell_ids[(size_t)cell * W.K + (idx - base)] : -1;
         }
         if constexpr (MODEL == 0) {
           double2 xj[4];
@@ -731,19 +872,16 @@ struct MemRow {
       return;
     }
     // a cell holding more agents than slots: members found by scanning
-    // cell_of, the row's candidates numbered cell by cell
-    const int nn[3] = {n0, n1, n2};
-    int idx = 0;
+    // cell_of, the candidates numbered cell by cell
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const int cnt = nn[k];
-      const int s0 = max(0, b0 - idx), s1 = min(cnt, b1 - idx);
-      idx += cnt;
+    for (int k = 0; k < 9; ++k) {
+      const int cnt = n[k];
+      const int s0 = max(0, b0 - pre[k]), s1 = min(cnt, b1 - pre[k]);
       if (s0 >= s1) continue;
-      const int gc = g0 + k;
+      const int gc = gr[k / 3] + k % 3;
       if (cnt <= W.K) {
         const int *ids = W.ell_ids + (size_t)gc * W.K;
-        for (int s = s0; s < s1; ++s) visit(self, ids[s], fn);
+        for (int s2 = s0; s2 < s1; ++s2) visit(self, ids[s2], fn);
       } else {
         int r = 0;
         for (int jx = a0; jx < a1 && r < s1; ++jx) {
@@ -756,13 +894,23 @@ struct MemRow {
   }
 };
 
-// Stages the scenario's first kWallTile wall segments in LDS (before the
-// per-agent work, so the loads overlap it).
+// Stages cnt wall segments and their frames (seg_frame) in LDS.
+__device__ __forceinline__ void stage_segs(const double4 *src, int cnt, double4 *sseg,
+                                           double4 *sfrm) {
+  for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
+    const double4 s = src[k];
+    sseg[k] = s;
+    sfrm[k] = seg_frame(s.x, s.y, s.z, s.w);
+  }
+}
+
+// Stages the scenario's first kWallTile wall segments (before the per-agent
+// work, so the loads overlap it).  sseg holds 2 kWallTile entries: the
+// segments, then their frames.
 __device__ __forceinline__ void stage_walls(const FusedArgs &fa, const ScnDesc &d, double4 *sseg,
                                             bool fused) {
   if (!fused || !(fa.flags & CD_USE_OBSTACLE)) return;
-  const int cnt = min(kWallTile, d.w1 - d.w0);
-  for (int k = threadIdx.x; k < cnt; k += blockDim.x) sseg[k] = fa.segs[d.w0 + k];
+  stage_segs(fa.segs + d.w0, min(kWallTile, d.w1 - d.w0), sseg, sseg + kWallTile);
 }
 
 __device__ __forceinline__ Fix128 shfl_xor(const Fix128 &v, int mask) {
@@ -773,60 +921,68 @@ __device__ __forceinline__ Fix128 shfl_xor(const Fix128 &v, int mask) {
   return t;
 }
 
-// All force terms of agent i, served by L lanes (L = 4 m, m = 1, 2 or 4;
-// q = lane in the group): lanes 0 .. 3m-1 take part q / 3 of stencil row
-// q % 3 of the pairs, lane 3m the agent's own terms (fluctuation, navigation,
-// orientation, adjusting), and all L lanes share the walls.  Pair and wall
-// terms are summed exactly, so results do not depend on L.  Every lane of
-// the block calls this (have = false for idle groups), so the wall staging
+// Agent-side inputs of the force pass, loaded together before any
+// neighbour is visited.
+template <typename T>
+struct Self {
+  Owner<T> o;
+  WallSelf<T> w;
+  V2<T> f;    // force after the step's own terms
+  double tq;  // torque after them
+};
+
+template <typename T, int MODEL>
+__device__ __forceinline__ Self<T> self_of(const Agents<T> &A, int i, double2 pos) {
+  Self<T> s;
+  s.o = owner_of(A, i);
+  s.w = wall_self(A, MODEL, i, pos);
+  s.f = A.force[i];
+  s.tq = MODEL ? A.torque[i] : 0.0;
+  return s;
+}
+
+// Pair and wall forces of agent i, served by L lanes (L = 4, 8 or 16; q =
+// lane in the group): lane q takes part q of the stencil's candidates and
+// every L-th wall from q.  Pair and wall terms are summed exactly, so
+// results do not depend on L.  Every lane of the
+// block calls this (have = false for idle groups), so the wall staging
 // barriers and the shuffles are uniform; the first kWallTile segments are
-// already in sseg.
+// already staged.  force (torque) = self.f (self.tq) + (pairs + walls): the
+// pair and wall terms are one exact sum, rounded once.
 template <typename T, int MODEL, bool FUSED, class Rows>
 __device__ __forceinline__ void agent_forces(const Agents<T> &A, const FusedArgs &fa,
-                                             const ScnDesc &d, uint64_t step, double4 *sseg,
-                                             int L, bool have, int i, double2 pos, V2<T> vel,
-                                             const TCView &me, const Rows &rows) {
+                                             const ScnDesc &d, double4 *sseg, int L, bool have,
+                                             int i, double2 pos, V2<T> vel, const TCView &me,
+                                             const Self<T> &self, const Rows &rows) {
   const int q = threadIdx.x & (L - 1);
-  const int m = L >> 2;
-  V2<T> f;
-  f.x = T(0);
-  f.y = T(0);
-  double tq = 0.0;
-  if (have && q == 3 * m) {
-    f = A.force[i];
-    if (MODEL) tq = A.torque[i];
-    if (FUSED) own_terms<T, MODEL>(A, fa, d.seed, step, i, i - d.a0, pos, vel, f, tq);
-  }
   PairSum ps;  // pairs
-  if (have && q < 3 * m) {
-    const Owner<T> o = owner_of(A, i);
-    const double ri = A.radius[i];
+  if (have) {
     if constexpr (MODEL == 0) {
-      rows.each(q % 3, q / 3, m, i, [&](int, double2 xj, V2<T> vj, double rj) {
-        pair_add_c<T>(pos, vel, ri, o, xj, vj, rj, ps);
+      rows.each(q, L, i, [&](int, double2 xj, V2<T> vj, double rj) {
+        pair_add_c<T>(pos, vel, self.w.r, self.o, xj, vj, rj, ps);
       });
     } else {
-      rows.each(q % 3, q / 3, m, i, [&](int j, const TCView &other) {
-        if constexpr (sizeof(T) == 8) pair_add_tc(i, j, me, other, o, ps);
+      rows.each(q, L, i, [&](int j, const TCView &other) {
+        if constexpr (sizeof(T) == 8) pair_add_tc(i, j, me, other, self.o, ps);
       });
     }
   }
-  PairSum ws;  // walls
   const bool walls = FUSED && (fa.flags & CD_USE_OBSTACLE);
-  if (walls) {
+  if (walls) {  // wall contacts join the same exact sums
     auto add = [&](T cx, T cy, double mm) {
-      ws.x.add((double)cx);
-      ws.y.add((double)cy);
-      if (MODEL) ws.m.add(mm);
+      ps.x.add((double)cx);
+      ps.y.add((double)cy);
+      if (MODEL) ps.m.add(mm);
     };
     const int nw = d.w1 - d.w0;
-    if (have) walls_each(A, MODEL, i, pos, vel, sseg, min(kWallTile, nw), add, q, L);
+    const double4 *sfrm = sseg + kWallTile;
+    if (have) walls_each(self.w, MODEL, pos, vel, sseg, sfrm, min(kWallTile, nw), add, q, L);
     for (int w0 = kWallTile; w0 < nw; w0 += kWallTile) {  // more segments than staged
       const int cnt = min(kWallTile, nw - w0);
       __syncthreads();
-      for (int k = threadIdx.x; k < cnt; k += blockDim.x) sseg[k] = fa.segs[d.w0 + w0 + k];
+      stage_segs(fa.segs + d.w0 + w0, cnt, sseg, sseg + kWallTile);
       __syncthreads();
-      if (have) walls_each(A, MODEL, i, pos, vel, sseg, cnt, add, q, L);
+      if (have) walls_each(self.w, MODEL, pos, vel, sseg, sfrm, cnt, add, q, L);
     }
     if (nw > kWallTile) {  // restore the first tile for the next agents
       __syncthreads();
@@ -839,36 +995,28 @@ __device__ __forceinline__ void agent_forces(const Agents<T> &A, const FusedArgs
     ps.x.add(shfl_xor(ps.x, mask));
     ps.y.add(shfl_xor(ps.y, mask));
     if (MODEL) ps.m.add(shfl_xor(ps.m, mask));
-    if (walls) {
-      ws.x.add(shfl_xor(ws.x, mask));
-      ws.y.add(shfl_xor(ws.y, mask));
-      if (MODEL) ws.m.add(shfl_xor(ws.m, mask));
-    }
   }
-  if (!have || q != 3 * m) return;
+  if (!have || q != 0) return;
+  V2<T> f = self.f;
+  double tq = self.tq;
   f.x += (T)ps.x.value();
   f.y += (T)ps.y.value();
   if (MODEL) tq += ps.m.value();
-  if (walls) {
-    f.x += (T)ws.x.value();
-    f.y += (T)ws.y.value();
-    if (MODEL) tq += ws.m.value();
-  }
   A.force[i] = f;
   if (MODEL) A.torque[i] = tq;
 }
 
 // Direct kernel: blocks of blockDim / L agents of one scenario, in agent
 // index order, L lanes each.  FUSED = false: the AgentAgentInteractions
-// node, force (torque) += pairs.  FUSED = true: all per-agent forces of one
-// step in the post-order of the standard logic tree (fluctuation,
-// navigation, orientation, adjusting, pairs, walls), added to force
-// (torque).  Nothing runs when the scenario's binning is invalid (the
-// integrator then adds the per-agent terms itself).
+// node, force (torque) += pairs.  FUSED = true: the pair and wall forces of
+// one fused step (after the own terms the step's first kernel added), in
+// the post-order of the standard logic tree.  Nothing runs when the
+// scenario's binning is invalid (the integrator then adds the walls
+// itself).
 template <typename T, int MODEL, bool FUSED>
 __global__ void __launch_bounds__(256) k_forces_direct(Agents<T> A, Work W, Batch B,
                                                        FusedArgs fa, int L) {
-  __shared__ double4 sseg[kWallTile];
+  __shared__ double4 sseg[2 * kWallTile];
   const int2 sk = block_e(B);
   const ScnDesc d = desc_of(B, sk.x);
   const ScnState *ss = &B.sst[sk.x];
@@ -886,17 +1034,19 @@ __global__ void __launch_bounds__(256) k_forces_direct(Agents<T> A, Work W, Batc
   vel.y = T(0);
   int c = d.c0;
   TCView me;
+  Self<T> self;
   if (have) {
     pos = A.pos[i];
     vel = A.vel[i];
     c = W.cell_of[i];
     if constexpr (MODEL == 1) me = tc_agent(A, i);
+    self = self_of<T, MODEL>(A, i, pos);
   }
   const int nx = ss->nx, ny = ss->ny;
   const int lc = c - d.c0, lx = lc / ny, ly = lc - lx * ny;
   const MemRow<T, MODEL> rows{A, W, d.c0, nx, ny, lx, ly, d.a0, d.a1};
   __syncthreads();  // walls staged
-  agent_forces<T, MODEL, FUSED>(A, fa, d, W.st->step - 1, sseg, L, have, i, pos, vel, me, rows);
+  agent_forces<T, MODEL, FUSED>(A, fa, d, sseg, L, have, i, pos, vel, me, self, rows);
 }
 
 struct TileArgs {
@@ -908,7 +1058,7 @@ struct TileArgs {
 // more agents than the LDS capacity.
 template <typename T, int MODEL, bool FUSED>
 __device__ bool tile_region(const Agents<T> &A, const Work &W, const FusedArgs &fa,
-                            const ScnDesc &d, int nx, int ny, uint64_t step, TileLds<T, MODEL> &S,
+                            const ScnDesc &d, int nx, int ny, TileLds<T, MODEL> &S,
                             double4 *sseg, int cx0, int cy0, int w, int h) {
   constexpr int CAP = TileLds<T, MODEL>::CAP;
   const int tid = threadIdx.x;
@@ -1007,8 +1157,10 @@ __device__ bool tile_region(const Agents<T> &A, const Work &W, const FusedArgs &
     const LdsRow<T, MODEL> rows{S, RH, kx, k - kx * RH};
     TCView me;
     if constexpr (MODEL == 1) me = tc_lds<T, MODEL>(S, e);
-    agent_forces<T, MODEL, FUSED>(A, fa, d, step, sseg, 4, have, S.id[e], S.pos[e], S.vel[e],
-                                  me, rows);
+    Self<T> self;
+    if (have) self = self_of<T, MODEL>(A, S.id[e], S.pos[e]);
+    agent_forces<T, MODEL, FUSED>(A, fa, d, sseg, 4, have, S.id[e], S.pos[e], S.vel[e], me, self,
+                                  rows);
   }
   __syncthreads();
   return true;
@@ -1017,8 +1169,8 @@ __device__ bool tile_region(const Agents<T> &A, const Work &W, const FusedArgs &
 // Serves the agents of grid cell (lx, ly) from memory (exact slow path).
 template <typename T, int MODEL, bool FUSED>
 __device__ void cell_from_memory(const Agents<T> &A, const Work &W, const FusedArgs &fa,
-                                 const ScnDesc &d, int nx, int ny, uint64_t step, double4 *sseg,
-                                 int lx, int ly) {
+                                 const ScnDesc &d, int nx, int ny, double4 *sseg, int lx,
+                                 int ly) {
   const int gc = d.c0 + lx * ny + ly;
   const MemRow<T, MODEL> rows{A, W, d.c0, nx, ny, lx, ly, d.a0, d.a1};
   for (int j0 = d.a0; j0 < d.a1; j0 += kTileGroups) {
@@ -1029,12 +1181,14 @@ __device__ void cell_from_memory(const Agents<T> &A, const Work &W, const FusedA
     vel.x = T(0);
     vel.y = T(0);
     TCView me;
+    Self<T> self;
     if (have) {
       pos = A.pos[i];
       vel = A.vel[i];
       if constexpr (MODEL == 1) me = tc_agent(A, i);
+      self = self_of<T, MODEL>(A, i, pos);
     }
-    agent_forces<T, MODEL, FUSED>(A, fa, d, step, sseg, 4, have, i, pos, vel, me, rows);
+    agent_forces<T, MODEL, FUSED>(A, fa, d, sseg, 4, have, i, pos, vel, me, self, rows);
   }
 }
 
@@ -1044,7 +1198,7 @@ template <typename T, int MODEL, bool FUSED>
 __global__ void __launch_bounds__(kTileThreads) k_forces_tiled(Agents<T> A, Work W, Batch B,
                                                                 FusedArgs fa, TileArgs ta) {
   __shared__ TileLds<T, MODEL> S;
-  __shared__ double4 sseg[kWallTile];
+  __shared__ double4 sseg[2 * kWallTile];
   const int2 sk = B.map_t ? B.map_t[blockIdx.x] : make_int2(0, (int)blockIdx.x);
   const ScnDesc d = desc_of(B, sk.x);
   const int nb = B.map_t ? d.nt : (int)gridDim.x;
@@ -1059,16 +1213,15 @@ __global__ void __launch_bounds__(kTileThreads) k_forces_tiled(Agents<T> A, Work
   if (sk.y >= ntiles) return;
   stage_walls(fa, d, sseg, FUSED);
   __syncthreads();
-  const uint64_t step = W.st->step - 1;  // counted by K_a of this step
   for (int t = sk.y; t < ntiles; t += nb) {
     const int cx0 = (t / nty) * ta.tx, cy0 = (t % nty) * ta.ty;
     const int w = min(ta.tx, nx - cx0), h = min(ta.ty, ny - cy0);
-    if (tile_region<T, MODEL, FUSED>(A, W, fa, d, nx, ny, step, S, sseg, cx0, cy0, w, h))
+    if (tile_region<T, MODEL, FUSED>(A, W, fa, d, nx, ny, S, sseg, cx0, cy0, w, h))
       continue;
     for (int c = 0; c < w * h; ++c) {
       const int lx = cx0 + c / h, ly = cy0 + c % h;
-      if (!tile_region<T, MODEL, FUSED>(A, W, fa, d, nx, ny, step, S, sseg, lx, ly, 1, 1))
-        cell_from_memory<T, MODEL, FUSED>(A, W, fa, d, nx, ny, step, sseg, lx, ly);
+      if (!tile_region<T, MODEL, FUSED>(A, W, fa, d, nx, ny, S, sseg, lx, ly, 1, 1))
+        cell_from_memory<T, MODEL, FUSED>(A, W, fa, d, nx, ny, sseg, lx, ly);
     }
   }
 }
@@ -1164,10 +1317,10 @@ __global__ void __launch_bounds__(256) k_integrate_node(Agents<T> A, int model, 
 
 // Fused step: dt from the values K_a reduced (scenario state).  When the
 // force kernel did not serve the scenario (pairs off, or binning invalid),
-// the per-agent terms and walls are added here first, in the same order.
-// Then Verlet and Reset of the scenario's agents (grid-stride over its
-// blocks), partials of the new state for the next step, and zeroing of this
-// step's cell counts.
+// the walls are added here first.  Then Verlet and Reset of the scenario's
+// agents (grid-stride over its blocks), partials of the new state for the
+// next step, and zeroing of this step's cell counts.  The step counter
+// advances here, the step's last kernel.
 template <typename T, int MODEL>
 __global__ void __launch_bounds__(256) k_integrate(Agents<T> A, Work W, Batch B, FusedArgs fa,
                                                    int do_reset) {
@@ -1177,28 +1330,25 @@ __global__ void __launch_bounds__(256) k_integrate(Agents<T> A, Work W, Batch B,
   unsigned long long prev[R_COUNT];
   for (int k = 0; k < R_COUNT; ++k) prev[k] = ss->red[k];
   const double dt = adaptive_dt(prev, d.dt_min, d.dt_max);
-  const bool own = !((fa.flags & CD_USE_AGENT_AGENT) && ss->pairs_done);
-  const uint64_t step = W.st->step - 1;  // counted by K_a of this step
+  const bool walls = (fa.flags & CD_USE_OBSTACLE) &&
+                     !((fa.flags & CD_USE_AGENT_AGENT) && ss->pairs_done);
   const int t0 = sk.y * blockDim.x + threadIdx.x, stride = d.nf * blockDim.x;
   const int clear = ss->C_clear;
   for (int c = t0; c < clear; c += stride) W.count[d.c0 + c] = 0;
   Red r;
   r.init();
   for (int i = d.a0 + t0; i < d.a1; i += stride) {
-    if (own) {
+    if (walls) {
       V2<T> f = A.force[i];
       double tq = MODEL ? A.torque[i] : 0.0;
       const double2 pos = A.pos[i];
       const V2<T> vel = A.vel[i];
-      own_terms<T, MODEL>(A, fa, d.seed, step, i, i - d.a0, pos, vel, f, tq);
-      if (fa.flags & CD_USE_OBSTACLE) {
-        T wx = T(0), wy = T(0);
-        double wtq = 0.0;
-        walls_agent(A, MODEL, i, pos, vel, fa.segs + d.w0, d.w1 - d.w0, wx, wy, wtq);
-        f.x += wx;
-        f.y += wy;
-        if (MODEL) tq += wtq;
-      }
+      T wx = T(0), wy = T(0);
+      double wtq = 0.0;
+      walls_agent(A, MODEL, i, pos, vel, fa.segs + d.w0, d.w1 - d.w0, wx, wy, wtq);
+      f.x += wx;
+      f.y += wy;
+      if (MODEL) tq += wtq;
       A.force[i] = f;
       if (MODEL) A.torque[i] = tq;
     }
@@ -1214,6 +1364,7 @@ __global__ void __launch_bounds__(256) k_integrate(Agents<T> A, Work W, Batch B,
     ss->time_tot += dt;
     ss->oob = 0;
   }
+  if (blockIdx.x == 0 && threadIdx.x == 0) W.st->step += 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -1279,6 +1430,22 @@ FusedArgs fused_args(const cd_step_params_t *p) {
   return fa;
 }
 
+// K_a of the fused step.
+template <typename T>
+void launch_step_begin(const cd_agents_t *a, const cd_step_params_t *p, const Work &W,
+                       const Batch &B, const FusedArgs &fa, const Launch &L, hipStream_t s) {
+  const Agents<T> A = typed<T>(a);
+  const int binning = (p->flags & CD_USE_AGENT_AGENT) ? 1 : 0;
+  if (a->model == CD_THREE_CIRCLE) {
+    if constexpr (sizeof(T) == 8)
+      hipLaunchKernelGGL((k_step_begin<T, 1>), dim3(L.blocks_f), dim3(kBS), 0, s, A, W, B, fa,
+                         p->cell_size, binning);
+  } else {
+    hipLaunchKernelGGL((k_step_begin<T, 0>), dim3(L.blocks_f), dim3(kBS), 0, s, A, W, B, fa,
+                       p->cell_size, binning);
+  }
+}
+
 // One step: K_a, K_e (when pairs are on), K_f; events ev[0..3] around them
 // when timed.
 template <typename T>
@@ -1288,8 +1455,7 @@ int step_kernels(const cd_agents_t *a, const cd_step_params_t *p, const Work &W,
   const bool pairs = (p->flags & CD_USE_AGENT_AGENT) != 0;
   const FusedArgs fa = fused_args(p);
   mark(ev, 0, s);
-  hipLaunchKernelGGL(k_bin_insert, dim3(L.blocks_f), dim3(kBS), 0, s, (const double2 *)A.pos,
-                     p->cell_size, W, B, pairs ? 1 : 0, 1);
+  launch_step_begin<T>(a, p, W, B, fa, L, s);
   mark(ev, 1, s);
   if (pairs) {
     const int rc = launch_forces<T>(a, W, B, fa, true, L, w, s);
@@ -1318,7 +1484,7 @@ int do_agent_agent(const cd_agents_t *a, double cell, const cd_workspace_t *w, h
   int rc;
   if ((rc = launch_reduce(A, W, B, L.blocks_f, true, s))) return rc;
   hipLaunchKernelGGL(k_bin_insert, dim3(L.blocks_f), dim3(kBS), 0, s, (const double2 *)A.pos,
-                     cell, W, B, 1, 0);
+                     cell, W, B, 1);
   if ((rc = check_launch())) return rc;
   FusedArgs fa;
   memset(&fa, 0, sizeof(fa));
@@ -1747,11 +1913,12 @@ int cd_kernel_timed(const cd_agents_t *a, const cd_step_params_t *p, const cd_wo
                                          p->seed, p->dt_min, p->dt_max);
   const Launch L = batched ? batch_launch(w) : single_launch(w);
   const FusedArgs fa = fused_args(p);
-  const bool pairs = (p->flags & CD_USE_AGENT_AGENT) != 0;
   const int reset = (p->flags & CD_USE_RESET) ? 1 : 0;
   auto bin = [&]() {
-    hipLaunchKernelGGL(k_bin_insert, dim3(L.blocks_f), dim3(kBS), 0, s, (const double2 *)a->position,
-                       p->cell_size, W, B, pairs ? 1 : 0, 1);
+    if (a->real == CD_F64)
+      launch_step_begin<double>(a, p, W, B, fa, L, s);
+    else
+      launch_step_begin<float>(a, p, W, B, fa, L, s);
   };
   auto integrate = [&]() {
     if (a->model == CD_THREE_CIRCLE)
