@@ -106,26 +106,38 @@ struct Fix128 {
   unsigned long long lo;
   bool bad;
   __host__ __device__ __forceinline__ Fix128() : hi(0), lo(0ull), bad(false) {}
+  // Adds trunc(|x| 2^64) with x's sign: the mantissa shifted into place
+  // (|x| = m 2^(e - 1075) for biased exponent e, so |x| 2^64 = m 2^s with
+  // s = e - 1011; subnormals use e = 1).  |x| >= 2^62, inf and NaN set bad.
   __host__ __device__ __forceinline__ void add(double x) {
-    if (!(fabs(x) < 4611686018427387904.0)) {  // 2^62
+    const unsigned long long bits = __builtin_bit_cast(unsigned long long, x);
+    const int e = (int)((bits >> 52) & 0x7ffull);
+    if (e >= 1085) {  // 2^62 = 2^(1085 - 1023)
       bad = true;
       return;
     }
-    const double a = fabs(x);  // convert |x| exactly, then negate
-    const double f = floor(a);
-    const double t = (a - f) * 4294967296.0;  // fraction * 2^32, exact
-    const double th = floor(t);
-    unsigned long long l = ((unsigned long long)(unsigned int)th << 32) |
-                           (unsigned long long)(unsigned int)((t - th) * 4294967296.0);
-    const double fh = floor(f * 2.3283064365386962890625e-10);  // f / 2^32
-    long long h = (long long)(unsigned int)fh * 4294967296LL +
-                  (long long)(unsigned int)(f - fh * 4294967296.0);
-    if (x < 0.0) {  // two's complement of (h, l)
-      h = ~h + (l == 0ull ? 1 : 0);
+    const unsigned long long m = (bits & 0xFFFFFFFFFFFFFull) | (e ? (1ull << 52) : 0ull);
+    const int s = (e ? e : 1) - 1011;  // <= 73
+    unsigned long long l, h;
+    if (s >= 64) {
+      l = 0ull;
+      h = m << (s - 64);
+    } else if (s > 0) {
+      l = m << s;
+      h = m >> (64 - s);
+    } else if (s > -64) {
+      l = m >> (-s);
+      h = 0ull;
+    } else {
+      l = 0ull;
+      h = 0ull;
+    }
+    if (bits >> 63) {  // two's complement of (h, l)
+      h = ~h + (l == 0ull ? 1ull : 0ull);
       l = ~l + 1ull;
     }
     lo += l;
-    hi += h + (lo < l ? 1 : 0);
+    hi += (long long)h + (lo < l ? 1 : 0);
   }
   __host__ __device__ __forceinline__ void add(const Fix128 &o) {
     lo += o.lo;

@@ -233,40 +233,74 @@ __global__ void __launch_bounds__(256) k_fluct(Agents<T> A, int model, uint64_t 
   if (model) A.torque[i] += m;
 }
 
-// Navigation lookup; returns false when e0 is kept.
-__device__ __forceinline__ bool nav_lookup(double px, double py, int64_t t, const double *maps,
-                                           int64_t T_, int64_t rows, int64_t cols, double minx,
-                                           double miny, double step, int mode, double &ex,
-                                           double &ey) {
-  if (t < 0 || t >= T_) return false;
+// Navigation lookup in two halves: nav_fetch checks the cell and issues the
+// map loads, nav_finish turns them into the direction (returns false when
+// e0 is kept), so a kernel can put independent work between the two.
+struct NavFetch {
+  bool ok;
+  int mode;
+  double fx, fy, step;
+  double p00, p01, p10, p11;  // potential (mode 1) or the direction in p00, p01 (mode 0)
+};
+
+__device__ __forceinline__ NavFetch nav_fetch(double px, double py, int64_t t, const double *maps,
+                                              int64_t T_, int64_t rows, int64_t cols,
+                                              double minx, double miny, double step, int mode) {
+  NavFetch n;
+  n.ok = false;
+  n.mode = mode;
+  n.step = step;
+  n.fx = n.fy = n.p00 = n.p01 = n.p10 = n.p11 = 0.0;
+  if (t < 0 || t >= T_) return n;
   const double qx = (px - minx) / step;
   const double qy = (py - miny) / step;
   if (mode == 0) {
     // indicer truncation + is_inside (quickest_path.py:41-44, navigation.py:59-78)
-    if (!(qx > -1.0 && qx < (double)cols && qy > -1.0 && qy < (double)rows)) return false;
+    if (!(qx > -1.0 && qx < (double)cols && qy > -1.0 && qy < (double)rows)) return n;
     const int64_t jj = (int64_t)qx, ii = (int64_t)qy;
     const double2 m = ((const double2 *)maps)[(t * rows + ii) * cols + jj];
-    ex = m.x;
-    ey = m.y;
-    return true;
+    n.p00 = m.x;
+    n.p01 = m.y;
+    n.ok = true;
+    return n;
   }
-  if (!(qx >= 0.0 && qx <= (double)(cols - 1) && qy >= 0.0 && qy <= (double)(rows - 1)))
-    return false;
+  if (!(qx >= 0.0 && qx <= (double)(cols - 1) && qy >= 0.0 && qy <= (double)(rows - 1))) return n;
   int64_t c0 = (int64_t)qx, r0 = (int64_t)qy;
   if (c0 > cols - 2) c0 = cols - 2;
   if (r0 > rows - 2) r0 = rows - 2;
-  if (c0 < 0 || r0 < 0) return false;
-  const double fx = qx - (double)c0, fy = qy - (double)r0;
+  if (c0 < 0 || r0 < 0) return n;
+  n.fx = qx - (double)c0;
+  n.fy = qy - (double)r0;
   const double *P = maps + t * rows * cols;
-  const double p00 = P[r0 * cols + c0], p01 = P[r0 * cols + c0 + 1];
-  const double p10 = P[(r0 + 1) * cols + c0], p11 = P[(r0 + 1) * cols + c0 + 1];
-  const double gx = ((p01 - p00) * (1.0 - fy) + (p11 - p10) * fy) / step;
-  const double gy = ((p10 - p00) * (1.0 - fx) + (p11 - p01) * fx) / step;
+  n.p00 = P[r0 * cols + c0];
+  n.p01 = P[r0 * cols + c0 + 1];
+  n.p10 = P[(r0 + 1) * cols + c0];
+  n.p11 = P[(r0 + 1) * cols + c0 + 1];
+  n.ok = true;
+  return n;
+}
+
+__device__ __forceinline__ bool nav_finish(const NavFetch &n, double &ex, double &ey) {
+  if (!n.ok) return false;
+  if (n.mode == 0) {
+    ex = n.p00;
+    ey = n.p01;
+    return true;
+  }
+  const double gx = ((n.p01 - n.p00) * (1.0 - n.fy) + (n.p11 - n.p10) * n.fy) / n.step;
+  const double gy = ((n.p10 - n.p00) * (1.0 - n.fx) + (n.p11 - n.p01) * n.fx) / n.step;
   const double g = hypot(gx, gy);
   if (!(g > 0.0)) return false;
   ex = -gx / g;
   ey = -gy / g;
   return true;
+}
+
+__device__ __forceinline__ bool nav_lookup(double px, double py, int64_t t, const double *maps,
+                                           int64_t T_, int64_t rows, int64_t cols, double minx,
+                                           double miny, double step, int mode, double &ex,
+                                           double &ey) {
+  return nav_finish(nav_fetch(px, py, t, maps, T_, rows, cols, minx, miny, step, mode), ex, ey);
 }
 
 template <typename T>
@@ -633,14 +667,15 @@ __device__ __forceinline__ void pair_add_tc(int i, int j, const TCView &me, cons
 
 // Own (per-agent) terms of the fused step in the reference's post-order:
 // fluctuation, navigation (e0), orientation (phi0), adjusting; force
-// (torque) += fluctuation + adjusting.  own_load issues every load they need
-// (including the navigation map's), own_apply computes and stores, so a
-// kernel can put independent work between the two.
+// (torque) += fluctuation + adjusting.  own_load issues the agent's loads,
+// own_nav the navigation map's (it needs the position and target), and
+// own_apply computes and stores, so a kernel can put independent work
+// between them; own_apply computes the noise before it first uses the map.
 template <typename T>
 struct OwnIn {
   V2<T> f, vel, e;
-  double tq, mass, sig_f, inertia, sig_t, tau_adj, v0, ex, ey;
-  bool nav;
+  double tq, mass, sig_f, inertia, sig_t, tau_adj, v0;
+  int64_t target;
 };
 
 template <typename T, int MODEL>
@@ -651,7 +686,7 @@ __device__ __forceinline__ OwnIn<T> own_load(const Agents<T> &A, const FusedArgs
   o.tq = MODEL ? A.torque[i] : 0.0;
   o.vel = A.vel[i];
   o.mass = A.mass[i];
-  o.sig_f = o.inertia = o.sig_t = o.tau_adj = o.v0 = o.ex = o.ey = 0.0;
+  o.sig_f = o.inertia = o.sig_t = o.tau_adj = o.v0 = 0.0;
   if (fa.flags & CD_USE_FLUCT) {
     o.sig_f = A.sig_f[i];
     if (MODEL) {
@@ -660,9 +695,7 @@ __device__ __forceinline__ OwnIn<T> own_load(const Agents<T> &A, const FusedArgs
     }
   }
   o.e = A.e0[i];
-  o.nav = (fa.flags & CD_USE_NAV) &&
-          nav_lookup(pos.x, pos.y, A.target[i], fa.maps, fa.nav_T, fa.nav_rows, fa.nav_cols,
-                     fa.nav_minx, fa.nav_miny, fa.nav_step, fa.nav_mode, o.ex, o.ey);
+  o.target = (fa.flags & CD_USE_NAV) ? A.target[i] : -1;
   if (fa.flags & CD_USE_ADJUST) {
     o.tau_adj = A.tau_adj[i];
     o.v0 = A.v0[i];
@@ -670,10 +703,17 @@ __device__ __forceinline__ OwnIn<T> own_load(const Agents<T> &A, const FusedArgs
   return o;
 }
 
+template <typename T>
+__device__ __forceinline__ NavFetch own_nav(const FusedArgs &fa, double2 pos, const OwnIn<T> &o) {
+  return nav_fetch(pos.x, pos.y, o.target, fa.maps, fa.nav_T, fa.nav_rows, fa.nav_cols,
+                   fa.nav_minx, fa.nav_miny, fa.nav_step, fa.nav_mode);
+}
+
 // `local` is the agent's index within its scenario (the noise counter).
 template <typename T, int MODEL>
 __device__ __forceinline__ void own_apply(const Agents<T> &A, const FusedArgs &fa, uint64_t seed,
-                                          uint64_t step, int i, int local, const OwnIn<T> &o) {
+                                          uint64_t step, int i, int local, const OwnIn<T> &o,
+                                          const NavFetch &nf) {
   V2<T> f = o.f;
   double tq = o.tq;
   if (fa.flags & CD_USE_FLUCT) {
@@ -684,9 +724,10 @@ __device__ __forceinline__ void own_apply(const Agents<T> &A, const FusedArgs &f
     if (MODEL) tq += nm;
   }
   V2<T> e = o.e;
-  if (o.nav) {
-    e.x = (T)o.ex;
-    e.y = (T)o.ey;
+  double ex, ey;
+  if ((fa.flags & CD_USE_NAV) && nav_finish(nf, ex, ey)) {
+    e.x = (T)ex;
+    e.y = (T)ey;
     A.e0[i] = e;
   }
   if (fa.flags & CD_USE_ADJUST) {
@@ -705,9 +746,9 @@ __device__ __forceinline__ void own_apply(const Agents<T> &A, const FusedArgs &f
 // K_a of the fused step: per scenario, combine the previous step's partials
 // and publish the grid; per agent, bin it (bin_scenario's contract) and add
 // the step's own terms.  The own terms need neither grid nor neighbours:
-// the first agent's inputs (and its navigation map entry) are loaded before
-// the partials are reduced, and its noise is computed while its slot atomic
-// is in flight.  The step's index is the counter's value: no other kernel
+// the first agent's inputs are loaded before the partials are reduced, and
+// its noise is computed while its slot atomic and navigation map loads are
+// in flight.  The step's index is the counter's value: no other kernel
 // of the step has advanced it yet (K_f does, last).
 template <typename T, int MODEL>
 __global__ void __launch_bounds__(256) k_step_begin(Agents<T> A, Work W, Batch B, FusedArgs fa,
@@ -742,7 +783,10 @@ __global__ void __launch_bounds__(256) k_step_begin(Agents<T> A, Work W, Batch B
         k = atomicAdd(&W.count[c], 1);
       }
     }
-    own_apply<T, MODEL>(A, fa, d.seed, step, i, i - d.a0, o);
+    NavFetch nf;
+    nf.ok = false;
+    if (fa.flags & CD_USE_NAV) nf = own_nav(fa, pos, o);
+    own_apply<T, MODEL>(A, fa, d.seed, step, i, i - d.a0, o, nf);
     if (c >= 0) {
       if (k < W.K)
         W.ell_ids[c * W.K + k] = i;
@@ -838,16 +882,19 @@ struct MemRow {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           const int idx = b + u;
-          // the cell holding candidate idx: the last k with pre[k] <= idx
-          int base = 0, cell = gr[0];
+          j[u] = -1;
+          if (idx < b1) {
+            // the cell holding candidate idx: the last k with pre[k] <= idx
+            int base = 0, cell = gr[0];
 #pragma unroll
-          for (int k = 1; k < 9; ++k) {
-            if (idx >= pre[k]) {
-              base = pre[k];
-              cell = gr[k / 3] + k % 3;
+            for (int k = 1; k < 9; ++k) {
+              if (idx >= pre[k]) {
+                base = pre[k];
+                cell = gr[k / 3] + k % 3;
+              }
             }
+            j[u] = W.ell_ids[(size_t)cell * W.K + (idx - base)];
           }
-          j[u] = idx < b1 ? W.ell_ids[(size_t)cell * W.K + (idx - base)] : -1;
         }
         if constexpr (MODEL == 0) {
           double2 xj[4];
@@ -855,10 +902,11 @@ struct MemRow {
           double rj[4];
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
-            const int jj = j[u] < 0 ? self : j[u];
-            xj[u] = A.pos[jj];
-            vj[u] = A.vel[jj];
-            rj[u] = A.radius[jj];
+            if (j[u] >= 0) {
+              xj[u] = A.pos[j[u]];
+              vj[u] = A.vel[j[u]];
+              rj[u] = A.radius[j[u]];
+            }
           }
 #pragma unroll
           for (int u = 0; u < 4; ++u)

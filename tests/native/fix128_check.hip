@@ -10,10 +10,49 @@
 
 #include "../../crowddynamics_amd/csrc/cd_device.h"
 
+// trunc(|x| 2^64) with x's sign as a 128-bit two's-complement pair, by
+// exact floating-point splitting (independent of the bit-level conversion
+// of Fix128::add): |x| = f + r, f integer, r in [0, 1).
+static void split_ref(double x, long long &hi, unsigned long long &lo) {
+  const double a = std::fabs(x), f = std::floor(a);
+  const double t = (a - f) * 4294967296.0, th = std::floor(t);
+  unsigned long long l = ((unsigned long long)(unsigned int)th << 32) |
+                         (unsigned long long)(unsigned int)((t - th) * 4294967296.0);
+  const double fh = std::floor(f * 2.3283064365386962890625e-10);
+  long long h = (long long)(unsigned int)fh * 4294967296LL + (long long)(unsigned int)(f - fh * 4294967296.0);
+  if (x < 0.0) {
+    h = ~h + (l == 0ull ? 1 : 0);
+    l = ~l + 1ull;
+  }
+  hi = h;
+  lo = l;
+}
+
 int main() {
   std::mt19937_64 g(1);
   std::uniform_real_distribution<double> u(-1.0, 1.0);
   int fails = 0;
+  // the conversion of one term equals the floor-based split over the whole
+  // exponent range below 2^62 (subnormals, tiny, negative, integers)
+  for (int k = 0; k < 400000; ++k) {
+    double x;
+    if (k < 2000) {
+      const double special[] = {0.0, -0.0, 1.0, -1.0, 0.5, 4294967296.0, -4294967296.0,
+                                std::ldexp(1.0, -64), std::ldexp(1.0, -65), std::ldexp(-1.0, 61),
+                                std::ldexp(1.0, 62) * (1.0 - std::ldexp(1.0, -53)), 4.9e-324};
+      x = special[k % 12];
+    } else {
+      x = u(g) * std::ldexp(1.0, (int)(g() % 1140) - 1075);
+    }
+    if (!(std::fabs(x) < std::ldexp(1.0, 62))) continue;
+    cd::Fix128 a;
+    a.add(x);
+    long long h;
+    unsigned long long l;
+    split_ref(x, h, l);
+    if ((a.hi != h || a.lo != l) && fails++ < 5)
+      std::printf("conversion %.17g: %lld %llu vs %lld %llu\n", x, a.hi, a.lo, h, l);
+  }
   for (int k = 0; k < 200000; ++k) {  // |x| in [2^-11, 2^50]: the conversion is exact
     const double x = u(g) * std::ldexp(1.0, (int)(g() % 61) - 10);
     if (std::fabs(x) < std::ldexp(1.0, -11)) continue;
