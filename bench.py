@@ -317,6 +317,8 @@ def main():
     ap.add_argument('--no-corridor', action='store_true')
     ap.add_argument('--no-adjoint', action='store_true')
     ap.add_argument('--adjoint-steps', type=int, default=500)
+    ap.add_argument('--lanes', type=int, default=0, choices=(0, 4, 8, 16),
+                    help='lanes per agent of the room force kernel (0: autotuned)')
     ap.add_argument('--no-tune', action='store_true',
                     help='skip the autotuners (room: agent-parallel force kernel with 8 '
                          'lanes per agent; ensemble: 4 lanes and the smallest '
@@ -357,8 +359,8 @@ def main():
                        cell_size=scn.cell_size, dt_min=scn.dt_min, dt_max=scn.dt_max,
                        seed=scn.seed, precision=prec, bounds=scn.bounds, tile=tile, lanes=lanes)
 
-    kern, kern_table = (0, 0, 8), []  # agent-parallel, 8 lanes per agent
-    if not args.no_tune:  # lanes per agent of the agent-parallel kernel
+    kern, kern_table = (0, 0, args.lanes or 8), []  # agent-parallel kernel
+    if not args.no_tune and not args.lanes:  # lanes per agent, autotuned
         with torch.cuda.stream(stream):
             kern, kern_table = tune_kernel(lambda t, l: make(args.precision, t, l),
                                            kernels=((0, 0, 4), (0, 0, 8), (0, 0, 16)))
