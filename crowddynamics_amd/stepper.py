@@ -130,9 +130,9 @@ class Stepper:
         return self.ws.state()
 
     def timed(self, steps):
-        """GPU milliseconds of the binning, force and integrate kernels and of
-        the whole step, summed over `steps` steps, from HIP events on the
-        current stream."""
+        """GPU milliseconds of the first kernel (per-agent terms and binning),
+        the force and integrate kernels and the whole step, summed over
+        `steps` steps, from HIP events on the current stream."""
         ms = (C.c_double * 4)()
         _lib.check(_lib.lib().cd_step_timed(self.agents.ref(), C.byref(self.params),
                                             self.ws.ref(), int(steps), ms, stream_handle()),
@@ -141,7 +141,8 @@ class Stepper:
 
     def kernel_timed(self, kernel, reps=20):
         """Average GPU milliseconds of `reps` back-to-back launches of one
-        step kernel ('bin', 'forces' or 'integrate').  Leaves the state
+        step kernel ('bin': the first kernel, per-agent terms and binning;
+        'forces'; 'integrate').  Leaves the state
         meaningless: time a copy, or prepare() again afterwards."""
         k = {'bin': 0, 'forces': 1, 'integrate': 2}[kernel]
         ms = C.c_double()

@@ -6,7 +6,7 @@
  * hipStream_t), performs no host synchronisation and no allocation, and may
  * be captured into a HIP graph, except the calls documented as synchronous
  * (cd_read_state, cd_batch_prepare, cd_batch_read_states, cd_step_timed,
- * cd_graph_*).  Calls return 0 on success or
+ * cd_kernel_timed, cd_graph_*).  Calls return 0 on success or
  * a negative CD_ERR_* code; kernels record data-dependent conditions (grid
  * overflow, non-finite positions, full cell slot lists) as sticky bits in
  * the workspace state, read back by cd_read_state.
@@ -221,9 +221,10 @@ typedef struct cd_step_params {
 /* One MultiAgentSimulation.update() (crowddynamics/simulation/multiagent.py:51-55)
  * of the tree Reset << (Integrator << (Fluctuation, Adjusting << (Navigation,
  * Orientation), AgentAgentInteractions, AgentObstacleInteractions))
- * (crowddynamics/examples/simulations.py:123-136) in three kernels (binning,
- * LDS-tiled forces, integration).  The
- * fluctuation step counter and every reduction stay on the device.
+ * (crowddynamics/examples/simulations.py:123-136) in three kernels: the
+ * per-agent terms (fluctuation, navigation, orientation, adjusting) with the
+ * binning, the pair and wall forces, and the integration.  The fluctuation
+ * step counter and every reduction stay on the device.
  * cd_step_prepare must run once before the first cd_step (and after any
  * host-side change of positions or velocities). */
 int cd_step_prepare(const cd_agents_t *agents, const cd_step_params_t *params,
@@ -291,17 +292,16 @@ int cd_adjoint_segment(const cd_agents_t *agents, const cd_step_params_t *params
 
 /* Synchronous: runs `steps` steps (cd_step, or cd_batch_step on a workspace
  * prepared by cd_batch_prepare) and returns in kernel_ms[4] the summed GPU
- * time of the binning kernel, the force kernel, the integrate kernel and the
- * whole step, measured with HIP events recorded on `stream` around each
- * launch.  No reference counterpart (measurement). */
+ * time of the first kernel (per-agent terms and binning), the force kernel,
+ * the integrate kernel and the whole step, measured with HIP events recorded
+ * on `stream` around each launch.  No reference counterpart (measurement). */
 int cd_step_timed(const cd_agents_t *agents, const cd_step_params_t *params,
                   const cd_workspace_t *ws, int64_t steps, double *kernel_ms,
                   cd_stream_t stream);
 
-/* ---- HIP graphs (synchronous helpers) ------------------------------------- */
-
-/* Synchronous measurement of one step kernel (0: binning, 1: forces,
- * 2: integrate): after one binning pass, `reps` back-to-back launches of it
+/* Synchronous measurement of one step kernel (0: the first kernel, per-agent
+ * terms and binning; 1: forces; 2: integrate): after one pass of the first
+ * kernel, `reps` back-to-back launches of it
  * between two HIP events on `stream`; *avg_ms = elapsed / reps.  Repeating
  * a kernel leaves the agent state and workspace meaningless (forces
  * accumulate, agents move, cell counts grow): use a copy, then discard it
@@ -309,6 +309,8 @@ int cd_step_timed(const cd_agents_t *agents, const cd_step_params_t *params,
 int cd_kernel_timed(const cd_agents_t *agents, const cd_step_params_t *params,
                     const cd_workspace_t *ws, int32_t kernel, int64_t reps, double *avg_ms,
                     cd_stream_t stream);
+
+/* ---- HIP graphs (synchronous helpers) ------------------------------------- */
 
 int cd_graph_begin(cd_stream_t stream);
 int cd_graph_end(cd_stream_t stream, void **graph_exec);
