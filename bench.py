@@ -26,8 +26,10 @@ sys.path.insert(0, ROOT)
 METRIC = ('agent-steps/s @20k Circular (GPU fp64/fp32 vs host numba); adjoint step time')
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
-KERNELS = ('step_begin', 'forces', 'integrate', 'step')
-STEP_KERNELS = KERNELS[:3]
+# the step's kernels and their slots in Stepper.timed() (slot 2, a separate
+# integration kernel, is empty: the force kernel integrates)
+KERNELS = {'step_begin': 0, 'forces': 1, 'step': 3}
+STEP_KERNELS = ('step_begin', 'forces')
 
 
 def log(*a):
@@ -61,32 +63,29 @@ def pair_tally(arr, cell, segments):
                 wall_pairs=w_all, wall_contacts=w_con, flops=flops)
 
 
-def step_begin_bytes(n, real_bytes=8):
-    """First kernel of the step (binning and the own terms), per agent.
-    Binning: read the position 16, write the cell index and the slot id
-    (4 + 4), read-modify-write the cell's count 8.  Own terms: read mass,
-    fluctuation sigma, tau_adj, target speed and target 40, the four
-    potential values of the bilinear navigation lookup 32, force, velocity
-    and target direction (2 reals each); write force and target direction
-    (2 reals each).  104 + 10 reals = 184 B/agent in fp64, 144 B in fp32."""
-    return n * (32 + 40 + 32 + 10 * real_bytes)
+def step_begin_bytes(n, cells, real_bytes=8):
+    """First kernel of the step, per agent.  Binning: read the position 16,
+    write the cell index and the slot id (4 + 4), read-modify-write the
+    cell's count 8.  Own terms: read mass, fluctuation sigma, tau_adj,
+    target speed and target 40, the four potential values of the bilinear
+    navigation lookup 32, force, velocity and target direction (2 reals
+    each); write force and target direction (2 reals each).  Snapshot for
+    the force kernel: write position 16 and velocity (2 reals).  120 + 12
+    reals = 216 B/agent in fp64, 168 B in fp32; plus clearing the previous
+    step's cell counts, 4 B per grid cell."""
+    return n * (32 + 40 + 32 + 16 + 12 * real_bytes) + 4 * cells
 
 
 def forces_bytes(n, real_bytes=8):
-    """Force kernel (circular, fused), per agent: read position 16, cell
-    index 4, radius and the six pair and contact parameters 56, its id in a
-    slot list 4, velocity and force (2 reals each); write force (2 reals).
-    80 + 6 reals = 128 B/agent in fp64, 104 B in fp32 (neighbour data is
-    counted once: every agent is read as a neighbour from cache)."""
-    return n * (16 + 4 + 56 + 4 + 6 * real_bytes)
-
-
-def integrate_bytes(n, cells, real_bytes=8):
-    """Integrate kernel: per agent read position 16, velocity, force,
-    force_prev (2 reals each), mass and target speed 16; write position 16,
-    velocity, force_prev and the reset force (2 reals each): 48 + 12 reals
-    = 144 B/agent in fp64; clear the grid's cell counts (4 B per cell)."""
-    return n * (48 + 12 * real_bytes) + 4 * cells
+    """Force kernel with the integrator (circular, fused), per agent: read
+    position 16, cell index 4, radius, mass and the five pair and contact
+    parameters 56, target speed 8, its id in a slot list 4, velocity, force
+    and previous force (2 reals each), and once as a neighbour its snapshot
+    (position 16, velocity 2 reals); write position 16, velocity, previous
+    force and the reset force (2 reals each).  120 + 14 reals = 232 B/agent
+    in fp64, 176 B in fp32 (neighbour data counted once: every agent is read
+    as a neighbour from cache)."""
+    return n * (16 + 4 + 56 + 8 + 4 + 16 + 16 + 14 * real_bytes)
 
 
 def measured_traffic(kernel, threads):
@@ -185,7 +184,7 @@ def bench_ensemble(n_scenarios, steps, loop_steps, stream, tune=True):
     return dict(scenarios=n_scenarios, agents=n, precision='f32 (f64 positions and dt)',
                 steps=reps * G, value=batched, unit='agent-steps/s',
                 ms_per_step=1e3 * el / (reps * G),
-                kernel_ms={k: kms[i] / 20 for i, k in enumerate(KERNELS)},
+                kernel_ms={k: kms[i] / 20 for k, i in KERNELS.items()},
                 autotuned=dict(cell_size=tuned['cell_size'],
                                tile=tuned['tile'], lanes=tuned['lanes'],
                                candidates=[dict(ms=t[0], cell_size=t[1], kernel=t[2])
@@ -223,7 +222,7 @@ def bench_corridor(steps, stream):
     log('corridor: %.3e agent-steps/s' % v)
     return dict(agents=scn.n, steps=reps * G, value=v, unit='agent-steps/s',
                 ms_per_step=1e3 * el / (reps * G), precision='f64',
-                kernel_ms={k: kms[i] / 20 for i, k in enumerate(KERNELS)})
+                kernel_ms={k: kms[i] / 20 for k, i in KERNELS.items()})
 
 
 def bench_adjoint(n, steps, checkpoint):
@@ -388,10 +387,9 @@ def main():
             # (the step's state is discarded afterwards)
             nt = 50
             ms = st.timed(nt)
-            kernel_ms[prec] = {k: ms[i] / nt for i, k in enumerate(KERNELS)}
+            kernel_ms[prec] = {k: ms[i] / nt for k, i in KERNELS.items()}
             b2b = {}
-            for k, name in (('forces', 'forces'), ('integrate', 'integrate'),
-                            ('bin', 'step_begin')):
+            for k, name in (('forces', 'forces'), ('bin', 'step_begin')):
                 st.prepare()
                 b2b[name] = st.kernel_timed(k, 20)
             kernel_b2b[prec] = b2b
@@ -414,20 +412,14 @@ def main():
     per_kernel = {}
     tname = 'double' if main_p == 'f64' else 'float'
     knames = {'step_begin': 'k_step_begin<%s, 0>' % tname,
-              'forces': 'k_forces_direct<%s, 0, true>' % tname,
-              'integrate': 'k_integrate<%s, 0>' % tname}
+              'forces': 'k_forces_direct<%s, 0, true>' % tname}
     # grid threads of each launch (the traffic summary is keyed by them)
-    f_threads = min(-(-scn.n // 256), 128) * 256  # blocks_f: one partial per block
-    grid = {'step_begin': f_threads, 'integrate': f_threads,
-            'forces': -(-scn.n // (256 // kern[2])) * 256}
-    sb = step_begin_bytes(scn.n, rb)
+    f_threads = min(-(-scn.n // 256), 512) * 256  # blocks_f of one scenario
+    grid = {'step_begin': f_threads, 'forces': -(-scn.n // (256 // kern[2])) * 256}
+    sb = step_begin_bytes(scn.n, cells, rb)
     per_kernel['step_begin'] = dict(bound='hbm', unit='GB/s', ms=km['step_begin'], bytes=sb,
                                     achieved=sb / (km['step_begin'] * 1e-3) / 1e9,
                                     peak=HBM_PEAK_GBS)
-    ib = integrate_bytes(scn.n, cells, rb)
-    per_kernel['integrate'] = dict(bound='hbm', unit='GB/s', ms=km['integrate'], bytes=ib,
-                                   achieved=ib / (km['integrate'] * 1e-3) / 1e9,
-                                   peak=HBM_PEAK_GBS)
     # the force kernel's arithmetic intensity (pair tally flops / bytes) is far
     # below the ridge point (78.6 TFLOP/s / 8 TB/s): memory-side roofline
     fb = forces_bytes(scn.n, rb)

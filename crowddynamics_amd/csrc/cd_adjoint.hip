@@ -150,7 +150,7 @@ __global__ void __launch_bounds__(256) k_adjoint(Agents<double> A, Work W, Batch
   const int i = d.a0 + sk.y * ((int)blockDim.x / L) + (int)threadIdx.x / L;
   const int q = threadIdx.x & (L - 1);
   const bool have = i < d.a1;
-  const bool binned = (flags & CD_USE_AGENT_AGENT) && ss->valid && !ss->oob;
+  const bool binned = (flags & CD_USE_AGENT_AGENT) && ss->valid && !ss->oob[0];
   double acc[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) acc[k] = 0.0;

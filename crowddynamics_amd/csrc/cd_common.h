@@ -57,22 +57,28 @@ struct ScnDesc {
   double dt_min, dt_max;
 };
 
-// Per-scenario state (device).
+// Per-scenario state (device).  The fused step keeps two cell-count
+// buffers and two reduction slots, selected by the parity of the step: the
+// step's first kernel fills buffer p and clears buffer 1 - p; its force
+// kernel merges the reduction of the new state into slot 1 - p, which the
+// next step reads.  A slot is kAccSub sub-slots (Work::acc), so that the
+// merging blocks spread their atomics.  The operator paths use buffer 0
+// only.
 struct ScnState {
   unsigned long long red[8];  // last reduced values (R_*), ord-bit form except R_VMAX
   double dt, time_tot;
   double origin[2];
   int nx, ny;
-  int valid;    // grid of the last binning fits the scenario's cells
-  int C;        // cells of the last binning
-  int C_clear;  // cells whose counts the next clearing pass must zero
-  int oob;      // an agent fell outside the grid (stale bounding box)
-  int pairs_done;  // the force kernel served this step's agents (set by K_e, read by K_f)
+  int valid;       // grid of the last binning fits the scenario's cells
+  int C;           // cells of the last binning
+  int C_clear[2];  // cells of count buffer b that its next clearing pass must zero
+  int oob[2];      // an agent fell outside the grid (stale bounding box), by buffer
+  int par;         // count buffer and reduction slot of the current fused step
 };
 
 constexpr int kBS = 256;              // threads per block of per-agent kernels
+constexpr int kAccSub = 256;          // sub-slots of a scenario's reduction slot
 constexpr int kMaxPartsPerScn = 128;  // blocks of the partial-producing kernels per scenario
-static_assert(kMaxPartsPerScn <= kBS, "one partial per thread of a consumer block");
 constexpr int kSingleParts = 512;     // the same for a single (unbatched) scenario
 constexpr int kAgentsPerForceBlock = 64;  // default agents per force block (256 threads)
 constexpr int kMinForceAgents = 4;  // agents per force block: 64 threads / 16 lanes
@@ -92,9 +98,10 @@ inline int64_t max_blocks_e(int64_t n, int64_t S) { return n / kMinForceAgents +
 inline int agent_lanes(const cd_workspace_t *w) { return w->agent_lanes > 0 ? (int)w->agent_lanes : 4; }
 
 struct Layout {
-  int64_t state, scn_desc, scn_state, map_f, map_e, map_t, partials, tiles;
+  int64_t state, scn_desc, scn_state, acc, map_f, map_e, map_t, partials, tiles;
   int64_t cell_of, rank, pidx, count, offset;
   int64_t ell_ids, e_pos, e_vel, e_r, e_ls, e_rs, e_phi, e_rt, e_rs_r, e_rts;
+  int64_t snap_pos, snap_vel, snap_ls, snap_rs, snap_phi;
   int64_t total;
 };
 
@@ -111,6 +118,7 @@ Layout make_layout(const cd_workspace_t *w) {
   L.state = take(256);
   L.scn_desc = take(sizeof(ScnDesc) * S);
   L.scn_state = take(sizeof(ScnState) * S);
+  L.acc = take(8 * 8 * kAccSub * 2 * S);  // [scenario][slot][sub-slot][8]
   L.map_f = take(8 * max_blocks_f(n, S));
   L.map_e = take(8 * max_blocks_e(n, S));
   L.map_t = take(8 * (cap + S + 1));  // at most one tile block per cell, plus one per scenario
@@ -119,7 +127,7 @@ Layout make_layout(const cd_workspace_t *w) {
   L.cell_of = take(4 * n);
   L.rank = take(4 * n);
   L.pidx = take(4 * n);
-  L.count = take(4 * cap);
+  L.count = take(4 * 2 * cap);  // two buffers
   L.offset = take(4 * (cap + 1));
   const int64_t slots = cap * K;
   L.ell_ids = take(4 * slots);
@@ -135,6 +143,17 @@ Layout make_layout(const cd_workspace_t *w) {
     L.e_rts = take(8 * slots);
   } else {
     L.e_ls = L.e_rs = L.e_phi = L.e_rt = L.e_rs_r = L.e_rts = -1;
+  }
+  // the state a fused step starts from, read as neighbour data while the
+  // force kernel integrates
+  L.snap_pos = take(16 * n);
+  L.snap_vel = take(16 * n);
+  if (w->model == CD_THREE_CIRCLE) {
+    L.snap_ls = take(16 * n);
+    L.snap_rs = take(16 * n);
+    L.snap_phi = take(8 * n);
+  } else {
+    L.snap_ls = L.snap_rs = L.snap_phi = -1;
   }
   L.total = o;
   return L;
@@ -193,6 +212,7 @@ Agents<T> typed(const cd_agents_t *a) {
 // Device views of the workspace arrays (passed by value to kernels).
 struct Work {
   DevState *st;
+  unsigned long long *acc;  // reduction slots of the fused step
   unsigned long long *partials, *tiles;
   int *cell_of, *rank, *pidx, *count, *offset;
   int *ell_ids;
@@ -201,6 +221,10 @@ struct Work {
   double *e_r;
   double2 *e_ls, *e_rs;
   double *e_phi, *e_rt, *e_rs_r, *e_rts;
+  double2 *snap_pos;
+  void *snap_vel;
+  double2 *snap_ls, *snap_rs;
+  double *snap_phi;
   int cap, K, ntiles;
 };
 
@@ -219,6 +243,7 @@ Work carve(const cd_workspace_t *w) {
   char *b = (char *)w->ptr;
   Work W;
   W.st = (DevState *)(b + L.state);
+  W.acc = (unsigned long long *)(b + L.acc);
   W.partials = (unsigned long long *)(b + L.partials);
   W.tiles = (unsigned long long *)(b + L.tiles);
   W.cell_of = (int *)(b + L.cell_of);
@@ -236,6 +261,11 @@ Work carve(const cd_workspace_t *w) {
   W.e_rt = L.e_rt >= 0 ? (double *)(b + L.e_rt) : nullptr;
   W.e_rs_r = L.e_rs_r >= 0 ? (double *)(b + L.e_rs_r) : nullptr;
   W.e_rts = L.e_rts >= 0 ? (double *)(b + L.e_rts) : nullptr;
+  W.snap_pos = (double2 *)(b + L.snap_pos);
+  W.snap_vel = (void *)(b + L.snap_vel);
+  W.snap_ls = L.snap_ls >= 0 ? (double2 *)(b + L.snap_ls) : nullptr;
+  W.snap_rs = L.snap_rs >= 0 ? (double2 *)(b + L.snap_rs) : nullptr;
+  W.snap_phi = L.snap_phi >= 0 ? (double *)(b + L.snap_phi) : nullptr;
   W.cap = (int)w->cell_capacity;
   W.K = (int)w->cell_slots;
   W.ntiles = (int)((w->cell_capacity + kScanTile - 1) / kScanTile);
@@ -313,7 +343,7 @@ __global__ void __launch_bounds__(256) k_ws_init(Work W, ScnState *sst, int S) {
   const int stride = gridDim.x * blockDim.x;
   if (t == 0) memset(W.st, 0, sizeof(DevState));
   for (int s = t; s < S; s += stride) memset(&sst[s], 0, sizeof(ScnState));
-  for (int c = t; c < W.cap; c += stride) W.count[c] = 0;
+  for (int c = t; c < 2 * W.cap; c += stride) W.count[c] = 0;
   for (int k = t; k < W.ntiles + 1; k += stride) W.tiles[k] = 0ull;
 }
 
@@ -398,36 +428,92 @@ __device__ __forceinline__ void store_partial(const Work &W, const Red &r) {
 }
 
 // Consumer side: every block combines its scenario's partials.
-// This thread's partial of the scenario (the identity past the scenario's
-// nf <= kMaxPartsPerScn <= blockDim partials): loaded, not yet combined, so
-// the loads stay in flight until block_reduce uses them.
-__device__ __forceinline__ Red load_partial(const Work &W, const ScnDesc &d) {
+__device__ Red reduce_partials(const Work &W, const ScnDesc &d) {
   Red r;
   r.init();
-  if ((int)threadIdx.x < d.nf)
-    for (int k = 0; k < R_COUNT; ++k) r.v[k] = W.partials[(d.f0 + threadIdx.x) * 8 + k];
+  for (int b = threadIdx.x; b < d.nf; b += blockDim.x) r.combine(&W.partials[(d.f0 + b) * 8]);
+  return block_reduce(r);
+}
+
+// Merges a block's reduction into a scenario's slot: exact and independent
+// of order (min / max of ordered bits, or of the non-negative speed bits).
+__device__ __forceinline__ void acc_merge(unsigned long long *acc, const Red &r) {
+  atomicMin(&acc[R_MINX], r.v[R_MINX]);
+  atomicMin(&acc[R_MINY], r.v[R_MINY]);
+  atomicMax(&acc[R_MAXX], r.v[R_MAXX]);
+  atomicMax(&acc[R_MAXY], r.v[R_MAXY]);
+  atomicMax(&acc[R_VMAX], r.v[R_VMAX]);
+  atomicMax(&acc[R_V0MAX], r.v[R_V0MAX]);
+  atomicMax(&acc[R_NONFINITE], r.v[R_NONFINITE]);
+}
+
+__device__ __forceinline__ void acc_reset(unsigned long long *acc) {
+  Red r;
+  r.init();
+  for (int k = 0; k < R_COUNT; ++k) acc[k] = r.v[k];
+}
+
+// Sub-slot `sub` of reduction slot `slot` of scenario `scn`.
+__device__ __forceinline__ unsigned long long *acc_at(const Work &W, int scn, int slot, int sub) {
+  return W.acc + ((size_t)(scn * 2 + slot) * kAccSub + sub) * 8;
+}
+
+// Wave-wide reduction; every lane returns the wave's result.  `first`: the
+// lanes hold the identity except every first-th one (a power of two), so
+// the exchanges below that distance are skipped.
+__device__ __forceinline__ Red wave_reduce(Red r, int first = 1) {
+  for (int o = 32; o >= first; o >>= 1) {
+    unsigned long long u[R_COUNT];
+#pragma unroll
+    for (int k = 0; k < R_COUNT; ++k) u[k] = shfl_xor64(r.v[k], o);
+    r.combine(u);
+  }
   return r;
 }
 
-__device__ Red reduce_partials(const Work &W, const ScnDesc &d) {
-  return block_reduce(load_partial(W, d));
+// The reduction held by a scenario's slot, combined by each wave on its own
+// (no block barrier); every thread returns it.
+__device__ __forceinline__ Red acc_read(const Work &W, int scn, int slot) {
+  Red r;
+  r.init();
+  for (int sub = threadIdx.x & 63; sub < kAccSub; sub += 64) r.combine(acc_at(W, scn, slot, sub));
+  return wave_reduce(r);
 }
 
 __device__ __forceinline__ double red_vmax(const unsigned long long *v) {
   return __longlong_as_double((long long)v[R_VMAX]);
 }
 
-// Partials of the current state (single-scenario operators and
-// cd_step_prepare); also zeroes the counts of the previous binning.
+// adaptive_timestep (core/integrator.py:81-97) from reduced values.
+__device__ __forceinline__ double adaptive_dt(const unsigned long long *v, double dt_min,
+                                              double dt_max) {
+  const double v_max = red_vmax(v);
+  if (v_max == 0.0) return dt_max;
+  const double v0max = from_ord_bits(v[R_V0MAX]);
+  const double dx_max = 1.1 * v0max * dt_max;
+  const double dt = dx_max / v_max;
+  if (dt > dt_max) return dt_max;
+  if (dt < dt_min) return dt_min;
+  return dt;
+}
+
+// Partials of the current state (operators and the step's preparation);
+// also zeroes the counts of the previous binnings (both buffers).  acc_slot
+// >= 0: the reduction is also merged into the scenario's acc[acc_slot]
+// (reset beforehand by k_scn_prepare).
 template <typename T>
-__global__ void __launch_bounds__(256) k_reduce(Agents<T> A, Work W, Batch B, int clear_counts) {
+__global__ void __launch_bounds__(256) k_reduce(Agents<T> A, Work W, Batch B, int clear_counts,
+                                                int acc_slot) {
   const int2 sk = block_f(B);
   const ScnDesc d = desc_of(B, sk.x);
   ScnState *ss = &B.sst[sk.x];
   const int t0 = sk.y * blockDim.x + threadIdx.x, stride = d.nf * blockDim.x;
   if (clear_counts) {
-    const int clear = ss->C_clear;
-    for (int c = t0; c < clear; c += stride) W.count[d.c0 + c] = 0;
+    for (int b = 0; b < 2; ++b) {
+      int *count = W.count + (size_t)b * W.cap + d.c0;
+      const int clear = ss->C_clear[b];
+      for (int c = t0; c < clear; c += stride) count[c] = 0;
+    }
   }
   Red r;
   r.init();
@@ -437,7 +523,23 @@ __global__ void __launch_bounds__(256) k_reduce(Agents<T> A, Work W, Batch B, in
   }
   r = block_reduce(r);
   store_partial(W, r);
-  if (sk.y == 0 && threadIdx.x == 0) ss->oob = 0;
+  if (acc_slot >= 0 && threadIdx.x == 0)
+    acc_merge(acc_at(W, sk.x, acc_slot, blockIdx.x % kAccSub), r);
+  if (sk.y == 0 && threadIdx.x == 0) ss->oob[0] = 0;
+}
+
+// Before the first fused step: reset reduction slot `slot` of every
+// scenario (k_reduce then fills it) and the out-of-grid flags.  One thread
+// per (scenario, sub-slot).
+__global__ void k_scn_prepare(Work W, ScnState *sst, int S, int slot) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= S * kAccSub) return;
+  const int s = t / kAccSub, sub = t % kAccSub;
+  acc_reset(acc_at(W, s, slot, sub));
+  if (sub == 0) {
+    sst[s].oob[0] = 0;
+    sst[s].oob[1] = 0;
+  }
 }
 
 // Grid of the binning from a reduced bounding box (contract in the header).
@@ -466,20 +568,25 @@ __device__ __forceinline__ Grid grid_of(const Red &r, double cell, int cap) {
   return g;
 }
 
-// Records a scenario's reduction and grid (one thread per scenario).
+// Records a scenario's reduction and grid (one thread per scenario); `buf`:
+// the count buffer the binning fills.
 __device__ __forceinline__ void publish_grid(const Work &W, ScnState *ss, const Red &r,
-                                             const Grid &g, bool binning, bool has_agents) {
+                                             const Grid &g, bool binning, bool has_agents,
+                                             int buf) {
   for (int k = 0; k < R_COUNT; ++k) ss->red[k] = r.v[k];
   if (r.v[R_NONFINITE])
     atomicOr((unsigned long long *)&W.st->status, (unsigned long long)CD_STATUS_NONFINITE);
-  if (!binning) return;
+  if (!binning) {
+    ss->valid = 0;
+    return;
+  }
   ss->origin[0] = g.ox;
   ss->origin[1] = g.oy;
   ss->nx = g.nx;
   ss->ny = g.ny;
   ss->valid = g.ok ? 1 : 0;
   ss->C = g.ok ? g.nx * g.ny : 0;
-  ss->C_clear = g.ok ? g.nx * g.ny : 0;
+  ss->C_clear[buf] = g.ok ? g.nx * g.ny : 0;
   if (!g.ok && has_agents)  // an empty scenario has no grid and no overflow
     atomicOr((unsigned long long *)&W.st->status, (unsigned long long)CD_STATUS_GRID_OVERFLOW);
 }
@@ -506,12 +613,12 @@ __device__ __forceinline__ void bin_scenario(const double2 *pos, double2 p0, dou
   const int stride = d.nf * blockDim.x;
   const int i0 = d.a0 + sk.y * blockDim.x + threadIdx.x;
   const Grid g = grid_of(r, cell, d.ccap);
-  if (sk.y == 0 && threadIdx.x == 0) publish_grid(W, ss, r, g, binning != 0, d.a1 > d.a0);
+  if (sk.y == 0 && threadIdx.x == 0) publish_grid(W, ss, r, g, binning != 0, d.a1 > d.a0, 0);
   if (!binning || !g.ok) return;
   for (int i = i0; i < d.a1; i += stride) {
     const int lc = cell_of_pos(i == i0 ? p0 : pos[i], g, cell);
     if (lc < 0) {  // positions changed after the bounding box was reduced
-      atomicOr(&ss->oob, 1);
+      atomicOr(&ss->oob[0], 1);
       W.cell_of[i] = -1;
       continue;
     }
@@ -652,8 +759,9 @@ inline int force_threads(const cd_workspace_t *w) {
 
 template <typename T>
 int launch_reduce(const Agents<T> &A, const Work &W, const Batch &B, int blocks,
-                  bool clear_counts, hipStream_t s) {
-  hipLaunchKernelGGL(k_reduce<T>, dim3(blocks), dim3(kBS), 0, s, A, W, B, clear_counts ? 1 : 0);
+                  bool clear_counts, hipStream_t s, int acc_slot = -1) {
+  hipLaunchKernelGGL(k_reduce<T>, dim3(blocks), dim3(kBS), 0, s, A, W, B, clear_counts ? 1 : 0,
+                     acc_slot);
   return check_launch();
 }
 

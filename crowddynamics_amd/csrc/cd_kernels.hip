@@ -4,21 +4,24 @@
 // Workspace layout, binning contract, reductions and the binning kernels
 // K_a / K_b are in cd_common.h.
 //
-// One step is three kernels, each launched once over all scenarios of a
-// batch (a single simulation is a batch of one):
-//   K_a  k_step_begin     per agent: the step's own terms (fluctuation,
-//                         navigation, orientation, adjusting); per scenario:
-//                         combine the previous step's partials (bounding
-//                         box, max |v|, max v0) and bin every agent into its
+// One step is two kernels, each launched once over all scenarios of a batch
+// (a single simulation is a batch of one):
+//   K_a  k_step_begin     per scenario: the reduction of the state the step
+//                         starts from (bounding box, max |v|, max v0), the
+//                         grid, dt; per agent: a snapshot of its state, the
+//                         step's own terms (fluctuation, navigation,
+//                         orientation, adjusting) and its insert into its
 //                         cell's slot list
-//   K_e  k_forces_direct  L = 4, 8 or 16 lanes per agent: stencil rows of
-//        (k_forces_tiled) the pairs and the walls, summed exactly
-//   K_f  k_integrate      Verlet update, Reset, per-block partials for the
-//                         next step, clear the cell counts, count the step
-// Every block of K_a, K_e and K_f serves a single scenario (block maps built
-// by cd_batch_prepare).  Pair forces are owner-computes: each agent sums its
+//   K_e  k_forces_direct  L = 4, 8 or 16 lanes per agent: the pairs of its
+//        (k_forces_tiled) 3 x 3 cell stencil and the walls, summed exactly;
+//                         then Verlet and Reset of the agent, the block's
+//                         reduction of the new state merged into the
+//                         scenario's slot for the next step, the step counted
+// Every block serves a single scenario (block maps built by
+// cd_batch_prepare).  Pair forces are owner-computes: each agent sums its
 // own side of every pair, so forces need no atomics, and the sums are exact,
-// so results do not depend on the lanes, tiles or insertion order.
+// so results do not depend on the lanes, tiles or insertion order.  K_e
+// reads neighbours from K_a's snapshot, since it moves agents as it goes.
 // K_b (k_cell_gather, cd_common.h) serves the adjoint only.
 
 #include "cd_common.h"
@@ -38,7 +41,7 @@ __global__ void __launch_bounds__(256) k_bl_count(const double2 *pos, double cel
   const Red r = reduce_partials(W, d);
   const Grid g = grid_of(r, cell, d.ccap);
   if (sk.y == 0 && threadIdx.x == 0) {
-    publish_grid(W, ss, r, g, true, d.a1 > d.a0);
+    publish_grid(W, ss, r, g, true, d.a1 > d.a0, 0);
     W.st->scan_epoch = (W.st->scan_epoch + 1u) & 0x3fffffffu;
     if (W.st->scan_epoch == 0u) W.st->scan_epoch = 1u;
     W.st->scan_ticket = 0u;
@@ -48,7 +51,7 @@ __global__ void __launch_bounds__(256) k_bl_count(const double2 *pos, double cel
   for (int i = d.a0 + sk.y * blockDim.x + threadIdx.x; i < d.a1; i += stride) {
     const int c = cell_of_pos(pos[i], g, cell);
     if (c < 0) {
-      atomicOr(&ss->oob, 1);
+      atomicOr(&ss->oob[0], 1);
       continue;
     }
     W.cell_of[i] = c;
@@ -73,7 +76,7 @@ __global__ void __launch_bounds__(kScanThreads) k_bl_scan(Work W, ScnState *ss) 
   __shared__ int s_tile, s_prefix;
   __shared__ int s_wave[kScanThreads / 64];
   DevState *st = W.st;
-  if (ss->oob) {
+  if (ss->oob[0]) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       ss->valid = 0;
       st->status |= CD_STATUS_GRID_OVERFLOW;
@@ -615,6 +618,59 @@ __device__ __forceinline__ TCView tc_agent(const Agents<T> &A, int j) {
   return t;
 }
 
+// Where the force kernels read neighbour state: the agent arrays (operator
+// path) or the snapshot the step's first kernel wrote (fused step, whose
+// force kernel moves its agents while other agents still read them).
+template <typename T>
+struct Nbr {
+  const double2 *pos;
+  const V2<T> *vel;
+  const double2 *ls, *rs;
+  const double *phi;
+};
+
+template <typename T>
+__host__ __device__ __forceinline__ Nbr<T> nbr_agents(const Agents<T> &A) {
+  Nbr<T> N;
+  N.pos = A.pos;
+  N.vel = A.vel;
+  N.ls = A.ls;
+  N.rs = A.rs;
+  N.phi = A.phi;
+  return N;
+}
+
+template <typename T>
+__host__ __device__ __forceinline__ Nbr<T> nbr_snapshot(const Work &W) {
+  Nbr<T> N;
+  N.pos = W.snap_pos;
+  N.vel = (const V2<T> *)W.snap_vel;
+  N.ls = W.snap_ls;
+  N.rs = W.snap_rs;
+  N.phi = W.snap_phi;
+  return N;
+}
+
+template <typename T>
+__device__ __forceinline__ TCView tc_nbr(const Agents<T> &A, const Nbr<T> &N, int j) {
+  TCView t;
+  const double2 p = N.pos[j], l = N.ls[j], r = N.rs[j];
+  t.px[0] = p.x;
+  t.py[0] = p.y;
+  t.px[1] = l.x;
+  t.py[1] = l.y;
+  t.px[2] = r.x;
+  t.py[2] = r.y;
+  const V2<T> v = N.vel[j];
+  t.vx = (double)v.x;
+  t.vy = (double)v.y;
+  t.r_t = A.r_t[j];
+  t.r_s = A.r_s[j];
+  t.r_ts = A.r_ts[j];
+  t.phi = N.phi[j];
+  return t;
+}
+
 template <typename T, int MODEL>
 __device__ __forceinline__ TCView tc_lds(const TileLds<T, MODEL> &S, int e) {
   TCView t;
@@ -743,13 +799,18 @@ __device__ __forceinline__ void own_apply(const Agents<T> &A, const FusedArgs &f
   if (MODEL) A.torque[i] = tq;
 }
 
-// K_a of the fused step: per scenario, combine the previous step's partials
-// and publish the grid; per agent, bin it (bin_scenario's contract) and add
-// the step's own terms.  The own terms need neither grid nor neighbours:
-// the first agent's inputs are loaded before the partials are reduced, and
-// its noise is computed while its slot atomic and navigation map loads are
-// in flight.  The step's index is the counter's value: no other kernel
-// of the step has advanced it yet (K_f does, last).
+// K_a of the fused step (step index s = the counter, parity p = s & 1).
+// Per scenario: read the reduction of the state the step starts from
+// (acc[p], merged by the previous step's force kernel or by the
+// preparation), publish the grid, dt and time_tot, reset acc[1 - p] and
+// the other buffer's out-of-grid flag for this step's force kernel.  Per
+// agent: snapshot the state (neighbour data of the force kernel), bin it
+// into count buffer p (bin_scenario's contract), add the step's own terms.
+// Count buffer 1 - p, filled by the previous step, is cleared here.  The
+// own terms need neither grid nor neighbours: the first agent's inputs are
+// loaded before the reduction is read, and its noise is computed while its
+// slot atomic and navigation map loads are in flight.  The counter is read
+// here only; the force kernel advances it.
 template <typename T, int MODEL>
 __global__ void __launch_bounds__(256) k_step_begin(Agents<T> A, Work W, Batch B, FusedArgs fa,
                                                     double cell, int binning) {
@@ -757,30 +818,56 @@ __global__ void __launch_bounds__(256) k_step_begin(Agents<T> A, Work W, Batch B
   const ScnDesc d = desc_of(B, sk.x);
   ScnState *ss = &B.sst[sk.x];
   const uint64_t step = W.st->step;
+  const int p = (int)(step & 1ull);
+  int *count = W.count + (size_t)p * W.cap;
   const int stride = d.nf * blockDim.x;
-  const int i0 = d.a0 + sk.y * blockDim.x + threadIdx.x;
-  const Red part = load_partial(W, d);
+  const int t0 = sk.y * blockDim.x + threadIdx.x;
+  const int i0 = d.a0 + t0;
   double2 p0 = make_double2(0.0, 0.0);
   OwnIn<T> o0;
   if (i0 < d.a1) {
     p0 = A.pos[i0];
     o0 = own_load<T, MODEL>(A, fa, i0, p0);
   }
-  const Red r = block_reduce(part);
+  {  // counts of the previous step's binning
+    int *other = W.count + (size_t)(1 - p) * W.cap + d.c0;
+    const int clear = ss->C_clear[1 - p];
+    for (int c = t0; c < clear; c += stride) other[c] = 0;
+  }
+  const Red r = acc_read(W, sk.x, p);
   const Grid g = grid_of(r, cell, d.ccap);
-  if (sk.y == 0 && threadIdx.x == 0) publish_grid(W, ss, r, g, binning != 0, d.a1 > d.a0);
+  if (sk.y == 0)
+    for (int sub = threadIdx.x; sub < kAccSub; sub += blockDim.x)
+      acc_reset(acc_at(W, sk.x, 1 - p, sub));
+  if (sk.y == 0 && threadIdx.x == 0) {
+    publish_grid(W, ss, r, g, binning != 0, d.a1 > d.a0, p);
+    if (!binning) ss->C_clear[p] = 0;
+    ss->oob[1 - p] = 0;
+    ss->par = p;
+    const double dt = adaptive_dt(r.v, d.dt_min, d.dt_max);
+    ss->dt = dt;
+    ss->time_tot += dt;
+  }
   const bool bin = binning && g.ok;
+  V2<T> *snap_vel = (V2<T> *)W.snap_vel;
   auto serve = [&](int i, double2 pos, const OwnIn<T> &o) {
+    W.snap_pos[i] = pos;
+    snap_vel[i] = o.vel;
+    if (MODEL) {
+      W.snap_ls[i] = A.ls[i];
+      W.snap_rs[i] = A.rs[i];
+      W.snap_phi[i] = A.phi[i];
+    }
     int c = -1, k = 0;
     if (bin) {
       const int lc = cell_of_pos(pos, g, cell);
       if (lc < 0) {  // positions changed after the bounding box was reduced
-        atomicOr(&ss->oob, 1);
+        atomicOr(&ss->oob[p], 1);
         W.cell_of[i] = -1;
       } else {
         c = d.c0 + lc;
         W.cell_of[i] = c;
-        k = atomicAdd(&W.count[c], 1);
+        k = atomicAdd(&count[c], 1);
       }
     }
     NavFetch nf;
@@ -842,14 +929,16 @@ template <typename T, int MODEL>
 struct MemRow {
   const Agents<T> &A;
   const Work &W;
+  const int *count;  // the count buffer of this binning
+  Nbr<T> N;
   int c0, nx, ny, lx, ly, a0, a1;
   template <class F>
   __device__ __forceinline__ void visit(int self, int j, F &fn) const {
     if (j == self) return;
     if constexpr (MODEL == 0)
-      fn(j, A.pos[j], A.vel[j], A.radius[j]);
+      fn(j, N.pos[j], N.vel[j], A.radius[j]);
     else
-      fn(j, tc_agent(A, j));
+      fn(j, tc_nbr(A, N, j));
   }
   // The nine counts are loaded together.  A part's candidates are walked
   // in batches of four whose id and data loads are issued together, so a
@@ -862,9 +951,9 @@ struct MemRow {
       const int ox = lx + r - 1;
       const bool okx = ox >= 0 && ox < nx;
       gr[r] = c0 + ox * ny + ly - 1;
-      n[3 * r] = (okx && ly - 1 >= 0) ? W.count[gr[r]] : 0;
-      n[3 * r + 1] = okx ? W.count[gr[r] + 1] : 0;
-      n[3 * r + 2] = (okx && ly + 1 < ny) ? W.count[gr[r] + 2] : 0;
+      n[3 * r] = (okx && ly - 1 >= 0) ? count[gr[r]] : 0;
+      n[3 * r + 1] = okx ? count[gr[r] + 1] : 0;
+      n[3 * r + 2] = (okx && ly + 1 < ny) ? count[gr[r] + 2] : 0;
     }
     int pre[10];
     pre[0] = 0;
@@ -903,8 +992,8 @@ struct MemRow {
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
             if (j[u] >= 0) {
-              xj[u] = A.pos[j[u]];
-              vj[u] = A.vel[j[u]];
+              xj[u] = N.pos[j[u]];
+              vj[u] = N.vel[j[u]];
               rj[u] = A.radius[j[u]];
             }
           }
@@ -969,6 +1058,58 @@ __device__ __forceinline__ Fix128 shfl_xor(const Fix128 &v, int mask) {
   return t;
 }
 
+// translational_verlet (core/integrator.py:164-176), rotational_verlet
+// (:179-193), shoulders (simulation/agents.py:473-486) and Reset
+// (simulation/logic.py:59-64) of agent i, given its force f (torque tq),
+// position x, velocity vel, previous force fp and mass m.  Writes position,
+// velocity, previous force and the force (zero after Reset, else f).
+template <typename T>
+__device__ __forceinline__ void verlet_with(const Agents<T> &A, int model, int i, double dt,
+                                            bool do_reset, V2<T> f, double tq, double2 x,
+                                            V2<T> vel, V2<T> fp, double m, double2 &x_out,
+                                            V2<T> &v_out) {
+  const T tm = (T)m;
+  const T aox = fp.x / tm, aoy = fp.y / tm;
+  const T anx = f.x / tm, any = f.y / tm;
+  vel.x += (aox + anx) / T(2) * (T)dt;
+  vel.y += (aoy + any) / T(2) * (T)dt;
+  if (sizeof(T) == 8) {
+    x.x += (double)vel.x * dt + (double)anx / 2.0 * (dt * dt);
+    x.y += (double)vel.y * dt + (double)any / 2.0 * (dt * dt);
+  } else {
+    // fp32 mode: fp64 position accumulation of the fp32 increments
+    x.x += (double)vel.x * dt + (double)(anx / T(2)) * (dt * dt);
+    x.y += (double)vel.y * dt + (double)(any / T(2)) * (dt * dt);
+  }
+  A.force_prev[i] = f;
+  A.vel[i] = vel;
+  A.pos[i] = x;
+  V2<T> fz = f;
+  if (do_reset) {
+    fz.x = T(0);
+    fz.y = T(0);
+  }
+  A.force[i] = fz;
+  if (model) {
+    const double I = A.inertia[i];
+    const double ao = A.torque_prev[i] / I;
+    const double an = tq / I;
+    A.torque_prev[i] = tq;
+    const double w = A.omega[i] + (ao + an) / 2.0 * dt;
+    double ph = A.phi[i] + (w * dt + an / 2.0 * (dt * dt));
+    ph = wrap_to_pi(ph);
+    A.omega[i] = w;
+    A.phi[i] = ph;
+    const double rts = A.r_ts[i];
+    const double ox = sin(ph) * rts, oy = (-cos(ph)) * rts;
+    A.ls[i] = make_double2(x.x - ox, x.y - oy);
+    A.rs[i] = make_double2(x.x + ox, x.y + oy);
+    A.torque[i] = do_reset ? 0.0 : tq;
+  }
+  x_out = x;
+  v_out = vel;
+}
+
 // Agent-side inputs of the force pass, loaded together before any
 // neighbour is visited.
 template <typename T>
@@ -990,21 +1131,23 @@ __device__ __forceinline__ Self<T> self_of(const Agents<T> &A, int i, double2 po
 }
 
 // Pair and wall forces of agent i, served by L lanes (L = 4, 8 or 16; q =
-// lane in the group): lane q takes part q of the stencil's candidates and
-// every L-th wall from q.  Pair and wall terms are summed exactly, so
-// results do not depend on L.  Every lane of the
-// block calls this (have = false for idle groups), so the wall staging
-// barriers and the shuffles are uniform; the first kWallTile segments are
-// already staged.  force (torque) = self.f (self.tq) + (pairs + walls): the
-// pair and wall terms are one exact sum, rounded once.
+// lane in the group): lane q takes part q of the stencil's candidates
+// (when `pairs`) and every L-th wall from q (fused step with walls).  Pair
+// and wall terms are summed exactly, so results do not depend on L.  Every
+// lane of the block calls this (have = false for idle groups), so the wall
+// staging barriers and the shuffles are uniform; the first kWallTile
+// segments are already staged.  Every lane of the group returns force
+// (torque) = self.f (self.tq) + (pairs + walls): the pair and wall terms
+// are one exact sum, rounded once.
 template <typename T, int MODEL, bool FUSED, class Rows>
-__device__ __forceinline__ void agent_forces(const Agents<T> &A, const FusedArgs &fa,
-                                             const ScnDesc &d, double4 *sseg, int L, bool have,
-                                             int i, double2 pos, V2<T> vel, const TCView &me,
-                                             const Self<T> &self, const Rows &rows) {
+__device__ __forceinline__ void agent_forces(const FusedArgs &fa, const ScnDesc &d,
+                                             double4 *sseg, int L, bool have, bool pairs, int i,
+                                             double2 pos, V2<T> vel, const TCView &me,
+                                             const Self<T> &self, const Rows &rows, V2<T> &f,
+                                             double &tq) {
   const int q = threadIdx.x & (L - 1);
   PairSum ps;  // pairs
-  if (have) {
+  if (have && pairs) {
     if constexpr (MODEL == 0) {
       rows.each(q, L, i, [&](int, double2 xj, V2<T> vj, double rj) {
         pair_add_c<T>(pos, vel, self.w.r, self.o, xj, vj, rj, ps);
@@ -1044,38 +1187,83 @@ __device__ __forceinline__ void agent_forces(const Agents<T> &A, const FusedArgs
     ps.y.add(shfl_xor(ps.y, mask));
     if (MODEL) ps.m.add(shfl_xor(ps.m, mask));
   }
-  if (!have || q != 0) return;
-  V2<T> f = self.f;
-  double tq = self.tq;
+  f = self.f;
+  tq = self.tq;
   f.x += (T)ps.x.value();
   f.y += (T)ps.y.value();
   if (MODEL) tq += ps.m.value();
-  A.force[i] = f;
-  if (MODEL) A.torque[i] = tq;
+}
+
+// Inputs of the integrator the force kernels of the fused step load early.
+template <typename T>
+struct Tail {
+  V2<T> fp;
+  double m, v0;
+};
+
+template <typename T>
+__device__ __forceinline__ Tail<T> tail_of(const Agents<T> &A, int i) {
+  Tail<T> t;
+  t.fp = A.force_prev[i];
+  t.m = A.mass[i];
+  t.v0 = A.v0[i];
+  return t;
+}
+
+// After an agent's force pass, on its lead lane: the fused step integrates
+// it (Verlet and Reset) and adds its new state to the block's reduction r;
+// the AgentAgentInteractions node stores the force.
+template <typename T, int MODEL, bool FUSED>
+__device__ __forceinline__ void agent_finish(const Agents<T> &A, int i, bool lead, V2<T> f,
+                                             double tq, double2 pos, V2<T> vel,
+                                             const Tail<T> &t, double dt, int reset, Red &r) {
+  if (!lead) return;
+  if constexpr (FUSED) {
+    double2 x;
+    V2<T> v;
+    verlet_with(A, MODEL, i, dt, reset != 0, f, tq, pos, vel, t.fp, t.m, x, v);
+    r.add_agent(x, dhypot((double)v.x, (double)v.y), t.v0);
+  } else {
+    A.force[i] = f;
+    if (MODEL) A.torque[i] = tq;
+  }
+}
+
+// End of a fused force kernel: each wave's reduction of the new state (held
+// by every L-th lane) joins a sub-slot of the scenario's slot for the next
+// step (no block barrier); the step is counted.
+__device__ __forceinline__ void step_end(const Work &W, int scn, int p, Red r, int L) {
+  r = wave_reduce(r, L);
+  const int wave = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  if ((threadIdx.x & 63) == 0) acc_merge(acc_at(W, scn, 1 - p, wave % kAccSub), r);
+  if (blockIdx.x == 0 && threadIdx.x == 0) W.st->step += 1;
 }
 
 // Direct kernel: blocks of blockDim / L agents of one scenario, in agent
 // index order, L lanes each.  FUSED = false: the AgentAgentInteractions
-// node, force (torque) += pairs.  FUSED = true: the pair and wall forces of
-// one fused step (after the own terms the step's first kernel added), in
-// the post-order of the standard logic tree.  Nothing runs when the
-// scenario's binning is invalid (the integrator then adds the walls
-// itself).
+// node, force (torque) += pairs; nothing runs when the scenario's binning
+// is invalid.  FUSED = true: the pair and wall forces of one fused step
+// (after the own terms the step's first kernel added), in the post-order of
+// the standard logic tree, then the integrator and Reset of each agent.
+// Neighbour state comes from the step's snapshot, because agents move as
+// they are integrated; without a valid binning only the walls are added.
 template <typename T, int MODEL, bool FUSED>
-__global__ void __launch_bounds__(256) k_forces_direct(Agents<T> A, Work W, Batch B,
-                                                       FusedArgs fa, int L) {
+__global__ void __launch_bounds__(256, MODEL == 0 ? 3 : 1) k_forces_direct(Agents<T> A, Work W, Batch B,
+                                                       FusedArgs fa, int L, int reset) {
   __shared__ double4 sseg[2 * kWallTile];
   const int2 sk = block_e(B);
   const ScnDesc d = desc_of(B, sk.x);
-  const ScnState *ss = &B.sst[sk.x];
+  ScnState *ss = &B.sst[sk.x];
+  const int p = FUSED ? ss->par : 0;
   const bool pairs_on = FUSED ? ((fa.flags & CD_USE_AGENT_AGENT) != 0) : true;
-  const bool ok = pairs_on && ss->valid && !ss->oob;
-  if (sk.y == 0 && threadIdx.x == 0) B.sst[sk.x].pairs_done = ok ? 1 : 0;
-  if (!ok) return;
+  const bool ok = pairs_on && ss->valid && !ss->oob[p];
+  if (!FUSED && !ok) return;
+  const double dt = FUSED ? ss->dt : 0.0;
   stage_walls(fa, d, sseg, FUSED);
   const int per = (int)blockDim.x / L;
   const int i = d.a0 + sk.y * per + (int)threadIdx.x / L;
   const bool have = i < d.a1;
+  const bool lead = have && (threadIdx.x & (L - 1)) == 0;
   double2 pos = make_double2(0.0, 0.0);
   V2<T> vel;
   vel.x = T(0);
@@ -1083,22 +1271,43 @@ __global__ void __launch_bounds__(256) k_forces_direct(Agents<T> A, Work W, Batc
   int c = d.c0;
   TCView me;
   Self<T> self;
+  Tail<T> tail;
   if (have) {
     pos = A.pos[i];
     vel = A.vel[i];
-    c = W.cell_of[i];
+    if (ok) c = W.cell_of[i];
     if constexpr (MODEL == 1) me = tc_agent(A, i);
     self = self_of<T, MODEL>(A, i, pos);
+    if (FUSED && lead) tail = tail_of(A, i);
   }
   const int nx = ss->nx, ny = ss->ny;
-  const int lc = c - d.c0, lx = lc / ny, ly = lc - lx * ny;
-  const MemRow<T, MODEL> rows{A, W, d.c0, nx, ny, lx, ly, d.a0, d.a1};
+  const int lc = c - d.c0, lx = ny > 0 ? lc / ny : 0, ly = lc - lx * ny;
+  const MemRow<T, MODEL> rows{A, W, W.count + (size_t)p * W.cap,
+                              FUSED ? nbr_snapshot<T>(W) : nbr_agents(A),
+                              d.c0, nx, ny, lx, ly, d.a0, d.a1};
   __syncthreads();  // walls staged
-  agent_forces<T, MODEL, FUSED>(A, fa, d, sseg, L, have, i, pos, vel, me, self, rows);
+  V2<T> f;
+  double tq;
+  agent_forces<T, MODEL, FUSED>(fa, d, sseg, L, have, ok, i, pos, vel, me, self, rows, f, tq);
+  Red r;
+  r.init();
+  agent_finish<T, MODEL, FUSED>(A, i, lead, f, tq, pos, vel, tail, dt, reset, r);
+  if constexpr (FUSED) step_end(W, sk.x, p, r, L);
 }
 
 struct TileArgs {
   int tx, ty;  // cells per tile
+};
+
+// What the tiled kernel's agents need beyond the force pass: the count
+// buffer of the binning, the neighbour state, and for the fused step dt and
+// Reset.
+template <typename T>
+struct TileCtx {
+  const int *count;
+  Nbr<T> N;
+  double dt;
+  int reset;
 };
 
 // Stages the region of w x h inner cells at grid cell (cx0, cy0) and serves
@@ -1107,7 +1316,8 @@ struct TileArgs {
 template <typename T, int MODEL, bool FUSED>
 __device__ bool tile_region(const Agents<T> &A, const Work &W, const FusedArgs &fa,
                             const ScnDesc &d, int nx, int ny, TileLds<T, MODEL> &S,
-                            double4 *sseg, int cx0, int cy0, int w, int h) {
+                            double4 *sseg, int cx0, int cy0, int w, int h,
+                            const TileCtx<T> &cx, Red &red) {
   constexpr int CAP = TileLds<T, MODEL>::CAP;
   const int tid = threadIdx.x;
   const int RH = h + 2, NC = (w + 2) * RH;
@@ -1117,7 +1327,7 @@ __device__ bool tile_region(const Agents<T> &A, const Work &W, const FusedArgs &
     if (k >= NC) return 0;
     const int lx = rx0 + k / RH, ly = ry0 + k % RH;
     if (lx < 0 || lx >= nx || ly < 0 || ly >= ny) return 0;
-    return W.count[d.c0 + lx * ny + ly];
+    return cx.count[d.c0 + lx * ny + ly];
   };
   // 1. counts and offsets (two cells per thread: NC <= 2 * kTileThreads)
   const int k0 = 2 * tid, k1 = 2 * tid + 1;
@@ -1168,13 +1378,13 @@ __device__ bool tile_region(const Agents<T> &A, const Work &W, const FusedArgs &
   // 3. neighbour data
   for (int e = tid; e < M; e += kTileThreads) {
     const int j = S.id[e];
-    S.pos[e] = A.pos[j];
-    S.vel[e] = A.vel[j];
+    S.pos[e] = cx.N.pos[j];
+    S.vel[e] = cx.N.vel[j];
     S.r[e] = A.radius[j];
     if constexpr (MODEL == 1) {
-      S.ls[e] = A.ls[j];
-      S.rs[e] = A.rs[j];
-      S.phi[e] = A.phi[j];
+      S.ls[e] = cx.N.ls[j];
+      S.rs[e] = cx.N.rs[j];
+      S.phi[e] = cx.N.phi[j];
       S.rt[e] = A.r_t[j];
       S.rsr[e] = A.r_s[j];
       S.rts[e] = A.r_ts[j];
@@ -1205,141 +1415,118 @@ __device__ bool tile_region(const Agents<T> &A, const Work &W, const FusedArgs &
     const LdsRow<T, MODEL> rows{S, RH, kx, k - kx * RH};
     TCView me;
     if constexpr (MODEL == 1) me = tc_lds<T, MODEL>(S, e);
+    const int i = S.id[e];
+    const double2 pos = S.pos[e];
+    const V2<T> vel = S.vel[e];
+    const bool lead = have && (tid & 3) == 0;
     Self<T> self;
-    if (have) self = self_of<T, MODEL>(A, S.id[e], S.pos[e]);
-    agent_forces<T, MODEL, FUSED>(A, fa, d, sseg, 4, have, S.id[e], S.pos[e], S.vel[e], me, self,
-                                  rows);
+    Tail<T> tail;
+    if (have) self = self_of<T, MODEL>(A, i, pos);
+    if (FUSED && lead) tail = tail_of(A, i);
+    V2<T> f;
+    double tq;
+    agent_forces<T, MODEL, FUSED>(fa, d, sseg, 4, have, true, i, pos, vel, me, self, rows, f, tq);
+    agent_finish<T, MODEL, FUSED>(A, i, lead, f, tq, pos, vel, tail, cx.dt, cx.reset, red);
   }
   __syncthreads();
   return true;
 }
 
 // Serves the agents of grid cell (lx, ly) from memory (exact slow path).
+// Also the fused step's fallback without a valid binning (pairs = false,
+// gc ignored): every agent of the scenario from block part `part` of `nb`,
+// walls and integration only.
 template <typename T, int MODEL, bool FUSED>
 __device__ void cell_from_memory(const Agents<T> &A, const Work &W, const FusedArgs &fa,
                                  const ScnDesc &d, int nx, int ny, double4 *sseg, int lx,
-                                 int ly) {
+                                 int ly, const TileCtx<T> &cx, Red &red, bool pairs = true,
+                                 int part = 0, int nb = 1) {
   const int gc = d.c0 + lx * ny + ly;
-  const MemRow<T, MODEL> rows{A, W, d.c0, nx, ny, lx, ly, d.a0, d.a1};
-  for (int j0 = d.a0; j0 < d.a1; j0 += kTileGroups) {
+  const MemRow<T, MODEL> rows{A, W, cx.count, cx.N, d.c0, nx, ny, lx, ly, d.a0, d.a1};
+  for (int j0 = d.a0 + part * kTileGroups; j0 < d.a1; j0 += nb * kTileGroups) {
     const int i = j0 + (threadIdx.x >> 2);
-    const bool have = i < d.a1 && W.cell_of[i] == gc;
+    const bool have = i < d.a1 && (!pairs || W.cell_of[i] == gc);
+    const bool lead = have && (threadIdx.x & 3) == 0;
     double2 pos = make_double2(0.0, 0.0);
     V2<T> vel;
     vel.x = T(0);
     vel.y = T(0);
     TCView me;
     Self<T> self;
+    Tail<T> tail;
     if (have) {
       pos = A.pos[i];
       vel = A.vel[i];
       if constexpr (MODEL == 1) me = tc_agent(A, i);
       self = self_of<T, MODEL>(A, i, pos);
+      if (FUSED && lead) tail = tail_of(A, i);
     }
-    agent_forces<T, MODEL, FUSED>(A, fa, d, sseg, 4, have, i, pos, vel, me, self, rows);
+    V2<T> f;
+    double tq;
+    agent_forces<T, MODEL, FUSED>(fa, d, sseg, 4, have, pairs, i, pos, vel, me, self, rows, f,
+                                  tq);
+    agent_finish<T, MODEL, FUSED>(A, i, lead, f, tq, pos, vel, tail, cx.dt, cx.reset, red);
   }
 }
 
 // Tiled kernel (same contract as k_forces_direct): blocks of a scenario
-// stride over its tiles.
+// stride over its tiles; without a valid binning (fused step only) they
+// stride over its agents.
 template <typename T, int MODEL, bool FUSED>
 __global__ void __launch_bounds__(kTileThreads) k_forces_tiled(Agents<T> A, Work W, Batch B,
-                                                                FusedArgs fa, TileArgs ta) {
+                                                                FusedArgs fa, TileArgs ta,
+                                                                int reset) {
   __shared__ TileLds<T, MODEL> S;
   __shared__ double4 sseg[2 * kWallTile];
   const int2 sk = B.map_t ? B.map_t[blockIdx.x] : make_int2(0, (int)blockIdx.x);
   const ScnDesc d = desc_of(B, sk.x);
   const int nb = B.map_t ? d.nt : (int)gridDim.x;
-  const ScnState *ss = &B.sst[sk.x];
+  ScnState *ss = &B.sst[sk.x];
+  const int p = FUSED ? ss->par : 0;
   const bool pairs_on = FUSED ? ((fa.flags & CD_USE_AGENT_AGENT) != 0) : true;
-  const bool ok = pairs_on && ss->valid && !ss->oob;
-  if (sk.y == 0 && threadIdx.x == 0) B.sst[sk.x].pairs_done = ok ? 1 : 0;
-  if (!ok) return;
-  const int nx = ss->nx, ny = ss->ny;
-  const int ntx = (nx + ta.tx - 1) / ta.tx, nty = (ny + ta.ty - 1) / ta.ty;
-  const int ntiles = ntx * nty;
-  if (sk.y >= ntiles) return;
+  const bool ok = pairs_on && ss->valid && !ss->oob[p];
+  if (!FUSED && !ok) return;
+  TileCtx<T> cx;
+  cx.count = W.count + (size_t)p * W.cap;
+  cx.N = FUSED ? nbr_snapshot<T>(W) : nbr_agents(A);
+  cx.dt = FUSED ? ss->dt : 0.0;
+  cx.reset = reset;
   stage_walls(fa, d, sseg, FUSED);
   __syncthreads();
-  for (int t = sk.y; t < ntiles; t += nb) {
-    const int cx0 = (t / nty) * ta.tx, cy0 = (t % nty) * ta.ty;
-    const int w = min(ta.tx, nx - cx0), h = min(ta.ty, ny - cy0);
-    if (tile_region<T, MODEL, FUSED>(A, W, fa, d, nx, ny, S, sseg, cx0, cy0, w, h))
-      continue;
-    for (int c = 0; c < w * h; ++c) {
-      const int lx = cx0 + c / h, ly = cy0 + c % h;
-      if (!tile_region<T, MODEL, FUSED>(A, W, fa, d, nx, ny, S, sseg, lx, ly, 1, 1))
-        cell_from_memory<T, MODEL, FUSED>(A, W, fa, d, nx, ny, sseg, lx, ly);
+  Red red;
+  red.init();
+  if (ok) {
+    const int nx = ss->nx, ny = ss->ny;
+    const int ntx = (nx + ta.tx - 1) / ta.tx, nty = (ny + ta.ty - 1) / ta.ty;
+    const int ntiles = ntx * nty;
+    for (int t = sk.y; t < ntiles; t += nb) {
+      const int cx0 = (t / nty) * ta.tx, cy0 = (t % nty) * ta.ty;
+      const int w = min(ta.tx, nx - cx0), h = min(ta.ty, ny - cy0);
+      if (tile_region<T, MODEL, FUSED>(A, W, fa, d, nx, ny, S, sseg, cx0, cy0, w, h, cx, red))
+        continue;
+      for (int c = 0; c < w * h; ++c) {
+        const int lx = cx0 + c / h, ly = cy0 + c % h;
+        if (!tile_region<T, MODEL, FUSED>(A, W, fa, d, nx, ny, S, sseg, lx, ly, 1, 1, cx, red))
+          cell_from_memory<T, MODEL, FUSED>(A, W, fa, d, nx, ny, sseg, lx, ly, cx, red);
+      }
     }
+  } else {
+    cell_from_memory<T, MODEL, FUSED>(A, W, fa, d, 0, 0, sseg, 0, 0, cx, red, false, sk.y, nb);
   }
+  if constexpr (FUSED) step_end(W, sk.x, p, red, 4);
 }
 
 // ---------------------------------------------------------------------------
-// K_f: integrator
+// Integrator node
 // ---------------------------------------------------------------------------
 
-// adaptive_timestep (core/integrator.py:81-97) from reduced values.
-__device__ __forceinline__ double adaptive_dt(const unsigned long long *v, double dt_min,
-                                              double dt_max) {
-  const double v_max = red_vmax(v);
-  if (v_max == 0.0) return dt_max;
-  const double v0max = from_ord_bits(v[R_V0MAX]);
-  const double dx_max = 1.1 * v0max * dt_max;
-  const double dt = dx_max / v_max;
-  if (dt > dt_max) return dt_max;
-  if (dt < dt_min) return dt_min;
-  return dt;
-}
-
-// translational_verlet (core/integrator.py:164-176), rotational_verlet
-// (:179-193) and shoulders (simulation/agents.py:473-486) of agent i.
+// The integrator of agent i with its force (and torque) given, see
+// verlet_with.
 template <typename T>
 __device__ __forceinline__ void verlet_agent(const Agents<T> &A, int model, int i, double dt,
                                              bool do_reset, double2 &x_out, V2<T> &v_out) {
-  const double m = A.mass[i];
-  const V2<T> f = A.force[i], fp = A.force_prev[i];
-  V2<T> vel = A.vel[i];
-  double2 x = A.pos[i];
-  const T tm = (T)m;
-  const T aox = fp.x / tm, aoy = fp.y / tm;
-  const T anx = f.x / tm, any = f.y / tm;
-  vel.x += (aox + anx) / T(2) * (T)dt;
-  vel.y += (aoy + any) / T(2) * (T)dt;
-  if (sizeof(T) == 8) {
-    x.x += (double)vel.x * dt + (double)anx / 2.0 * (dt * dt);
-    x.y += (double)vel.y * dt + (double)any / 2.0 * (dt * dt);
-  } else {
-    // fp32 mode: fp64 position accumulation of the fp32 increments
-    x.x += (double)vel.x * dt + (double)(anx / T(2)) * (dt * dt);
-    x.y += (double)vel.y * dt + (double)(any / T(2)) * (dt * dt);
-  }
-  A.force_prev[i] = f;
-  A.vel[i] = vel;
-  A.pos[i] = x;
-  if (do_reset) {
-    V2<T> z;
-    z.x = T(0);
-    z.y = T(0);
-    A.force[i] = z;
-  }
-  if (model) {
-    const double I = A.inertia[i];
-    const double ao = A.torque_prev[i] / I;
-    const double an = A.torque[i] / I;
-    A.torque_prev[i] = A.torque[i];
-    const double w = A.omega[i] + (ao + an) / 2.0 * dt;
-    double ph = A.phi[i] + (w * dt + an / 2.0 * (dt * dt));
-    ph = wrap_to_pi(ph);
-    A.omega[i] = w;
-    A.phi[i] = ph;
-    const double rts = A.r_ts[i];
-    const double ox = sin(ph) * rts, oy = (-cos(ph)) * rts;
-    A.ls[i] = make_double2(x.x - ox, x.y - oy);
-    A.rs[i] = make_double2(x.x + ox, x.y + oy);
-    if (do_reset) A.torque[i] = 0.0;
-  }
-  x_out = x;
-  v_out = vel;
+  verlet_with(A, model, i, dt, do_reset, A.force[i], model ? A.torque[i] : 0.0, A.pos[i],
+              A.vel[i], A.force_prev[i], A.mass[i], x_out, v_out);
 }
 
 // Integrator node: dt from the partials the preceding k_reduce wrote.
@@ -1363,58 +1550,6 @@ __global__ void __launch_bounds__(256) k_integrate_node(Agents<T> A, int model, 
   }
 }
 
-// Fused step: dt from the values K_a reduced (scenario state).  When the
-// force kernel did not serve the scenario (pairs off, or binning invalid),
-// the walls are added here first.  Then Verlet and Reset of the scenario's
-// agents (grid-stride over its blocks), partials of the new state for the
-// next step, and zeroing of this step's cell counts.  The step counter
-// advances here, the step's last kernel.
-template <typename T, int MODEL>
-__global__ void __launch_bounds__(256) k_integrate(Agents<T> A, Work W, Batch B, FusedArgs fa,
-                                                   int do_reset) {
-  const int2 sk = block_f(B);
-  const ScnDesc d = desc_of(B, sk.x);
-  ScnState *ss = &B.sst[sk.x];
-  unsigned long long prev[R_COUNT];
-  for (int k = 0; k < R_COUNT; ++k) prev[k] = ss->red[k];
-  const double dt = adaptive_dt(prev, d.dt_min, d.dt_max);
-  const bool walls = (fa.flags & CD_USE_OBSTACLE) &&
-                     !((fa.flags & CD_USE_AGENT_AGENT) && ss->pairs_done);
-  const int t0 = sk.y * blockDim.x + threadIdx.x, stride = d.nf * blockDim.x;
-  const int clear = ss->C_clear;
-  for (int c = t0; c < clear; c += stride) W.count[d.c0 + c] = 0;
-  Red r;
-  r.init();
-  for (int i = d.a0 + t0; i < d.a1; i += stride) {
-    if (walls) {
-      V2<T> f = A.force[i];
-      double tq = MODEL ? A.torque[i] : 0.0;
-      const double2 pos = A.pos[i];
-      const V2<T> vel = A.vel[i];
-      T wx = T(0), wy = T(0);
-      double wtq = 0.0;
-      walls_agent(A, MODEL, i, pos, vel, fa.segs + d.w0, d.w1 - d.w0, wx, wy, wtq);
-      f.x += wx;
-      f.y += wy;
-      if (MODEL) tq += wtq;
-      A.force[i] = f;
-      if (MODEL) A.torque[i] = tq;
-    }
-    double2 x;
-    V2<T> v;
-    verlet_agent(A, MODEL, i, dt, do_reset != 0, x, v);
-    r.add_agent(x, dhypot((double)v.x, (double)v.y), A.v0[i]);
-  }
-  r = block_reduce(r);
-  store_partial(W, r);
-  if (sk.y == 0 && threadIdx.x == 0) {
-    ss->dt = dt;
-    ss->time_tot += dt;
-    ss->oob = 0;
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) W.st->step += 1;
-}
-
 // ---------------------------------------------------------------------------
 // Host helpers
 // ---------------------------------------------------------------------------
@@ -1422,35 +1557,39 @@ __global__ void __launch_bounds__(256) k_integrate(Agents<T> A, Work W, Batch B,
 
 template <typename T, int MODEL, bool FUSED>
 void launch_forces_m(const Agents<T> &A, const Work &W, const Batch &B, const FusedArgs &fa,
-                     const Launch &L, const cd_workspace_t *w, hipStream_t s) {
+                     const Launch &L, const cd_workspace_t *w, hipStream_t s, int reset) {
   if (w->tile_x > 0 && w->tile_y > 0) {
     TileArgs ta;
     ta.tx = (int)w->tile_x;
     ta.ty = (int)w->tile_y;
     hipLaunchKernelGGL((k_forces_tiled<T, MODEL, FUSED>), dim3(L.blocks_t), dim3(kTileThreads), 0,
-                       s, A, W, B, fa, ta);
+                       s, A, W, B, fa, ta, reset);
   } else {
     hipLaunchKernelGGL((k_forces_direct<T, MODEL, FUSED>), dim3(L.blocks_e), dim3(L.threads_e), 0,
-                       s, A, W, B, fa, L.lanes);
+                       s, A, W, B, fa, L.lanes, reset);
   }
 }
 
+// The force kernel: of the fused step (the step's second and last kernel,
+// forces then integration; `reset` as CD_USE_RESET), or of the
+// AgentAgentInteractions node.
 template <typename T>
 int launch_forces(const cd_agents_t *a, const Work &W, const Batch &B, const FusedArgs &fa,
-                  bool fused, const Launch &L, const cd_workspace_t *w, hipStream_t s) {
+                  bool fused, const Launch &L, const cd_workspace_t *w, hipStream_t s,
+                  int reset = 0) {
   const Agents<T> A = typed<T>(a);
   if (a->model == CD_THREE_CIRCLE) {
     if constexpr (sizeof(T) == 8) {
       if (fused)
-        launch_forces_m<T, 1, true>(A, W, B, fa, L, w, s);
+        launch_forces_m<T, 1, true>(A, W, B, fa, L, w, s, reset);
       else
-        launch_forces_m<T, 1, false>(A, W, B, fa, L, w, s);
+        launch_forces_m<T, 1, false>(A, W, B, fa, L, w, s, reset);
     }
   } else {
     if (fused)
-      launch_forces_m<T, 0, true>(A, W, B, fa, L, w, s);
+      launch_forces_m<T, 0, true>(A, W, B, fa, L, w, s, reset);
     else
-      launch_forces_m<T, 0, false>(A, W, B, fa, L, w, s);
+      launch_forces_m<T, 0, false>(A, W, B, fa, L, w, s, reset);
   }
   return check_launch();
 }
@@ -1494,31 +1633,20 @@ void launch_step_begin(const cd_agents_t *a, const cd_step_params_t *p, const Wo
   }
 }
 
-// One step: K_a, K_e (when pairs are on), K_f; events ev[0..3] around them
-// when timed.
+// One step: K_a, then the force kernel (forces and integration); events
+// ev[0..3] around them when timed (ev[2] and ev[3] both after the force
+// kernel: the step has no separate integration kernel).
 template <typename T>
 int step_kernels(const cd_agents_t *a, const cd_step_params_t *p, const Work &W, const Batch &B,
                  const Launch &L, const cd_workspace_t *w, hipStream_t s, hipEvent_t *ev) {
-  const Agents<T> A = typed<T>(a);
-  const bool pairs = (p->flags & CD_USE_AGENT_AGENT) != 0;
   const FusedArgs fa = fused_args(p);
+  const int reset = (p->flags & CD_USE_RESET) ? 1 : 0;
   mark(ev, 0, s);
   launch_step_begin<T>(a, p, W, B, fa, L, s);
   mark(ev, 1, s);
-  if (pairs) {
-    const int rc = launch_forces<T>(a, W, B, fa, true, L, w, s);
-    if (rc) return rc;
-  }
+  const int rc = launch_forces<T>(a, W, B, fa, true, L, w, s, reset);
+  if (rc) return rc;
   mark(ev, 2, s);
-  const int reset = (p->flags & CD_USE_RESET) ? 1 : 0;
-  if (a->model == CD_THREE_CIRCLE) {
-    if constexpr (sizeof(T) == 8)
-      hipLaunchKernelGGL((k_integrate<T, 1>), dim3(L.blocks_f), dim3(kBS), 0, s, A, W, B, fa,
-                         reset);
-  } else {
-    hipLaunchKernelGGL((k_integrate<T, 0>), dim3(L.blocks_f), dim3(kBS), 0, s, A, W, B, fa,
-                       reset);
-  }
   mark(ev, 3, s);
   return check_launch();
 }
@@ -1844,8 +1972,10 @@ int cd_step_prepare(const cd_agents_t *a, const cd_step_params_t *p, const cd_wo
   if (a->n == 0) return CD_OK;
   const Batch B = single_batch(w, p->n_segments, p->seed, p->dt_min, p->dt_max);
   const Launch L = single_launch(w);
-  if (a->real == CD_F64) return launch_reduce(typed<double>(a), W, B, L.blocks_f, true, s);
-  return launch_reduce(typed<float>(a), W, B, L.blocks_f, true, s);
+  const int slot = (int)(first_step & 1ull);
+  hipLaunchKernelGGL(k_scn_prepare, dim3(1), dim3(kBS), 0, s, W, B.sst, 1, slot);
+  if (a->real == CD_F64) return launch_reduce(typed<double>(a), W, B, L.blocks_f, true, s, slot);
+  return launch_reduce(typed<float>(a), W, B, L.blocks_f, true, s, slot);
 }
 
 int cd_step(const cd_agents_t *a, const cd_step_params_t *p, const cd_workspace_t *w,
@@ -1891,9 +2021,12 @@ int cd_batch_prepare(const cd_agents_t *a, const cd_step_params_t *p,
   const Work W = carve(w);
   const Batch B = batch_of(w);
   hipLaunchKernelGGL(k_set_step, dim3(1), dim3(1), 0, s, W.st, (unsigned long long)first_step);
+  const int slot = (int)(first_step & 1ull);
+  hipLaunchKernelGGL(k_scn_prepare, dim3(grid_for((int64_t)n_scenarios * kAccSub, kBS)),
+                     dim3(kBS), 0, s, W, B.sst, (int)n_scenarios, slot);
   if ((rc = check_launch())) return rc;
-  if (a->real == CD_F64) return launch_reduce(typed<double>(a), W, B, w->blocks_f, true, s);
-  return launch_reduce(typed<float>(a), W, B, w->blocks_f, true, s);
+  if (a->real == CD_F64) return launch_reduce(typed<double>(a), W, B, w->blocks_f, true, s, slot);
+  return launch_reduce(typed<float>(a), W, B, w->blocks_f, true, s, slot);
 }
 
 int cd_batch_step(const cd_agents_t *a, const cd_step_params_t *p, const cd_workspace_t *w,
@@ -1950,7 +2083,7 @@ int cd_kernel_timed(const cd_agents_t *a, const cd_step_params_t *p, const cd_wo
                     int32_t kernel, int64_t reps, double *avg_ms, cd_stream_t stream) {
   int rc = check_step(a, p, w);
   if (rc) return rc;
-  if (!avg_ms || reps < 1 || kernel < 0 || kernel > 2) return CD_ERR_ARG;
+  if (!avg_ms || reps < 1 || kernel < 0 || kernel > 1) return CD_ERR_ARG;
   *avg_ms = 0.0;
   if (a->n == 0) return CD_OK;
   hipStream_t s = (hipStream_t)stream;
@@ -1968,17 +2101,6 @@ int cd_kernel_timed(const cd_agents_t *a, const cd_step_params_t *p, const cd_wo
     else
       launch_step_begin<float>(a, p, W, B, fa, L, s);
   };
-  auto integrate = [&]() {
-    if (a->model == CD_THREE_CIRCLE)
-      hipLaunchKernelGGL((k_integrate<double, 1>), dim3(L.blocks_f), dim3(kBS), 0, s,
-                         typed<double>(a), W, B, fa, reset);
-    else if (a->real == CD_F64)
-      hipLaunchKernelGGL((k_integrate<double, 0>), dim3(L.blocks_f), dim3(kBS), 0, s,
-                         typed<double>(a), W, B, fa, reset);
-    else
-      hipLaunchKernelGGL((k_integrate<float, 0>), dim3(L.blocks_f), dim3(kBS), 0, s,
-                         typed<float>(a), W, B, fa, reset);
-  };
   bin();
   if ((rc = check_launch())) return rc;
   hipEvent_t ev[2];
@@ -1988,11 +2110,9 @@ int cd_kernel_timed(const cd_agents_t *a, const cd_step_params_t *p, const cd_wo
   for (int64_t r = 0; r < reps && rc == CD_OK; ++r) {
     if (kernel == 0)
       bin();
-    else if (kernel == 1)
-      rc = a->real == CD_F64 ? launch_forces<double>(a, W, B, fa, true, L, w, s)
-                             : launch_forces<float>(a, W, B, fa, true, L, w, s);
     else
-      integrate();
+      rc = a->real == CD_F64 ? launch_forces<double>(a, W, B, fa, true, L, w, s, reset)
+                             : launch_forces<float>(a, W, B, fa, true, L, w, s, reset);
     if (rc == CD_OK) rc = check_launch();
   }
   (void)hipEventRecord(ev[1], s);

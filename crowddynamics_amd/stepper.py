@@ -131,8 +131,9 @@ class Stepper:
 
     def timed(self, steps):
         """GPU milliseconds of the first kernel (per-agent terms and binning),
-        the force and integrate kernels and the whole step, summed over
-        `steps` steps, from HIP events on the current stream."""
+        the force kernel (forces and integration), an empty slot (0) and the
+        whole step, summed over `steps` steps, from HIP events on the current
+        stream."""
         ms = (C.c_double * 4)()
         _lib.check(_lib.lib().cd_step_timed(self.agents.ref(), C.byref(self.params),
                                             self.ws.ref(), int(steps), ms, stream_handle()),
@@ -142,9 +143,9 @@ class Stepper:
     def kernel_timed(self, kernel, reps=20):
         """Average GPU milliseconds of `reps` back-to-back launches of one
         step kernel ('bin': the first kernel, per-agent terms and binning;
-        'forces'; 'integrate').  Leaves the state
+        'forces': the force kernel, which also integrates).  Leaves the state
         meaningless: time a copy, or prepare() again afterwards."""
-        k = {'bin': 0, 'forces': 1, 'integrate': 2}[kernel]
+        k = {'bin': 0, 'forces': 1}[kernel]
         ms = C.c_double()
         _lib.check(_lib.lib().cd_kernel_timed(self.agents.ref(), C.byref(self.params),
                                               self.ws.ref(), k, int(reps), C.byref(ms),

@@ -221,10 +221,10 @@ typedef struct cd_step_params {
 /* One MultiAgentSimulation.update() (crowddynamics/simulation/multiagent.py:51-55)
  * of the tree Reset << (Integrator << (Fluctuation, Adjusting << (Navigation,
  * Orientation), AgentAgentInteractions, AgentObstacleInteractions))
- * (crowddynamics/examples/simulations.py:123-136) in three kernels: the
+ * (crowddynamics/examples/simulations.py:123-136) in two kernels: the
  * per-agent terms (fluctuation, navigation, orientation, adjusting) with the
- * binning, the pair and wall forces, and the integration.  The fluctuation
- * step counter and every reduction stay on the device.
+ * binning, then the pair and wall forces with the integration.  The
+ * fluctuation step counter and every reduction stay on the device.
  * cd_step_prepare must run once before the first cd_step (and after any
  * host-side change of positions or velocities). */
 int cd_step_prepare(const cd_agents_t *agents, const cd_step_params_t *params,
@@ -233,7 +233,7 @@ int cd_step(const cd_agents_t *agents, const cd_step_params_t *params, const cd_
             cd_stream_t stream);
 
 /* Batched ensemble: independent scenarios advanced together, one launch per
- * step kernel over all scenarios (three launches per step, graph-capturable).
+ * step kernel over all scenarios (two launches per step, graph-capturable).
  * Synchronous: uploads the scenario table and block maps into the
  * workspace (which needs max_scenarios >= n_scenarios) and fills ws->
  * n_scenarios / blocks_f / blocks_e.  params supplies the cell size, flags,
@@ -292,16 +292,17 @@ int cd_adjoint_segment(const cd_agents_t *agents, const cd_step_params_t *params
 
 /* Synchronous: runs `steps` steps (cd_step, or cd_batch_step on a workspace
  * prepared by cd_batch_prepare) and returns in kernel_ms[4] the summed GPU
- * time of the first kernel (per-agent terms and binning), the force kernel,
- * the integrate kernel and the whole step, measured with HIP events recorded
- * on `stream` around each launch.  No reference counterpart (measurement). */
+ * time of the first kernel (per-agent terms and binning), the force kernel
+ * (forces and integration), 0 (no separate integration kernel) and the
+ * whole step, measured with HIP events recorded on `stream` around each
+ * launch.  No reference counterpart (measurement). */
 int cd_step_timed(const cd_agents_t *agents, const cd_step_params_t *params,
                   const cd_workspace_t *ws, int64_t steps, double *kernel_ms,
                   cd_stream_t stream);
 
 /* Synchronous measurement of one step kernel (0: the first kernel, per-agent
- * terms and binning; 1: forces; 2: integrate): after one pass of the first
- * kernel, `reps` back-to-back launches of it
+ * terms and binning; 1: the force kernel, forces and integration): after one
+ * pass of the first kernel, `reps` back-to-back launches of it
  * between two HIP events on `stream`; *avg_ms = elapsed / reps.  Repeating
  * a kernel leaves the agent state and workspace meaningless (forces
  * accumulate, agents move, cell counts grow): use a copy, then discard it

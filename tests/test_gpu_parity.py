@@ -458,3 +458,76 @@ def test_step_room_fp64_and_fp32(room):
     dev = np.hypot(*(out['position'] - ref['position']).T)
     say('  f32 after 20 steps: median |dx| %.3e m, max %.3e m' % (np.median(dev), dev.max()))
     assert np.median(dev) < 1e-5
+
+
+def _pairs_off_stepper(scn, tile=None):
+    from crowddynamics_amd import _lib
+    from crowddynamics_amd.stepper import ALL_NODES
+    return Stepper(scn.agents.copy(), segments=scn.segments, nav=scn.nav,
+                   cell_size=scn.cell_size, dt_min=scn.dt_min, dt_max=scn.dt_max,
+                   seed=scn.seed, flags=ALL_NODES & ~_lib.CD_USE_AGENT_AGENT,
+                   bounds=scn.bounds, tile=tile)
+
+
+def test_step_without_pairs(hallway):
+    """AgentAgentInteractions off: the force kernel still adds the walls and
+    integrates every agent.  Five steps against the oracle's tree without
+    the node, with the agent-parallel kernel and with the tiled kernel's
+    path for a step without binning (the two agree bit for bit)."""
+    say('fused 5 steps without agent-agent pairs')
+    nv = hallway.nav
+    sc = O.Scenario(cell_size=hallway.cell_size, dt_min=hallway.dt_min,
+                    dt_max=hallway.dt_max, seed=hallway.seed, segments=hallway.segments,
+                    nav_maps=nv['maps'] if nv else None, nav_mode=nv['mode'] if nv else 0,
+                    nav_origin=(nv['minx'], nv['miny']) if nv else (0.0, 0.0),
+                    nav_step=nv['step'] if nv else 0.1, agent_agent=False)
+    ref = hallway.agents.copy()
+    dts = O.run(ref, sc, 5)
+    outs = []
+    for tile in (None, (2, 2)):
+        st = _pairs_off_stepper(hallway, tile)
+        st.step(5)
+        s = st.check()
+        assert s.step == 5
+        assert abs(s.time_tot - float(np.sum(dts))) < 1e-12
+        outs.append(st.download(hallway.agents.copy()))
+    for f in ('position', 'velocity', 'force_prev'):
+        assert np.array_equal(outs[0][f], outs[1][f]), f
+        r = ref[f]
+        assert_vec_close(outs[0][f], r, np.hypot(*r.T) + 1.0, 1e-12, 'pairs off ' + f)
+    assert np.all(outs[0]['force'] == 0.0)
+
+
+def test_step_grid_overflow_reported(hallway):
+    """A cell capacity far below the grid: the step's binning is invalid, the
+    force kernel adds only the walls and integrates, and the overflow is
+    reported (GridOverflow) instead of touching cells beyond the capacity."""
+    from crowddynamics_amd.stepper import GridOverflow
+    say('fused step with a grid beyond the cell capacity')
+    st = Stepper(hallway.agents.copy(), segments=hallway.segments, cell_size=hallway.cell_size,
+                 dt_min=hallway.dt_min, dt_max=hallway.dt_max, seed=hallway.seed,
+                 cell_capacity=4, bounds=hallway.bounds)
+    st.step(2)
+    with pytest.raises(GridOverflow):
+        st.check()
+    out = st.download(hallway.agents.copy())
+    assert np.all(np.isfinite(out['position']))
+    assert st.state().step == 2
+
+
+@pytest.mark.parametrize('tile', [(1, 1), (3, 2)])
+def test_step_tiled_equals_direct(hallway, tile):
+    """Ten fused steps with the tiled force kernel (which integrates the
+    agents of each tile it serves) equal the agent-parallel kernel bit for
+    bit."""
+    say('fused 10 steps, tiled %s vs agent-parallel' % (tile,))
+    outs = []
+    for t in (None, tile):
+        st = Stepper(hallway.agents.copy(), segments=hallway.segments, nav=hallway.nav,
+                     cell_size=hallway.cell_size, dt_min=hallway.dt_min,
+                     dt_max=hallway.dt_max, seed=hallway.seed, bounds=hallway.bounds, tile=t)
+        st.step(10)
+        st.check()
+        outs.append(st.download(hallway.agents.copy()))
+    for f in ('position', 'velocity', 'force_prev', 'target_direction'):
+        assert np.array_equal(outs[0][f], outs[1][f]), f
