@@ -94,59 +94,95 @@ __device__ __forceinline__ Owner<double> owner_at(const Agents<double> &A, int j
   return o;
 }
 
-// Pair adjoints of agent i with part `part` (of m) of the candidates in its
-// stencil row through flat cell g0 (cells g0 .. g0 + 2 with counts n[]),
-// numbered cell by cell.
-__device__ __forceinline__ void row_vjp(const Agents<double> &A, const Work &W, const AdjView &J,
-                                        int g0, const int n[3], int part, int m, int a0, int a1,
-                                        int i, double2 xi, double2 vi, double ri,
-                                        const Owner<double> &oi, double2 lfi, double acc[8]) {
-  const int tot = n[0] + n[1] + n[2];
-  const int b0 = (tot * part) / m, b1 = (tot * (part + 1)) / m;
-  const double2 *e_vel = (const double2 *)W.e_vel;
-  int idx = 0;
+// Pair adjoints of agent i with part q (of L equal parts) of the candidates
+// of its 3 x 3 stencil, taken as one sequence (rows dx = -1, 0, +1, each
+// row's cells dy = -1, 0, +1, each cell's slots in agent order).
+__device__ __forceinline__ void stencil_vjp(const Agents<double> &A, const Work &W,
+                                            const AdjView &J, int c0, int nx, int ny, int lx,
+                                            int ly, int q, int L, int a0, int a1, int i,
+                                            double2 xi, double2 vi, double ri,
+                                            const Owner<double> &oi, double2 lfi,
+                                            double acc[8]) {
+  int gr[3], n[9];
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
+  for (int r = 0; r < 3; ++r) {
+    const int ox = lx + r - 1;
+    const bool okx = ox >= 0 && ox < nx;
+    gr[r] = c0 + ox * ny + ly - 1;
+    n[3 * r] = (okx && ly - 1 >= 0) ? W.count[gr[r]] : 0;
+    n[3 * r + 1] = okx ? W.count[gr[r] + 1] : 0;
+    n[3 * r + 2] = (okx && ly + 1 < ny) ? W.count[gr[r] + 2] : 0;
+  }
+  int pre[10];
+  pre[0] = 0;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) pre[k + 1] = pre[k] + n[k];
+  const int tot = pre[9];
+  const int b0 = (int)(((long long)tot * q) / L), b1 = (int)(((long long)tot * (q + 1)) / L);
+  const double2 *e_vel = (const double2 *)W.e_vel;
+  bool over = false;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) over = over || n[k] > W.K;
+  if (!over) {  // one copy of the pair code: the candidates in order
+    for (int idx = b0; idx < b1; ++idx) {
+      int base = 0, c = gr[0];
+#pragma unroll
+      for (int k = 1; k < 9; ++k) {
+        if (idx >= pre[k]) {
+          base = pre[k];
+          c = gr[k / 3] + k % 3;
+        }
+      }
+      const size_t e = (size_t)c * W.K + (idx - base);
+      const int j = W.ell_ids[e];
+      if (j == i) continue;
+      const double2 xj = W.e_pos[e], vj = e_vel[e];
+      pair_vjp(xi.x - xj.x, xi.y - xj.y, vi.x - vj.x, vi.y - vj.y, ri + W.e_r[e], oi,
+               owner_at(A, j), lfi, J.lam_f[j], acc);
+    }
+    return;
+  }
+  // a cell with more agents than slots (exact slow path): members in agent
+  // order, cell by cell
+#pragma unroll 1
+  for (int k = 0; k < 9; ++k) {
     const int cnt = n[k];
-    const int s0 = max(0, b0 - idx), s1 = min(cnt, b1 - idx);
-    idx += cnt;
+    const int s0 = max(0, b0 - pre[k]), s1 = min(cnt, b1 - pre[k]);
     if (s0 >= s1) continue;
-    const int c = g0 + k;
-    if (cnt <= W.K) {
-      const size_t e0 = (size_t)c * W.K;
-      for (int s = s0; s < s1; ++s) {
-        const size_t e = e0 + s;
+    const int c = gr[k / 3] + k % 3;
+    if (cnt <= W.K) {  // an ordinary cell: its slots
+      for (int s2 = s0; s2 < s1; ++s2) {
+        const size_t e = (size_t)c * W.K + s2;
         const int j = W.ell_ids[e];
         if (j == i) continue;
         const double2 xj = W.e_pos[e], vj = e_vel[e];
         pair_vjp(xi.x - xj.x, xi.y - xj.y, vi.x - vj.x, vi.y - vj.y, ri + W.e_r[e], oi,
                  owner_at(A, j), lfi, J.lam_f[j], acc);
       }
-    } else {  // more agents than slots (exact slow path), members in agent order
-      int r = 0;
-      for (int j = a0; j < a1 && r < s1; ++j) {
-        if (W.cell_of[j] != c) continue;
-        if (r >= s0 && j != i) {
-          const double2 xj = A.pos[j];
-          const V2<double> vj = A.vel[j];
-          pair_vjp(xi.x - xj.x, xi.y - xj.y, vi.x - vj.x, vi.y - vj.y, ri + A.radius[j], oi,
-                   owner_at(A, j), lfi, J.lam_f[j], acc);
-        }
-        ++r;
+      continue;
+    }
+    int r = 0;
+    for (int j = a0; j < a1 && r < s1; ++j) {
+      if (W.cell_of[j] != c) continue;
+      if (r >= s0 && j != i) {
+        const double2 xj = A.pos[j];
+        const V2<double> vj = A.vel[j];
+        pair_vjp(xi.x - xj.x, xi.y - xj.y, vi.x - vj.x, vi.y - vj.y, ri + A.radius[j], oi,
+                 owner_at(A, j), lfi, J.lam_f[j], acc);
       }
+      ++r;
     }
   }
 }
 
-// L lanes per agent (L = 4 m) as in the forward force kernel: lanes
-// 0 .. 3m-1 take part q / 3 of stencil row dx = q % 3 - 1, lane 3m the
-// agent's own (adjusting) terms and the combination.
+// L lanes per agent (4, 8 or 16) as in the forward force kernel: lane q
+// takes part q of the stencil's candidates; lane 0 adds the agent's own
+// (adjusting) terms and writes the results.
 __global__ void __launch_bounds__(256) k_adjoint(Agents<double> A, Work W, Batch B, AdjView J,
                                                  int flags, int L) {
   const int2 sk = block_e(B);
   const ScnDesc d = desc_of(B, sk.x);
   const ScnState *ss = &B.sst[sk.x];
-  const int m = L >> 2;
   const int i = d.a0 + sk.y * ((int)blockDim.x / L) + (int)threadIdx.x / L;
   const int q = threadIdx.x & (L - 1);
   const bool have = i < d.a1;
@@ -156,22 +192,15 @@ __global__ void __launch_bounds__(256) k_adjoint(Agents<double> A, Work W, Batch
   for (int k = 0; k < 8; ++k) acc[k] = 0.0;
   double2 lfi = make_double2(0.0, 0.0);
   if (have) lfi = J.lam_f[i];
-  if (have && binned && q < 3 * m) {
+  if (have && binned) {
     const double2 xi = A.pos[i];
     const V2<double> v = A.vel[i];
     const double2 vi = make_double2(v.x, v.y);
     const int c = W.cell_of[i] - d.c0;
     const int ny = ss->ny, nx = ss->nx;
     const int lx = c / ny, ly = c - lx * ny;
-    const int ox = lx + q % 3 - 1;
-    if (ox >= 0 && ox < nx) {
-      const Owner<double> oi = owner_at(A, i);
-      const double ri = A.radius[i];
-      const int g0 = d.c0 + ox * ny + ly - 1;
-      const int n[3] = {ly - 1 >= 0 ? W.count[g0] : 0, W.count[g0 + 1],
-                        ly + 1 < ny ? W.count[g0 + 2] : 0};
-      row_vjp(A, W, J, g0, n, q / 3, m, d.a0, d.a1, i, xi, vi, ri, oi, lfi, acc);
-    }
+    stencil_vjp(A, W, J, d.c0, nx, ny, lx, ly, q, L, d.a0, d.a1, i, xi, vi, A.radius[i],
+                owner_at(A, i), lfi, acc);
   }
   // sums over the L lanes (butterfly in a fixed order: deterministic)
   for (int mask = 1; mask < L; mask <<= 1) {
@@ -181,7 +210,7 @@ __global__ void __launch_bounds__(256) k_adjoint(Agents<double> A, Work W, Batch
   double sum[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) sum[k] = acc[k];
-  if (!have || q != 3 * m) return;
+  if (!have || q != 0) return;
   double2 lv = J.lam_vt[i];
   double g_v0 = 0.0;
   if (flags & CD_USE_ADJUST) {
@@ -249,7 +278,8 @@ int cd_adjoint_step(const cd_agents_t *a, const cd_step_params_t *p, const doubl
     if ((rc = launch_reduce(A, W, B, L.blocks_f, true, s))) return rc;
     hipLaunchKernelGGL(k_bin_insert, dim3(L.blocks_f), dim3(kBS), 0, s, (const double2 *)A.pos,
                        p->cell_size, W, B, 1);
-    hipLaunchKernelGGL(k_cell_gather<double>, dim3(grid_for(W.cap, kBS)), dim3(kBS), 0, s, A, W);
+    hipLaunchKernelGGL(k_cell_gather<double>, dim3(grid_for((int64_t)W.cap * 64, kBS)),
+                       dim3(kBS), 0, s, A, W);
     if ((rc = check_launch())) return rc;
   }
   hipLaunchKernelGGL(k_adjoint, dim3(L.blocks_e), dim3(L.threads_e), 0, s, A, W, B, J,

@@ -645,65 +645,63 @@ __global__ void __launch_bounds__(256) k_bin_insert(const double2 *pos, double c
   bin_scenario(pos, p0, cell, W, d, ss, sk, r, binning);
 }
 
-// K_b: one thread per cell of the workspace grid.  Orders the cell's slot
-// list by agent index and gathers the neighbour data; cells with more agents
-// than slots are flagged (the force kernel visits their members through the
-// exact slow path).
+// K_b: one wave per cell of the workspace grid.  Orders the cell's slot list
+// by agent index and gathers the neighbour data in that order.  Up to 64
+// agents, each lane takes one id and finds its rank among the cell's ids by
+// shuffles (ids are distinct, so the ranks are a permutation); larger cells
+// are ordered by the wave's first lane.  Cells with more agents than slots
+// are flagged (the adjoint visits their members through the exact slow
+// path).
+template <typename T>
+__device__ __forceinline__ void gather_slot(const Agents<T> &A, const Work &W, size_t e, int i) {
+  V2<T> *e_vel = (V2<T> *)W.e_vel;
+  W.e_pos[e] = A.pos[i];
+  e_vel[e] = A.vel[i];
+  W.e_r[e] = A.radius[i];
+  if (W.e_ls) {
+    W.e_ls[e] = A.ls[i];
+    W.e_rs[e] = A.rs[i];
+    W.e_phi[e] = A.phi[i];
+    W.e_rt[e] = A.r_t[i];
+    W.e_rs_r[e] = A.r_s[i];
+    W.e_rts[e] = A.r_ts[i];
+  }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_cell_gather(Agents<T> A, Work W) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= W.cap) return;
+  const int lane = threadIdx.x & 63;
+  const int c = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
+  if (c >= W.cap) return;  // whole waves
   const int cnt = W.count[c];
   if (cnt == 0) return;
   if (cnt > W.K) {
-    atomicOr((unsigned long long *)&W.st->status, (unsigned long long)CD_STATUS_SLOTS_FULL);
+    if (lane == 0)
+      atomicOr((unsigned long long *)&W.st->status, (unsigned long long)CD_STATUS_SLOTS_FULL);
     return;
   }
   int *ids = W.ell_ids + (size_t)c * W.K;
-  if (cnt <= 8) {
-    // odd-even transposition network on registers (padding sorts last)
-    int v[8];
-#pragma unroll
-    for (int s = 0; s < 8; ++s) v[s] = s < cnt ? ids[s] : 0x7fffffff;
-#pragma unroll
-    for (int ph = 0; ph < 8; ++ph) {
-#pragma unroll
-      for (int s = ph & 1; s + 1 < 8; s += 2) {
-        const int lo = min(v[s], v[s + 1]), hi = max(v[s], v[s + 1]);
-        v[s] = lo;
-        v[s + 1] = hi;
-      }
+  if (cnt <= 64) {
+    const int id = lane < cnt ? ids[lane] : 0x7fffffff;
+    int rank = 0;
+    for (int t = 0; t < cnt; ++t) rank += __shfl(id, t, 64) < id ? 1 : 0;
+    if (lane < cnt) {
+      ids[rank] = id;
+      gather_slot(A, W, (size_t)c * W.K + rank, id);
     }
-#pragma unroll
-    for (int s = 0; s < 8; ++s)
-      if (s < cnt) ids[s] = v[s];
-  } else {
-    for (int s = 1; s < cnt; ++s) {  // insertion sort of a crowded cell
-      const int key = ids[s];
-      int t = s - 1;
-      while (t >= 0 && ids[t] > key) {
-        ids[t + 1] = ids[t];
-        --t;
-      }
-      ids[t + 1] = key;
-    }
+    return;
   }
-  V2<T> *e_vel = (V2<T> *)W.e_vel;
-  for (int s = 0; s < cnt; ++s) {
-    const int i = ids[s];
-    const size_t e = (size_t)c * W.K + s;
-    W.e_pos[e] = A.pos[i];
-    e_vel[e] = A.vel[i];
-    W.e_r[e] = A.radius[i];
-    if (W.e_ls) {
-      W.e_ls[e] = A.ls[i];
-      W.e_rs[e] = A.rs[i];
-      W.e_phi[e] = A.phi[i];
-      W.e_rt[e] = A.r_t[i];
-      W.e_rs_r[e] = A.r_s[i];
-      W.e_rts[e] = A.r_ts[i];
+  if (lane != 0) return;
+  for (int s = 1; s < cnt; ++s) {  // insertion sort of a crowded cell
+    const int key = ids[s];
+    int t = s - 1;
+    while (t >= 0 && ids[t] > key) {
+      ids[t + 1] = ids[t];
+      --t;
     }
+    ids[t + 1] = key;
   }
+  for (int s = 0; s < cnt; ++s) gather_slot(A, W, (size_t)c * W.K + s, ids[s]);
 }
 
 // ---------------------------------------------------------------------------

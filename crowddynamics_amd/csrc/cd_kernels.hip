@@ -1009,8 +1009,9 @@ struct MemRow {
       return;
     }
     // a cell holding more agents than slots: members found by scanning
-    // cell_of, the candidates numbered cell by cell
-#pragma unroll
+    // cell_of, the candidates numbered cell by cell (not unrolled: one copy
+    // of the pair code)
+#pragma unroll 1
     for (int k = 0; k < 9; ++k) {
       const int cnt = n[k];
       const int s0 = max(0, b0 - pre[k]), s1 = min(cnt, b1 - pre[k]);
