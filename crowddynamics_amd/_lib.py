@@ -112,6 +112,7 @@ SIGNATURES = {
     'cd_reset': (C.c_int, [_P(CDAgents), _vp]),
     'cd_step_prepare': (C.c_int, [_P(CDAgents), _P(CDStepParams), _W, C.c_uint64, _vp]),
     'cd_step': (C.c_int, [_P(CDAgents), _P(CDStepParams), _W, _vp]),
+    'cd_step_tape': (C.c_int, [_P(CDAgents), _P(CDStepParams), _W, _vp, _vp, _vp]),
     'cd_batch_prepare': (C.c_int, [_P(CDAgents), _P(CDStepParams), _P(CDScenario), C.c_int32,
                                    _W, C.c_uint64, _vp]),
     'cd_batch_step': (C.c_int, [_P(CDAgents), _P(CDStepParams), _W, _vp]),

@@ -308,8 +308,8 @@ int cd_rollout_record(const cd_agents_t *a, const cd_step_params_t *p, const cd_
           !copy(ck_fprev + c, a->force_prev))
         return CD_ERR_HIP;
     }
-    if ((rc = cd_step(a, p, w, stream))) return rc;
-    if (!copy(x_out + (size_t)(k + 1) * a->n * 2, a->position)) return CD_ERR_HIP;
+    if ((rc = cd_step_tape(a, p, w, x_out + (size_t)(k + 1) * a->n * 2, nullptr, stream)))
+      return rc;
   }
   return CD_OK;
 }
@@ -337,10 +337,9 @@ int cd_adjoint_segment(const cd_agents_t *a, const cd_step_params_t *p, const cd
   // the states k0 .. k1 - 1 of the segment
   if (!copy(tape_x, a->position) || !copy(tape_v, a->velocity)) return CD_ERR_HIP;
   for (int64_t k = k0; k < k1 - 1; ++k) {
-    if ((rc = cd_step(a, p, w, stream))) return rc;
-    if (!copy(tape_x + (size_t)(k + 1 - k0) * n2, a->position) ||
-        !copy(tape_v + (size_t)(k + 1 - k0) * n2, a->velocity))
-      return CD_ERR_HIP;
+    if ((rc = cd_step_tape(a, p, w, tape_x + (size_t)(k + 1 - k0) * n2,
+                           tape_v + (size_t)(k + 1 - k0) * n2, stream)))
+      return rc;
   }
   // reverse sweep over the segment
   cd_agents_t view = *a;

@@ -525,6 +525,8 @@ struct FusedArgs {
   int64_t nav_T, nav_rows, nav_cols;
   double nav_minx, nav_miny, nav_step;
   const double4 *segs;
+  double2 *tape_x;  // when set, the step also writes its new positions here ([n])
+  void *tape_v;     // ... and its new velocities
 };
 
 constexpr int kTileThreads = 256;
@@ -1215,15 +1217,18 @@ __device__ __forceinline__ Tail<T> tail_of(const Agents<T> &A, int i) {
 // it (Verlet and Reset) and adds its new state to the block's reduction r;
 // the AgentAgentInteractions node stores the force.
 template <typename T, int MODEL, bool FUSED>
-__device__ __forceinline__ void agent_finish(const Agents<T> &A, int i, bool lead, V2<T> f,
-                                             double tq, double2 pos, V2<T> vel,
-                                             const Tail<T> &t, double dt, int reset, Red &r) {
+__device__ __forceinline__ void agent_finish(const Agents<T> &A, const FusedArgs &fa, int i,
+                                             bool lead, V2<T> f, double tq, double2 pos,
+                                             V2<T> vel, const Tail<T> &t, double dt, int reset,
+                                             Red &r) {
   if (!lead) return;
   if constexpr (FUSED) {
     double2 x;
     V2<T> v;
     verlet_with(A, MODEL, i, dt, reset != 0, f, tq, pos, vel, t.fp, t.m, x, v);
     r.add_agent(x, dhypot((double)v.x, (double)v.y), t.v0);
+    if (fa.tape_x) fa.tape_x[i] = x;
+    if (fa.tape_v) ((V2<T> *)fa.tape_v)[i] = v;
   } else {
     A.force[i] = f;
     if (MODEL) A.torque[i] = tq;
@@ -1292,7 +1297,7 @@ __global__ void __launch_bounds__(256, MODEL == 0 ? 3 : 1) k_forces_direct(Agent
   agent_forces<T, MODEL, FUSED>(fa, d, sseg, L, have, ok, i, pos, vel, me, self, rows, f, tq);
   Red r;
   r.init();
-  agent_finish<T, MODEL, FUSED>(A, i, lead, f, tq, pos, vel, tail, dt, reset, r);
+  agent_finish<T, MODEL, FUSED>(A, fa, i, lead, f, tq, pos, vel, tail, dt, reset, r);
   if constexpr (FUSED) step_end(W, sk.x, p, r, L);
 }
 
@@ -1427,7 +1432,7 @@ __device__ bool tile_region(const Agents<T> &A, const Work &W, const FusedArgs &
     V2<T> f;
     double tq;
     agent_forces<T, MODEL, FUSED>(fa, d, sseg, 4, have, true, i, pos, vel, me, self, rows, f, tq);
-    agent_finish<T, MODEL, FUSED>(A, i, lead, f, tq, pos, vel, tail, cx.dt, cx.reset, red);
+    agent_finish<T, MODEL, FUSED>(A, fa, i, lead, f, tq, pos, vel, tail, cx.dt, cx.reset, red);
   }
   __syncthreads();
   return true;
@@ -1466,7 +1471,7 @@ __device__ void cell_from_memory(const Agents<T> &A, const Work &W, const FusedA
     double tq;
     agent_forces<T, MODEL, FUSED>(fa, d, sseg, 4, have, pairs, i, pos, vel, me, self, rows, f,
                                   tq);
-    agent_finish<T, MODEL, FUSED>(A, i, lead, f, tq, pos, vel, tail, cx.dt, cx.reset, red);
+    agent_finish<T, MODEL, FUSED>(A, fa, i, lead, f, tq, pos, vel, tail, cx.dt, cx.reset, red);
   }
 }
 
@@ -1615,6 +1620,8 @@ FusedArgs fused_args(const cd_step_params_t *p) {
   fa.nav_miny = p->nav_miny;
   fa.nav_step = p->nav_step;
   fa.segs = (const double4 *)p->segments;
+  fa.tape_x = nullptr;
+  fa.tape_v = nullptr;
   return fa;
 }
 
@@ -1639,8 +1646,11 @@ void launch_step_begin(const cd_agents_t *a, const cd_step_params_t *p, const Wo
 // kernel: the step has no separate integration kernel).
 template <typename T>
 int step_kernels(const cd_agents_t *a, const cd_step_params_t *p, const Work &W, const Batch &B,
-                 const Launch &L, const cd_workspace_t *w, hipStream_t s, hipEvent_t *ev) {
-  const FusedArgs fa = fused_args(p);
+                 const Launch &L, const cd_workspace_t *w, hipStream_t s, hipEvent_t *ev,
+                 double *tape_x = nullptr, double *tape_v = nullptr) {
+  FusedArgs fa = fused_args(p);
+  fa.tape_x = (double2 *)tape_x;
+  fa.tape_v = (void *)tape_v;
   const int reset = (p->flags & CD_USE_RESET) ? 1 : 0;
   mark(ev, 0, s);
   launch_step_begin<T>(a, p, W, B, fa, L, s);
@@ -1979,8 +1989,8 @@ int cd_step_prepare(const cd_agents_t *a, const cd_step_params_t *p, const cd_wo
   return launch_reduce(typed<float>(a), W, B, L.blocks_f, true, s, slot);
 }
 
-int cd_step(const cd_agents_t *a, const cd_step_params_t *p, const cd_workspace_t *w,
-            cd_stream_t stream) {
+int cd_step_tape(const cd_agents_t *a, const cd_step_params_t *p, const cd_workspace_t *w,
+                 double *tape_x, void *tape_v, cd_stream_t stream) {
   int rc = check_step(a, p, w);
   if (rc) return rc;
   if (a->n == 0) return CD_OK;
@@ -1989,8 +1999,15 @@ int cd_step(const cd_agents_t *a, const cd_step_params_t *p, const cd_workspace_
                                p->dt_min, p->dt_max);
   const Launch L = single_launch(w);
   if (a->real == CD_F64)
-    return step_kernels<double>(a, p, W, B, L, w, (hipStream_t)stream, nullptr);
-  return step_kernels<float>(a, p, W, B, L, w, (hipStream_t)stream, nullptr);
+    return step_kernels<double>(a, p, W, B, L, w, (hipStream_t)stream, nullptr, tape_x,
+                                (double *)tape_v);
+  return step_kernels<float>(a, p, W, B, L, w, (hipStream_t)stream, nullptr, tape_x,
+                             (double *)tape_v);
+}
+
+int cd_step(const cd_agents_t *a, const cd_step_params_t *p, const cd_workspace_t *w,
+            cd_stream_t stream) {
+  return cd_step_tape(a, p, w, nullptr, nullptr, stream);
 }
 
 int cd_batch_prepare(const cd_agents_t *a, const cd_step_params_t *p,

@@ -231,6 +231,12 @@ int cd_step_prepare(const cd_agents_t *agents, const cd_step_params_t *params,
                     const cd_workspace_t *ws, uint64_t first_step, cd_stream_t stream);
 int cd_step(const cd_agents_t *agents, const cd_step_params_t *params, const cd_workspace_t *ws,
             cd_stream_t stream);
+/* cd_step that also writes the new positions to tape_x ([n][2] fp64) and,
+ * when tape_v is not null, the new velocities to tape_v ([n][2] in the
+ * agents' real type): a trajectory recorded by the step itself, with no
+ * copies.  No reference counterpart (the rollout drivers below use it). */
+int cd_step_tape(const cd_agents_t *agents, const cd_step_params_t *params,
+                 const cd_workspace_t *ws, double *tape_x, void *tape_v, cd_stream_t stream);
 
 /* Batched ensemble: independent scenarios advanced together, one launch per
  * step kernel over all scenarios (two launches per step, graph-capturable).
