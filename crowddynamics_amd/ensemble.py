@@ -1,5 +1,5 @@
 """Batched ensemble: independent scenarios of uneven agent counts advanced
-together, three kernel launches per step over all of them (capturable in a
+together, two kernel launches per step over all of them (capturable in a
 HIP graph).
 
 The reference runs one simulation per process

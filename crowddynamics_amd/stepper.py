@@ -1,4 +1,4 @@
-"""Fused multi-step driver over cd_step (three kernels per step) with HIP
+"""Fused multi-step driver over cd_step (two kernels per step) with HIP
 graph capture.
 
 ``Stepper`` runs one MultiAgentSimulation.update() of the standard logic
