@@ -373,10 +373,13 @@ __global__ void __launch_bounds__(256) k_reset(Agents<T> A, int model) {
   if (model) A.torque[i] = 0.0;
 }
 
-// Agent-side inputs of the wall contacts (loaded once per agent).
+// Agent-side inputs of the wall contacts (loaded once per agent).  reach2:
+// the square of how far from the agent's centre its body extends (radius,
+// or shoulder offset + the larger radius for three-circle agents), plus
+// 1e-6 m; a segment farther than that from the centre cannot touch it.
 template <typename T>
 struct WallSelf {
-  double r, rt, rsr;
+  double r, rt, rsr, reach2;
   double2 ls, rs;
   T mu, kappa, damping;
 };
@@ -398,6 +401,8 @@ __device__ __forceinline__ WallSelf<T> wall_self(const Agents<T> &A, int model, 
   w.mu = (T)A.mu[i];
   w.kappa = (T)A.kappa[i];
   w.damping = (T)A.damping[i];
+  const double reach = (model ? A.r_ts[i] + fmax(w.rt, w.rsr) : w.r) + 1e-6;
+  w.reach2 = reach * reach;
   return w;
 }
 
@@ -405,11 +410,23 @@ __device__ __forceinline__ WallSelf<T> wall_self(const Agents<T> &A, int model, 
 // w_begin + w_stride, ... (core/interactions.py:107-141, 169-186):
 // add(force x, force y, torque) for each contact.
 // `frm`: the segments' frames (seg_frame), or null to compute them here.
+// `box`: the segments' bounding boxes {min x, min y, max x, max y}, or null;
+// with boxes, a segment whose box lies farther than the agent's reach from
+// its centre is skipped.  Every circle of the agent is then more than 1e-6 m
+// from the segment, so the skipped contact term is exactly the zero that
+// evaluating it would give.
 template <typename T, class Add>
 __device__ __forceinline__ void walls_each(const WallSelf<T> &me, int model, double2 p, V2<T> v,
                                            const double4 *seg, const double4 *frm, int W,
-                                           Add &&add, int w_begin = 0, int w_stride = 1) {
+                                           Add &&add, int w_begin = 0, int w_stride = 1,
+                                           const double4 *box = nullptr) {
   for (int w = w_begin; w < W; w += w_stride) {
+    if (box) {
+      const double4 b = box[w];
+      const double dx = fmax(fmax(b.x - p.x, p.x - b.z), 0.0);
+      const double dy = fmax(fmax(b.y - p.y, p.y - b.w), 0.0);
+      if (dx * dx + dy * dy > me.reach2) continue;
+    }
     const double4 s = seg[w];
     const double4 fr = frm ? frm[w] : seg_frame(s.x, s.y, s.z, s.w);
     double h, nx, ny, rmx = 0.0, rmy = 0.0;
@@ -1034,23 +1051,25 @@ struct MemRow {
   }
 };
 
-// Stages cnt wall segments and their frames (seg_frame) in LDS.
-__device__ __forceinline__ void stage_segs(const double4 *src, int cnt, double4 *sseg,
-                                           double4 *sfrm) {
+// Stages cnt wall segments, their frames (seg_frame) and bounding boxes in
+// LDS: sseg holds 3 kWallTile entries, the segments, then the frames, then
+// the boxes.
+__device__ __forceinline__ void stage_segs(const double4 *src, int cnt, double4 *sseg) {
   for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
     const double4 s = src[k];
     sseg[k] = s;
-    sfrm[k] = seg_frame(s.x, s.y, s.z, s.w);
+    sseg[kWallTile + k] = seg_frame(s.x, s.y, s.z, s.w);
+    sseg[2 * kWallTile + k] =
+        make_double4(fmin(s.x, s.z), fmin(s.y, s.w), fmax(s.x, s.z), fmax(s.y, s.w));
   }
 }
 
 // Stages the scenario's first kWallTile wall segments (before the per-agent
-// work, so the loads overlap it).  sseg holds 2 kWallTile entries: the
-// segments, then their frames.
+// work, so the loads overlap it).
 __device__ __forceinline__ void stage_walls(const FusedArgs &fa, const ScnDesc &d, double4 *sseg,
                                             bool fused) {
   if (!fused || !(fa.flags & CD_USE_OBSTACLE)) return;
-  stage_segs(fa.segs + d.w0, min(kWallTile, d.w1 - d.w0), sseg, sseg + kWallTile);
+  stage_segs(fa.segs + d.w0, min(kWallTile, d.w1 - d.w0), sseg);
 }
 
 __device__ __forceinline__ Fix128 shfl_xor(const Fix128 &v, int mask) {
@@ -1169,14 +1188,15 @@ __device__ __forceinline__ void agent_forces(const FusedArgs &fa, const ScnDesc 
       if (MODEL) ps.m.add(mm);
     };
     const int nw = d.w1 - d.w0;
-    const double4 *sfrm = sseg + kWallTile;
-    if (have) walls_each(self.w, MODEL, pos, vel, sseg, sfrm, min(kWallTile, nw), add, q, L);
+    const double4 *sfrm = sseg + kWallTile, *sbox = sseg + 2 * kWallTile;
+    if (have)
+      walls_each(self.w, MODEL, pos, vel, sseg, sfrm, min(kWallTile, nw), add, q, L, sbox);
     for (int w0 = kWallTile; w0 < nw; w0 += kWallTile) {  // more segments than staged
       const int cnt = min(kWallTile, nw - w0);
       __syncthreads();
-      stage_segs(fa.segs + d.w0 + w0, cnt, sseg, sseg + kWallTile);
+      stage_segs(fa.segs + d.w0 + w0, cnt, sseg);
       __syncthreads();
-      if (have) walls_each(self.w, MODEL, pos, vel, sseg, sfrm, cnt, add, q, L);
+      if (have) walls_each(self.w, MODEL, pos, vel, sseg, sfrm, cnt, add, q, L, sbox);
     }
     if (nw > kWallTile) {  // restore the first tile for the next agents
       __syncthreads();
@@ -1256,7 +1276,7 @@ __device__ __forceinline__ void step_end(const Work &W, int scn, int p, Red r, i
 template <typename T, int MODEL, bool FUSED>
 __global__ void __launch_bounds__(256, MODEL == 0 ? 3 : 1) k_forces_direct(Agents<T> A, Work W, Batch B,
                                                        FusedArgs fa, int L, int reset) {
-  __shared__ double4 sseg[2 * kWallTile];
+  __shared__ double4 sseg[3 * kWallTile];
   const int2 sk = block_e(B);
   const ScnDesc d = desc_of(B, sk.x);
   ScnState *ss = &B.sst[sk.x];
@@ -1483,7 +1503,7 @@ __global__ void __launch_bounds__(kTileThreads) k_forces_tiled(Agents<T> A, Work
                                                                 FusedArgs fa, TileArgs ta,
                                                                 int reset) {
   __shared__ TileLds<T, MODEL> S;
-  __shared__ double4 sseg[2 * kWallTile];
+  __shared__ double4 sseg[3 * kWallTile];
   const int2 sk = B.map_t ? B.map_t[blockIdx.x] : make_int2(0, (int)blockIdx.x);
   const ScnDesc d = desc_of(B, sk.x);
   const int nb = B.map_t ? d.nt : (int)gridDim.x;

@@ -531,3 +531,31 @@ def test_step_tiled_equals_direct(hallway, tile):
         outs.append(st.download(hallway.agents.copy()))
     for f in ('position', 'velocity', 'force_prev', 'target_direction'):
         assert np.array_equal(outs[0][f], outs[1][f]), f
+
+
+def test_step_wall_contacts(hallway, corridor_small):
+    """Agents pushed into the walls: one fused step (whose force kernel skips
+    the segments out of an agent's reach by their bounding boxes) against
+    the oracle's step, circular and three-circle."""
+    for scn in (hallway, corridor_small):
+        say('fused step with wall contacts', scn.name)
+        arr = scn.agents.copy()
+        arr['position'][:20, 1] = scn.bounds[1] + 0.1
+        if 'position_ls' in arr.dtype.names:
+            shoulders_host(arr)
+        ref = arr.copy()
+        O.run(ref, _oracle_scenario(scn, fluct=False), 1)
+        from crowddynamics_amd import _lib
+        from crowddynamics_amd.stepper import ALL_NODES
+        st = Stepper(arr.copy(), segments=scn.segments, nav=scn.nav, cell_size=scn.cell_size,
+                     dt_min=scn.dt_min, dt_max=scn.dt_max, seed=scn.seed,
+                     flags=ALL_NODES & ~_lib.CD_USE_FLUCT, bounds=scn.bounds)
+        st.step(1)
+        st.check()
+        out = st.download(arr.copy())
+        # the pushed agents are in contact: their step is dominated by the wall force
+        moved = np.hypot(*(ref['force_prev'][:20]).T)
+        assert np.all(moved > 1.0)
+        for f in ('position', 'velocity', 'force_prev'):
+            r = ref[f]
+            assert_vec_close(out[f], r, np.hypot(*r.T) + 1.0, 1e-12, '%s %s' % (scn.name, f))
