@@ -839,7 +839,9 @@ __global__ void __launch_bounds__(256) k_step_begin(Agents<T> A, Work W, Batch B
   const uint64_t step = W.st->step;
   const int p = (int)(step & 1ull);
   int *count = W.count + (size_t)p * W.cap;
-  const int stride = d.nf * blockDim.x;
+  // blocks of the scenario: its block map's, or the whole grid for a single
+  // scenario (launched as 64-thread blocks, one wave each, spread over all CUs)
+  const int stride = (B.map_f ? d.nf : (int)gridDim.x) * blockDim.x;
   const int t0 = sk.y * blockDim.x + threadIdx.x;
   const int i0 = d.a0 + t0;
   double2 p0 = make_double2(0.0, 0.0);
@@ -1655,8 +1657,11 @@ void launch_step_begin(const cd_agents_t *a, const cd_step_params_t *p, const Wo
     if constexpr (sizeof(T) == 8)
       hipLaunchKernelGGL((k_step_begin<T, 1>), dim3(L.blocks_f), dim3(kBS), 0, s, A, W, B, fa,
                          p->cell_size, binning);
-  } else {
+  } else if (B.map_f) {
     hipLaunchKernelGGL((k_step_begin<T, 0>), dim3(L.blocks_f), dim3(kBS), 0, s, A, W, B, fa,
+                       p->cell_size, binning);
+  } else {  // one scenario: one-wave blocks, so the waves spread over all CUs
+    hipLaunchKernelGGL((k_step_begin<T, 0>), dim3(4 * L.blocks_f), dim3(64), 0, s, A, W, B, fa,
                        p->cell_size, binning);
   }
 }
