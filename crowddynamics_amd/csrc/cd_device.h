@@ -31,13 +31,25 @@ __device__ __forceinline__ double dsqrt(double a) { return sqrt(a); }
 __device__ __forceinline__ float dsqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
 __device__ __forceinline__ double dexp(double a) { return exp(a); }
 __device__ __forceinline__ float dexp(float a) { return __expf(a); }
+
+// 1/b by the hardware reciprocal (v_rcp_f32, 1 ulp).  The instruction does
+// not keep subnormal inputs or results, so |b| outside [2^-64, 2^64] is
+// scaled by 2^64 or 2^-64 on the way in and again on the way out (exact
+// power-of-two scalings).  IEEE's special cases hold: 1/(+-0) = +-inf,
+// 1/inf = 0, 1/NaN = NaN, and 1/b overflows to inf where IEEE's does.
+__device__ __forceinline__ float frcp(float b) {
+  const float ab = __builtin_fabsf(b);
+  const float s = ab > 0x1p64f ? 0x1p-64f : (ab < 0x1p-64f ? 0x1p64f : 1.0f);
+  return s * __builtin_amdgcn_rcpf(b * s);
+}
+
 template <typename T>
 __device__ __forceinline__ T ddiv(T a, T b) {
   return a / b;
 }
 template <>
 __device__ __forceinline__ float ddiv<float>(float a, float b) {
-  return __fdividef(a, b);
+  return a * frcp(b);
 }
 __device__ __forceinline__ bool is_nan(double a) { return __builtin_isnan(a); }
 __device__ __forceinline__ bool is_nan(float a) { return __builtin_isnan(a); }
@@ -203,6 +215,56 @@ __device__ __forceinline__ bool pair_circular(T x, T y, T vx, T vy, T r_tot, con
       ny = ddiv(y, d);
     }
     T cx, cy;
+    force_contact(h, nx, ny, vx, vy, ny, -nx, o.mu, o.kappa, o.damping, cx, cy);
+    sx += cx;
+    sy += cy;
+  }
+  fx = sx;
+  fy = sy;
+  return true;
+}
+
+// fp32 mode of the same pair: the same terms and the same sight and tau
+// gates on the same distance, with the divisions by a, ds, tau and tau_0
+// taken as products with one shared reciprocal each (frcp), so a pair costs
+// three reciprocals instead of eleven divisions.  Each such product is
+// within 2 ulp of the quotient it replaces.
+template <>
+__device__ __forceinline__ bool pair_circular<float>(float x, float y, float vx, float vy,
+                                                     float r_tot, const Owner<float> &o,
+                                                     float &fx, float &fy) {
+  const float d2 = x * x + y * y;
+  const float d = __builtin_amdgcn_sqrtf(d2);
+  const float h = d - r_tot;
+  if (!(h < float(kSightSoc))) return false;
+  float sx = 0.0f, sy = 0.0f;
+  const float a = vx * vx + vy * vy;
+  const float b = -(x * vx + y * vy);
+  const float c = d2 - r_tot * r_tot;
+  const float ds = __builtin_amdgcn_sqrtf(b * b - a * c);
+  if (!(is_nan(ds) || ds == 0.0f || a == 0.0f)) {
+    const float ra = frcp(a);
+    const float tau = (b - ds) * ra;
+    if (!(tau <= 0.0f || tau > float(kTauMax))) {
+      const float rds = frcp(ds);
+      const float gx = (vx - (vx * b + x * a) * rds) * ra;
+      const float gy = (vy - (vy * b + y * a) * rds) * ra;
+      const float rt = frcp(tau), rt0 = frcp(o.tau_0);
+      // magnitude(tau, tau_0) = (2/tau + 1/tau_0) exp(-tau/tau_0) / tau^2
+      const float m = ((2.0f * rt + rt0) * __expf(-tau * rt0)) * (rt * rt);
+      sx = (o.neg_mk * gx) * m;
+      sy = (o.neg_mk * gy) * m;
+      truncate(sx, sy, float(kFSocMax));
+    }
+  }
+  if (h < 0.0f) {
+    float nx = 0.0f, ny = 0.0f;
+    if (d != 0.0f) {
+      const float rd = frcp(d);
+      nx = x * rd;
+      ny = y * rd;
+    }
+    float cx, cy;
     force_contact(h, nx, ny, vx, vy, ny, -nx, o.mu, o.kappa, o.damping, cx, cy);
     sx += cx;
     sy += cy;
