@@ -142,12 +142,13 @@ def bench_ensemble(n_scenarios, steps, loop_steps, stream, tune=True):
         if tune:
             tuned = tune_ensemble(scns, precision='f32', cell_sizes=sizes, steps=5, warmup=2)
         else:
-            tuned = dict(cell_size=sizes[0], tile=(0, 0), lanes=4, ms_per_step=float('nan'),
-                         table=[])
-        log('autotune: cell %.3f, tile %s, lanes %d (%.3f ms/step)' % (
-            tuned['cell_size'], tuned['tile'], tuned['lanes'], tuned['ms_per_step']))
+            tuned = dict(cell_size=sizes[0], tile=(0, 0), lanes=4, order_every=10,
+                         ms_per_step=float('nan'), table=[])
+        log('autotune: cell %.3f, tile %s, lanes %d, order_every %d (%.3f ms/step)' % (
+            tuned['cell_size'], tuned['tile'], tuned['lanes'], tuned['order_every'],
+            tuned['ms_per_step']))
         ens = Ensemble(scns, precision='f32', cell_size=tuned['cell_size'], tile=tuned['tile'],
-                       lanes=tuned['lanes'])
+                       lanes=tuned['lanes'], order_every=tuned['order_every'])
         ens.step(10)
         G = 50
         graph = ens.capture(G)
@@ -187,6 +188,7 @@ def bench_ensemble(n_scenarios, steps, loop_steps, stream, tune=True):
                 kernel_ms={k: kms[i] / 20 for k, i in KERNELS.items()},
                 autotuned=dict(cell_size=tuned['cell_size'],
                                tile=tuned['tile'], lanes=tuned['lanes'],
+                               order_every=tuned['order_every'],
                                candidates=[dict(ms=t[0], cell_size=t[1], kernel=t[2])
                                            for t in tuned['table']]),
                 per_scenario_loop=dict(value=loop, steps=loop_steps,

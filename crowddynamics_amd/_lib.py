@@ -55,7 +55,8 @@ class CDWorkspace(C.Structure):
                 ('max_scenarios', C.c_int32), ('n_scenarios', C.c_int32),
                 ('blocks_f', C.c_int32), ('blocks_e', C.c_int32),
                 ('force_threads', C.c_int32), ('tile_x', C.c_int32), ('tile_y', C.c_int32),
-                ('blocks_t', C.c_int32), ('agent_lanes', C.c_int32)]
+                ('blocks_t', C.c_int32), ('agent_lanes', C.c_int32),
+                ('order_every', C.c_int32)]
 
 
 class CDScenario(C.Structure):

@@ -33,17 +33,18 @@ def neutral_cell_sizes(r_max, factors=(1.0, 1.15, 1.35, 1.6), sight=SIGHT_SOC):
 
 
 def tune_ensemble(scenarios, precision='f32', cell_sizes=None,
-                  kernels=((0, 0, 4), (0, 0, 8), (1, 1, 4), (2, 2, 4)), steps=10, warmup=3,
-                  flags=None):
+                  kernels=((0, 0, 4), (0, 0, 8), (0, 0, 4, 10), (0, 0, 8, 10), (1, 1, 4),
+                           (2, 2, 4)), steps=10, warmup=3, flags=None):
     """Time every (cell size, force kernel) pair over `steps` batched steps
     (after `warmup` steps) and return the fastest.
 
-    A force kernel is (tile_x, tile_y, lanes): tile (0, 0) selects the
-    agent-parallel kernel with `lanes` lanes per agent, otherwise LDS tiles of
-    tile_x * tile_y cells.  The choice never changes results (pair and wall
-    terms are summed exactly).  Returns dict(cell_size, tile, lanes,
-    ms_per_step, table) where table lists (ms_per_step, cell_size, kernel,
-    per-kernel ms)."""
+    A force kernel is (tile_x, tile_y, lanes[, order_every]): tile (0, 0)
+    selects the agent-parallel kernel with `lanes` lanes per agent, serving
+    the agents in index order, or in cell order rebuilt every `order_every`
+    steps; otherwise LDS tiles of tile_x * tile_y cells.  The choice never
+    changes results (pair and wall terms are summed exactly).  Returns
+    dict(cell_size, tile, lanes, order_every, ms_per_step, table) where table
+    lists (ms_per_step, cell_size, kernel, per-kernel ms)."""
     if cell_sizes is None:
         cell_sizes = neutral_cell_sizes(max_body_radius(scenarios))
     kw = {} if flags is None else dict(flags=flags)
@@ -51,14 +52,15 @@ def tune_ensemble(scenarios, precision='f32', cell_sizes=None,
     for cs in cell_sizes:
         for kern in kernels:
             ens = Ensemble(scenarios, precision=precision, cell_size=cs, tile=kern[:2],
-                           lanes=kern[2], **kw)
+                           lanes=kern[2], order_every=kern[3] if len(kern) > 3 else 0, **kw)
             ens.step(warmup)
             ms = ens.timed(steps)
             ens.check()
             table.append((ms[3] / steps, cs, tuple(kern), [m / steps for m in ms]))
             del ens
     best = min(table, key=lambda t: t[0])
-    return dict(cell_size=best[1], tile=best[2][:2], lanes=best[2][2], ms_per_step=best[0],
+    return dict(cell_size=best[1], tile=best[2][:2], lanes=best[2][2],
+                order_every=best[2][3] if len(best[2]) > 3 else 0, ms_per_step=best[0],
                 table=table)
 
 

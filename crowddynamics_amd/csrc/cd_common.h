@@ -102,6 +102,7 @@ struct Layout {
   int64_t cell_of, rank, pidx, count, offset;
   int64_t ell_ids, e_pos, e_vel, e_r, e_ls, e_rs, e_phi, e_rt, e_rs_r, e_rts;
   int64_t snap_pos, snap_vel, snap_ls, snap_rs, snap_phi;
+  int64_t order, order_off;
   int64_t total;
 };
 
@@ -155,6 +156,10 @@ Layout make_layout(const cd_workspace_t *w) {
   } else {
     L.snap_ls = L.snap_rs = L.snap_phi = -1;
   }
+  // processing order of the agent-parallel force kernel and the cell
+  // offsets it is built from
+  L.order = take(4 * n);
+  L.order_off = take(4 * cap);
   L.total = o;
   return L;
 }
@@ -225,7 +230,9 @@ struct Work {
   void *snap_vel;
   double2 *snap_ls, *snap_rs;
   double *snap_phi;
+  int *order, *order_off;  // force kernel's processing order (cell order when order_every > 0)
   int cap, K, ntiles;
+  int order_every;
 };
 
 // Scenario view of a launch: the table (null for a single scenario, which is
@@ -266,6 +273,10 @@ Work carve(const cd_workspace_t *w) {
   W.snap_ls = L.snap_ls >= 0 ? (double2 *)(b + L.snap_ls) : nullptr;
   W.snap_rs = L.snap_rs >= 0 ? (double2 *)(b + L.snap_rs) : nullptr;
   W.snap_phi = L.snap_phi >= 0 ? (double *)(b + L.snap_phi) : nullptr;
+  W.order = (int *)(b + L.order);
+  W.order_off = (int *)(b + L.order_off);
+  // only the agent-parallel force kernel follows the order
+  W.order_every = (w->tile_x > 0 && w->tile_y > 0) ? 0 : (int)w->order_every;
   W.cap = (int)w->cell_capacity;
   W.K = (int)w->cell_slots;
   W.ntiles = (int)((w->cell_capacity + kScanTile - 1) / kScanTile);
@@ -740,6 +751,7 @@ bool ws_sizes_ok(const cd_workspace_t *w) {
          w->tile_x >= 0 && w->tile_x <= 16 && w->tile_y >= 0 && w->tile_y <= 16 &&
          (w->agent_lanes == 0 || w->agent_lanes == 4 || w->agent_lanes == 8 ||
           w->agent_lanes == 16) &&
+         w->order_every >= 0 &&
          (w->model == CD_CIRCULAR || w->model == CD_THREE_CIRCLE);
 }
 
@@ -769,7 +781,7 @@ inline void mark(hipEvent_t *ev, int k, hipStream_t s) {
 
 // Grid sizes of one step: per-scenario kernels (f) and force kernel (e).
 struct Launch {
-  int blocks_f, blocks_e, threads_e, blocks_t, lanes;
+  int blocks_f, blocks_e, threads_e, blocks_t, lanes, scenarios;
 };
 
 Launch single_launch(const cd_workspace_t *w) {
@@ -779,6 +791,7 @@ Launch single_launch(const cd_workspace_t *w) {
   L.blocks_e = std::max(1, grid_for(w->n, force_threads(w) / L.lanes));
   L.threads_e = force_threads(w);
   L.blocks_t = tile_blocks(w->cell_capacity, w);
+  L.scenarios = 1;
   return L;
 }
 

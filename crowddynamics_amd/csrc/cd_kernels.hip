@@ -870,6 +870,8 @@ __global__ void __launch_bounds__(256) k_step_begin(Agents<T> A, Work W, Batch B
     ss->time_tot += dt;
   }
   const bool bin = binning && g.ok;
+  // steps that rebuild the force kernel's processing order keep the slot ranks
+  const bool ranks = W.order_every > 0 && step % (uint64_t)W.order_every == 0ull;
   V2<T> *snap_vel = (V2<T> *)W.snap_vel;
   auto serve = [&](int i, double2 pos, const OwnIn<T> &o) {
     W.snap_pos[i] = pos;
@@ -889,6 +891,7 @@ __global__ void __launch_bounds__(256) k_step_begin(Agents<T> A, Work W, Batch B
         c = d.c0 + lc;
         W.cell_of[i] = c;
         k = atomicAdd(&count[c], 1);
+        if (ranks) W.rank[i] = k;
       }
     }
     NavFetch nf;
@@ -907,6 +910,67 @@ __global__ void __launch_bounds__(256) k_step_begin(Agents<T> A, Work W, Batch B
     const double2 pos = A.pos[i];
     serve(i, pos, own_load<T, MODEL>(A, fa, i, pos));
   }
+}
+
+// Processing order of the agent-parallel force kernel: each scenario's
+// agents by cell, a counting sort of the step's binning (exclusive scan of
+// the scenario's cell counts, then each agent at its cell's offset plus its
+// slot rank).  A wave's agents then share their neighbour cells, so the
+// lanes that walk the same part of the stencil load the same candidates.
+// One block per scenario, after K_a, on steps whose counter is a multiple
+// of order_every.  A scenario whose binning is not valid, or whose counts do
+// not add up to its agents, keeps its previous order: the order always
+// holds each scenario's agents once.
+constexpr int kOrderThreads = 1024;
+__global__ void __launch_bounds__(kOrderThreads) k_order_build(Work W, Batch B) {
+  __shared__ int s_wave[kOrderThreads / 64];
+  if (W.st->step % (uint64_t)W.order_every != 0ull) return;
+  const ScnDesc d = desc_of(B, (int)blockIdx.x);
+  const ScnState *ss = &B.sst[blockIdx.x];
+  const int p = ss->par;
+  const int n = d.a1 - d.a0;
+  if (!ss->valid || ss->oob[p] || n <= 0) return;
+  const int C = ss->C;
+  const int *count = W.count + (size_t)p * W.cap + d.c0;
+  int *off = W.order_off + d.c0;
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  const int per = (C + kOrderThreads - 1) / kOrderThreads;
+  const int k0 = min(C, t * per), k1 = min(C, k0 + per);
+  int sum = 0;
+  for (int k = k0; k < k1; ++k) sum += count[k];
+  int inc = sum;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += u;
+  }
+  if (lane == 63) s_wave[wid] = inc;
+  __syncthreads();
+  int before = 0, total = 0;
+#pragma unroll
+  for (int w = 0; w < kOrderThreads / 64; ++w) {
+    if (w < wid) before += s_wave[w];
+    total += s_wave[w];
+  }
+  if (total != n) return;  // the same for every thread of the block
+  int run = before + inc - sum;
+  for (int k = k0; k < k1; ++k) {
+    off[k] = run;
+    run += count[k];
+  }
+  __syncthreads();  // offsets visible to the block
+  for (int i = d.a0 + t; i < d.a1; i += kOrderThreads) {
+    const int k = W.cell_of[i] - d.c0;
+    if (k < 0 || k >= C) continue;
+    const int e = off[k] + W.rank[i];
+    if (e >= 0 && e < n) W.order[d.a0 + e] = i;
+  }
+}
+
+// Identity processing order (preparation).
+__global__ void __launch_bounds__(256) k_order_identity(int *order, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    order[i] = i;
 }
 
 // Neighbour sources.  each(q, L, self, fn) visits part q (of L equal
@@ -1277,7 +1341,8 @@ __device__ __forceinline__ void step_end(const Work &W, int scn, int p, Red r, i
 // they are integrated; without a valid binning only the walls are added.
 template <typename T, int MODEL, bool FUSED>
 __global__ void __launch_bounds__(256, MODEL == 0 ? 3 : 1) k_forces_direct(Agents<T> A, Work W, Batch B,
-                                                       FusedArgs fa, int L, int reset) {
+                                                       FusedArgs fa, int L, int reset,
+                                                       int ordered) {
   __shared__ double4 sseg[3 * kWallTile];
   const int2 sk = block_e(B);
   const ScnDesc d = desc_of(B, sk.x);
@@ -1289,8 +1354,12 @@ __global__ void __launch_bounds__(256, MODEL == 0 ? 3 : 1) k_forces_direct(Agent
   const double dt = FUSED ? ss->dt : 0.0;
   stage_walls(fa, d, sseg, FUSED);
   const int per = (int)blockDim.x / L;
-  const int i = d.a0 + sk.y * per + (int)threadIdx.x / L;
-  const bool have = i < d.a1;
+  // the agent: the next in index order, or in the processing order (an
+  // entry outside the scenario, which a valid order never holds, serves none)
+  const int local = sk.y * per + (int)threadIdx.x / L;
+  int i = d.a0 + local;
+  if (ordered && i < d.a1) i = W.order[i];
+  const bool have = i >= d.a0 && i < d.a1;
   const bool lead = have && (threadIdx.x & (L - 1)) == 0;
   double2 pos = make_double2(0.0, 0.0);
   V2<T> vel;
@@ -1594,7 +1663,7 @@ void launch_forces_m(const Agents<T> &A, const Work &W, const Batch &B, const Fu
                        s, A, W, B, fa, ta, reset);
   } else {
     hipLaunchKernelGGL((k_forces_direct<T, MODEL, FUSED>), dim3(L.blocks_e), dim3(L.threads_e), 0,
-                       s, A, W, B, fa, L.lanes, reset);
+                       s, A, W, B, fa, L.lanes, reset, (FUSED && W.order_every > 0) ? 1 : 0);
   }
 }
 
@@ -1627,6 +1696,7 @@ Launch batch_launch(const cd_workspace_t *w) {
   L.blocks_f = w->blocks_f;
   L.blocks_e = std::max(1, (int)w->blocks_e);
   L.blocks_t = std::max(1, (int)w->blocks_t);
+  L.scenarios = std::max(1, (int)w->n_scenarios);
   return L;
 }
 
@@ -1664,6 +1734,8 @@ void launch_step_begin(const cd_agents_t *a, const cd_step_params_t *p, const Wo
     hipLaunchKernelGGL((k_step_begin<T, 0>), dim3(4 * L.blocks_f), dim3(64), 0, s, A, W, B, fa,
                        p->cell_size, binning);
   }
+  if (W.order_every > 0 && binning)
+    hipLaunchKernelGGL(k_order_build, dim3(L.scenarios), dim3(kOrderThreads), 0, s, W, B);
 }
 
 // One step: K_a, then the force kernel (forces and integration); events
@@ -2006,6 +2078,8 @@ int cd_step_prepare(const cd_agents_t *a, const cd_step_params_t *p, const cd_wo
   hipLaunchKernelGGL(k_set_step, dim3(1), dim3(1), 0, s, W.st, (unsigned long long)first_step);
   if ((rc = check_launch())) return rc;
   if (a->n == 0) return CD_OK;
+  hipLaunchKernelGGL(k_order_identity, dim3(std::min(grid_for(a->n, kBS), 1024)), dim3(kBS), 0, s,
+                     W.order, (int)a->n);
   const Batch B = single_batch(w, p->n_segments, p->seed, p->dt_min, p->dt_max);
   const Launch L = single_launch(w);
   const int slot = (int)(first_step & 1ull);
@@ -2064,6 +2138,9 @@ int cd_batch_prepare(const cd_agents_t *a, const cd_step_params_t *p,
   const Work W = carve(w);
   const Batch B = batch_of(w);
   hipLaunchKernelGGL(k_set_step, dim3(1), dim3(1), 0, s, W.st, (unsigned long long)first_step);
+  if (a->n > 0)
+    hipLaunchKernelGGL(k_order_identity, dim3(std::min(grid_for(a->n, kBS), 1024)), dim3(kBS), 0,
+                       s, W.order, (int)a->n);
   const int slot = (int)(first_step & 1ull);
   hipLaunchKernelGGL(k_scn_prepare, dim3(grid_for((int64_t)n_scenarios * kAccSub, kBS)),
                      dim3(kBS), 0, s, W, B.sst, (int)n_scenarios, slot);

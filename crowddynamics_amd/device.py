@@ -121,7 +121,7 @@ class Workspace:
     """Caller-owned workspace of the C ABI (slot lists, block lists, state)."""
 
     def __init__(self, n, cell_capacity, model, cell_slots=16, device=None, stream=None,
-                 max_scenarios=0, force_threads=0, tile=None, lanes=0):
+                 max_scenarios=0, force_threads=0, tile=None, lanes=0, order_every=0):
         require_gpu()
         L = _lib.lib()
         self.desc = _lib.CDWorkspace()
@@ -135,6 +135,7 @@ class Workspace:
         if tile is not None:
             d.tile_x, d.tile_y = int(tile[0]), int(tile[1])
         d.agent_lanes = int(lanes)
+        d.order_every = int(order_every)
         self.layout = _lib.CDLayout()
         nbytes = L.cd_workspace_bytes(C.byref(d), C.byref(self.layout))
         if nbytes <= 0:

@@ -47,7 +47,8 @@ class Ensemble(Stepper):
     _entry = 'cd_batch_step'
 
     def __init__(self, scenarios, precision='f32', flags=ALL_NODES, cell_size=None,
-                 cell_slots=None, tile=None, lanes=0, first_step=0, device=None, margin=0.5):
+                 cell_slots=None, tile=None, lanes=0, order_every=0, first_step=0, device=None,
+                 margin=0.5):
         scenarios = list(scenarios)
         if not scenarios:
             raise InvalidValue('an ensemble needs at least one scenario')
@@ -100,7 +101,7 @@ class Ensemble(Stepper):
                              for s in scenarios)
         self.ws = Workspace(self.n, int(self.cell_offsets[-1]), self.agents.model,
                             cell_slots=cell_slots, device=dev, max_scenarios=self.S,
-                            tile=tile, lanes=lanes)
+                            tile=tile, lanes=lanes, order_every=order_every)
         self.table = (_lib.CDScenario * self.S)()
         for k, s in enumerate(scenarios):
             t = self.table[k]

@@ -56,7 +56,7 @@ class Stepper:
     def __init__(self, array, segments=None, nav=None, cell_size=3.6, dt_min=0.01,
                  dt_max=0.01, seed=1, flags=ALL_NODES, precision='f64', cell_capacity=None,
                  bounds=None, first_step=0, device=None, device_agents=None,
-                 cell_slots=None, force_threads=0, tile=None, lanes=0):
+                 cell_slots=None, force_threads=0, tile=None, lanes=0, order_every=0):
         if device_agents is not None:
             self.agents = device_agents
         else:
@@ -86,7 +86,8 @@ class Stepper:
         if cell_slots is None:
             cell_slots = cell_slots_for(cell_size, min_body_radius(array))
         self.ws = Workspace(self.n, cell_capacity, self.agents.model, cell_slots=cell_slots,
-                            device=dev, force_threads=force_threads, tile=tile, lanes=lanes)
+                            device=dev, force_threads=force_threads, tile=tile, lanes=lanes,
+                            order_every=order_every)
         self.params = make_params(cell_size, dt_min, dt_max, flags, seed, self.segments,
                                   self.maps, nav)
         self.flags = flags

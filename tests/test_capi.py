@@ -41,6 +41,11 @@ def test_workspace_sizing_is_host_only():
     assert L.cd_workspace_bytes(ctypes.byref(w), None) < 0
     w.n, w.cell_slots = 10, 0
     assert L.cd_workspace_bytes(ctypes.byref(w), None) < 0
+    w.cell_slots, w.order_every = 12, 7   # the processing order adds 4 (n + cells) bytes
+    m = L.cd_workspace_bytes(ctypes.byref(w), None)
+    assert m > 0
+    w.order_every = -1
+    assert L.cd_workspace_bytes(ctypes.byref(w), None) < 0
     assert L.cd_version().startswith(b'cdamd')
 
 

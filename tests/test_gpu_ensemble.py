@@ -83,17 +83,21 @@ def test_ensemble_equals_standalone(precision):
 def test_ensemble_graph_replay_and_tiles(cell_size):
     """Graph replays equal eager batched steps, and the force kernel changes
     nothing, bit for bit: the agent-parallel kernel with 4, 8 or 16 lanes per
-    agent and LDS tiles of 1 x 1 up to 16 x 16 cells (at 8 m cells the large tiles exceed the LDS
+    agent, in index order or in cell order (rebuilt every step or every third),
+    and LDS tiles of 1 x 1 up to 16 x 16 cells (at 8 m cells the large tiles exceed the LDS
     capacity and fall back to 3 x 3 regions)."""
     scns = _small_ensemble(S=5, seed=11)
     stream = torch.cuda.Stream()
     res = []
-    for tile, lanes, graph in (((0, 0), 4, False), ((0, 0), 4, True), ((0, 0), 8, False),
-                               ((0, 0), 16, True), ((1, 1), 4, False), ((3, 5), 4, True),
-                               ((16, 16), 4, False)):
-        say('tile', tile, 'lanes', lanes, 'graph', graph, 'cell', cell_size)
+    for tile, lanes, graph, every in (((0, 0), 4, False, 0), ((0, 0), 4, True, 0),
+                                      ((0, 0), 8, False, 0), ((0, 0), 16, True, 0),
+                                      ((0, 0), 4, False, 1), ((0, 0), 8, True, 3),
+                                      ((1, 1), 4, False, 0), ((3, 5), 4, True, 0),
+                                      ((16, 16), 4, False, 0)):
+        say('tile', tile, 'lanes', lanes, 'graph', graph, 'order_every', every, 'cell', cell_size)
         with torch.cuda.stream(stream):
-            ens = Ensemble(scns, precision='f32', tile=tile, lanes=lanes, cell_size=cell_size)
+            ens = Ensemble(scns, precision='f32', tile=tile, lanes=lanes, cell_size=cell_size,
+                           order_every=every)
             if graph:
                 g = ens.capture(10)
                 for _ in range(3):
@@ -152,7 +156,9 @@ def test_autotuned_cell_sizes_are_physics_neutral():
             assert np.max(np.abs(a['velocity'] - b['velocity'])) <= 1e-9
     with torch.cuda.stream(stream):
         best = tune_ensemble(scns, steps=3, warmup=1, cell_sizes=sizes[:2],
-                             kernels=((0, 0, 8), (2, 2, 4)))
-    say('autotune', best['cell_size'], best['tile'], best['lanes'], best['ms_per_step'])
+                             kernels=((0, 0, 8), (0, 0, 4, 5), (2, 2, 4)))
+    say('autotune', best['cell_size'], best['tile'], best['lanes'], best['order_every'],
+        best['ms_per_step'])
     assert best['cell_size'] in sizes[:2] and best['tile'] in ((0, 0), (2, 2))
-    assert len(best['table']) == 4
+    assert best['order_every'] in (0, 5)
+    assert len(best['table']) == 6

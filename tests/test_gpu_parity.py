@@ -460,6 +460,28 @@ def test_step_room_fp64_and_fp32(room):
     assert np.median(dev) < 1e-5
 
 
+def test_step_processing_order(room, corridor_small):
+    """The agent-parallel force kernel serving agents in cell order (rebuilt
+    every step, or every fourth) equals index order bit for bit: room fp64
+    and fp32 (10 steps), three-circle corridor fp64 (5 steps)."""
+    from crowddynamics_amd.stepper import ALL_NODES
+    for scn, prec, steps in ((room, 'f64', 10), (room, 'f32', 10), (corridor_small, 'f64', 5)):
+        outs = []
+        for every in (0, 1, 4):
+            say('%s %s %d steps, order_every %d' % (scn.name, prec, steps, every))
+            st = Stepper(scn.agents.copy(), segments=scn.segments, nav=scn.nav,
+                         cell_size=scn.cell_size, dt_min=scn.dt_min, dt_max=scn.dt_max,
+                         seed=scn.seed, precision=prec, bounds=scn.bounds, flags=ALL_NODES,
+                         order_every=every)
+            st.step(steps)
+            st.check()
+            outs.append((st.download(scn.agents.copy()), st.state().time_tot))
+        for out, tt in outs[1:]:
+            assert tt == outs[0][1]
+            for f in ('position', 'velocity', 'force_prev', 'target_direction'):
+                assert np.array_equal(out[f], outs[0][0][f]), (scn.name, prec, f)
+
+
 def _pairs_off_stepper(scn, tile=None):
     from crowddynamics_amd import _lib
     from crowddynamics_amd.stepper import ALL_NODES
