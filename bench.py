@@ -124,7 +124,7 @@ def cpu_baseline(scn, seconds, threads=None):
     return scn.n * steps / el, steps, el
 
 
-def bench_ensemble(n_scenarios, steps, loop_steps, stream, tune=True):
+def bench_ensemble(n_scenarios, steps, loop_steps, stream, tune=True, order_every=10):
     """Batched ensemble of uneven rooms (fp32, fp64 positions): autotuned
     batched steps replayed from a HIP graph versus a per-scenario loop of
     plain GPU steps (one Stepper per scenario, same cell size and
@@ -142,7 +142,7 @@ def bench_ensemble(n_scenarios, steps, loop_steps, stream, tune=True):
         if tune:
             tuned = tune_ensemble(scns, precision='f32', cell_sizes=sizes, steps=5, warmup=2)
         else:
-            tuned = dict(cell_size=sizes[0], tile=(0, 0), lanes=4, order_every=10,
+            tuned = dict(cell_size=sizes[0], tile=(0, 0), lanes=4, order_every=order_every,
                          ms_per_step=float('nan'), table=[])
         log('autotune: cell %.3f, tile %s, lanes %d, order_every %d (%.3f ms/step)' % (
             tuned['cell_size'], tuned['tile'], tuned['lanes'], tuned['order_every'],
@@ -326,8 +326,11 @@ def main():
                     help='lanes per agent of the room force kernel (0: autotuned)')
     ap.add_argument('--no-tune', action='store_true',
                     help='skip the autotuners (room: agent-parallel force kernel with 8 '
-                         'lanes per agent; ensemble: 4 lanes and the smallest '
+                         'lanes per agent; ensemble: 4 lanes, cell order, and the smallest '
                          'physics-neutral cell; results are identical for every choice)')
+    ap.add_argument('--ensemble-order-every', type=int, default=10,
+                    help='with --no-tune: the ensemble force kernel serves agents in cell order '
+                         'rebuilt every N steps (0: index order)')
     ap.add_argument('--calibrate-copy', action='store_true',
                     help='first run a plain 256 MiB device copy five times (a kernel of '
                          'known traffic for scaling hardware byte counters)')
@@ -450,7 +453,8 @@ def main():
     ensemble = None
     if not args.no_ensemble:
         ensemble = bench_ensemble(args.ensemble_scenarios, args.ensemble_steps,
-                                  args.ensemble_loop_steps, stream, tune=not args.no_tune)
+                                  args.ensemble_loop_steps, stream, tune=not args.no_tune,
+                                  order_every=args.ensemble_order_every)
     corridor = None
     if not args.no_corridor:
         corridor = bench_corridor(300, stream)
