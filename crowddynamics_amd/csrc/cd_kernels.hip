@@ -1051,6 +1051,13 @@ struct MemRow {
     const int tot = pre[9];
     const int b0 = (int)(((long long)tot * q) / L), b1 = (int)(((long long)tot * (q + 1)) / L);
     if (!over) {
+      // slot-list position of candidate idx in cell k: so[k] + idx (empty
+      // cells, which hold no candidate, keep 0; every real cell * K is below
+      // 2^31 by the workspace's size check)
+      int so[9];
+#pragma unroll
+      for (int k = 0; k < 9; ++k)
+        so[k] = n[k] ? (gr[k / 3] + k % 3) * W.K - pre[k] : 0;
       for (int b = b0; b < b1; b += 4) {
         int j[4];
 #pragma unroll
@@ -1059,15 +1066,11 @@ struct MemRow {
           j[u] = -1;
           if (idx < b1) {
             // the cell holding candidate idx: the last k with pre[k] <= idx
-            int base = 0, cell = gr[0];
+            int o = so[0];
 #pragma unroll
-            for (int k = 1; k < 9; ++k) {
-              if (idx >= pre[k]) {
-                base = pre[k];
-                cell = gr[k / 3] + k % 3;
-              }
-            }
-            j[u] = W.ell_ids[(size_t)cell * W.K + (idx - base)];
+            for (int k = 1; k < 9; ++k)
+              if (idx >= pre[k]) o = so[k];
+            j[u] = W.ell_ids[o + idx];
           }
         }
         if constexpr (MODEL == 0) {

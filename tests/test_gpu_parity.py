@@ -460,21 +460,29 @@ def test_step_room_fp64_and_fp32(room):
     assert np.median(dev) < 1e-5
 
 
-def test_step_processing_order(room, corridor_small):
+def test_step_processing_order(room, corridor_small, hallway):
     """The agent-parallel force kernel serving agents in cell order (rebuilt
     every step, or every fourth) equals index order bit for bit: room fp64
-    and fp32 (10 steps), three-circle corridor fp64 (5 steps)."""
+    and fp32 (10 steps), three-circle corridor fp64 (5 steps), and the
+    hallway with 2 slots per cell, so that overfull cells take the exact slow
+    path (5 steps)."""
+    import warnings
     from crowddynamics_amd.stepper import ALL_NODES
-    for scn, prec, steps in ((room, 'f64', 10), (room, 'f32', 10), (corridor_small, 'f64', 5)):
+    for scn, prec, steps, slots in ((room, 'f64', 10, None), (room, 'f32', 10, None),
+                                    (corridor_small, 'f64', 5, None), (hallway, 'f64', 5, 2)):
         outs = []
         for every in (0, 1, 4):
-            say('%s %s %d steps, order_every %d' % (scn.name, prec, steps, every))
+            say('%s %s %d steps, order_every %d, slots %s' % (scn.name, prec, steps, every, slots))
             st = Stepper(scn.agents.copy(), segments=scn.segments, nav=scn.nav,
                          cell_size=scn.cell_size, dt_min=scn.dt_min, dt_max=scn.dt_max,
                          seed=scn.seed, precision=prec, bounds=scn.bounds, flags=ALL_NODES,
-                         order_every=every)
+                         order_every=every, cell_slots=slots)
             st.step(steps)
-            st.check()
+            with warnings.catch_warnings(record=True) as caught:
+                warnings.simplefilter('always')
+                st.check()
+            if slots is not None:  # the slow path was taken
+                assert any('slots' in str(w.message) for w in caught)
             outs.append((st.download(scn.agents.copy()), st.state().time_tot))
         for out, tt in outs[1:]:
             assert tt == outs[0][1]
