@@ -253,7 +253,7 @@ def bench_adjoint(n, steps, checkpoint):
 
     # lanes per agent: timed on a short rollout (forward + backward)
     lane_ms = {}
-    for lanes in (4, 8, 16):
+    for lanes in (4, 8, 16, 32, 64):
         Rt = rollout(2 * checkpoint, lanes)
         p = params(True)
         torch.cuda.synchronize()

@@ -198,17 +198,18 @@ def test_adjoint_central_fd_persistent_contacts():
 
 
 def test_adjoint_lanes_per_agent():
-    """16 lanes per agent (rows split in four parts) give the gradients of
-    4 lanes up to summation order."""
+    """16, 32 and 64 lanes per agent (the stencil's candidates split in more
+    parts) give the gradients of 4 lanes up to summation order."""
     arr = AC.packed(40, seed=8)
     out = []
-    for lanes in (4, 16):
+    for lanes in (4, 16, 32, 64):
         R = Rollout(arr, steps=20, dt=DT, cell_size=CELL, checkpoint=6, lanes=lanes)
         Xhat = _targets(R, arr, seed=4)
         out.append(_adjoint_grads(R, arr, Xhat)[1])
-    for a, b in zip(*out):
-        scale = float(torch.max(torch.abs(a))) + 1e-300
-        assert float(torch.max(torch.abs(a - b))) <= 1e-12 * scale
+    for other in out[1:]:
+        for a, b in zip(out[0], other):
+            scale = float(torch.max(torch.abs(a))) + 1e-300
+            assert float(torch.max(torch.abs(a - b))) <= 1e-12 * scale
 
 
 def test_adjoint_checkpoint_interval_is_exact():

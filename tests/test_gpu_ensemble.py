@@ -82,7 +82,7 @@ def test_ensemble_equals_standalone(precision):
 @pytest.mark.parametrize('cell_size', [3.6, 8.0])
 def test_ensemble_graph_replay_and_tiles(cell_size):
     """Graph replays equal eager batched steps, and the force kernel changes
-    nothing, bit for bit: the agent-parallel kernel with 4, 8 or 16 lanes per
+    nothing, bit for bit: the agent-parallel kernel with 4 to 64 lanes per
     agent, in index order or in cell order (rebuilt every step or every third),
     and LDS tiles of 1 x 1 up to 16 x 16 cells (at 8 m cells the large tiles exceed the LDS
     capacity and fall back to 3 x 3 regions)."""
@@ -92,6 +92,7 @@ def test_ensemble_graph_replay_and_tiles(cell_size):
     for tile, lanes, graph, every in (((0, 0), 4, False, 0), ((0, 0), 4, True, 0),
                                       ((0, 0), 8, False, 0), ((0, 0), 16, True, 0),
                                       ((0, 0), 4, False, 1), ((0, 0), 8, True, 3),
+                                      ((0, 0), 32, False, 0), ((0, 0), 64, True, 2),
                                       ((1, 1), 4, False, 0), ((3, 5), 4, True, 0),
                                       ((16, 16), 4, False, 0)):
         say('tile', tile, 'lanes', lanes, 'graph', graph, 'order_every', every, 'cell', cell_size)

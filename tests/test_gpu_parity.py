@@ -490,6 +490,25 @@ def test_step_processing_order(room, corridor_small, hallway):
                 assert np.array_equal(out[f], outs[0][0][f]), (scn.name, prec, f)
 
 
+def test_step_wide_lane_groups(hallway, corridor_small):
+    """32 and 64 lanes per agent (one agent per half wave or per wave) give
+    the 4-lane step bit for bit: hallway (circular) and corridor
+    (three-circle), 5 fused steps, fp64."""
+    for scn in (hallway, corridor_small):
+        outs = []
+        for lanes in (4, 32, 64):
+            say('%s 5 steps, %d lanes per agent' % (scn.name, lanes))
+            st = Stepper(scn.agents.copy(), segments=scn.segments, nav=scn.nav,
+                         cell_size=scn.cell_size, dt_min=scn.dt_min, dt_max=scn.dt_max,
+                         seed=scn.seed, bounds=scn.bounds, lanes=lanes)
+            st.step(5)
+            st.check()
+            outs.append(st.download(scn.agents.copy()))
+        for out in outs[1:]:
+            for f in ('position', 'velocity', 'force_prev'):
+                assert np.array_equal(out[f], outs[0][f]), (scn.name, f)
+
+
 def _pairs_off_stepper(scn, tile=None):
     from crowddynamics_amd import _lib
     from crowddynamics_amd.stepper import ALL_NODES
