@@ -43,6 +43,11 @@ __device__ __forceinline__ float frcp(float b) {
   return s * __builtin_amdgcn_rcpf(b * s);
 }
 
+// 1/b by the bare hardware reciprocal, for a b known to be a normal float
+// well inside the exponent range, where frcp's scale factor is exactly 1 and
+// the two give the same bits.
+__device__ __forceinline__ float frcp_normal(float b) { return __builtin_amdgcn_rcpf(b); }
+
 template <typename T>
 __device__ __forceinline__ T ddiv(T a, T b) {
   return a / b;
@@ -226,9 +231,13 @@ __device__ __forceinline__ bool pair_circular(T x, T y, T vx, T vy, T r_tot, con
 
 // fp32 mode of the same pair: the same terms and the same sight and tau
 // gates on the same distance, with the divisions by a, ds, tau and tau_0
-// taken as products with one shared reciprocal each (frcp), so a pair costs
-// three reciprocals instead of eleven divisions.  Each such product is
-// within 2 ulp of the quotient it replaces.
+// taken as products with one shared reciprocal each, so a pair costs three
+// reciprocals instead of eleven divisions.  Each such product is within
+// 2 ulp of the quotient it replaces.  Only a = |v|^2 can leave the normal
+// range (|v| below 2^-32 m/s), so only it takes the scaled frcp; ds and d are
+// square roots of nonzero floats (at least 2^-75), tau_0 is a positive
+// parameter, and a tau below 2^-64 makes 1/tau^2 overflow fp32 however the
+// quotient is formed.
 template <>
 __device__ __forceinline__ bool pair_circular<float>(float x, float y, float vx, float vy,
                                                      float r_tot, const Owner<float> &o,
@@ -246,10 +255,10 @@ __device__ __forceinline__ bool pair_circular<float>(float x, float y, float vx,
     const float ra = frcp(a);
     const float tau = (b - ds) * ra;
     if (!(tau <= 0.0f || tau > float(kTauMax))) {
-      const float rds = frcp(ds);
+      const float rds = frcp_normal(ds);
       const float gx = (vx - (vx * b + x * a) * rds) * ra;
       const float gy = (vy - (vy * b + y * a) * rds) * ra;
-      const float rt = frcp(tau), rt0 = frcp(o.tau_0);
+      const float rt = frcp_normal(tau), rt0 = frcp_normal(o.tau_0);
       // magnitude(tau, tau_0) = (2/tau + 1/tau_0) exp(-tau/tau_0) / tau^2
       const float m = ((2.0f * rt + rt0) * __expf(-tau * rt0)) * (rt * rt);
       sx = (o.neg_mk * gx) * m;
@@ -260,7 +269,7 @@ __device__ __forceinline__ bool pair_circular<float>(float x, float y, float vx,
   if (h < 0.0f) {
     float nx = 0.0f, ny = 0.0f;
     if (d != 0.0f) {
-      const float rd = frcp(d);
+      const float rd = frcp_normal(d);
       nx = x * rd;
       ny = y * rd;
     }

@@ -547,21 +547,30 @@ def test_step_without_pairs(hallway):
     assert np.all(outs[0]['force'] == 0.0)
 
 
-def test_step_grid_overflow_reported(hallway):
+@pytest.mark.parametrize('order_every', [0, 1])
+def test_step_grid_overflow_reported(hallway, order_every):
     """A cell capacity far below the grid: the step's binning is invalid, the
     force kernel adds only the walls and integrates, and the overflow is
-    reported (GridOverflow) instead of touching cells beyond the capacity."""
+    reported (GridOverflow) instead of touching cells beyond the capacity.
+    In cell order the order is not rebuilt from the invalid binning, so every
+    agent is still integrated exactly once: the run equals index order."""
     from crowddynamics_amd.stepper import GridOverflow
-    say('fused step with a grid beyond the cell capacity')
-    st = Stepper(hallway.agents.copy(), segments=hallway.segments, cell_size=hallway.cell_size,
-                 dt_min=hallway.dt_min, dt_max=hallway.dt_max, seed=hallway.seed,
-                 cell_capacity=4, bounds=hallway.bounds)
-    st.step(2)
-    with pytest.raises(GridOverflow):
-        st.check()
-    out = st.download(hallway.agents.copy())
-    assert np.all(np.isfinite(out['position']))
-    assert st.state().step == 2
+    say('fused step with a grid beyond the cell capacity, order_every %d' % order_every)
+    outs = []
+    for every in sorted({0, order_every}):
+        st = Stepper(hallway.agents.copy(), segments=hallway.segments,
+                     cell_size=hallway.cell_size, dt_min=hallway.dt_min,
+                     dt_max=hallway.dt_max, seed=hallway.seed, cell_capacity=4,
+                     bounds=hallway.bounds, order_every=every)
+        st.step(2)
+        with pytest.raises(GridOverflow):
+            st.check()
+        out = st.download(hallway.agents.copy())
+        assert np.all(np.isfinite(out['position']))
+        assert st.state().step == 2
+        outs.append(out)
+    for f in ('position', 'velocity'):
+        assert np.array_equal(outs[-1][f], outs[0][f])
 
 
 @pytest.mark.parametrize('tile', [(1, 1), (3, 2)])
