@@ -118,7 +118,11 @@ __device__ __forceinline__ void stencil_vjp(const Agents<double> &A, const Work 
 #pragma unroll
   for (int k = 0; k < 9; ++k) pre[k + 1] = pre[k] + n[k];
   const int tot = pre[9];
-  const int b0 = (int)(((long long)tot * q) / L), b1 = (int)(((long long)tot * (q + 1)) / L);
+  // floor(tot q / L) by a shift (L is a power of two), in 64 bits: the
+  // counts of overfull cells are not bounded by the slots
+  const int lg = __builtin_ctz(L);
+  const int b0 = (int)(((long long)tot * q) >> lg);
+  const int b1 = (int)(((long long)tot * (q + 1)) >> lg);
   const double2 *e_vel = (const double2 *)W.e_vel;
   bool over = false;
 #pragma unroll
@@ -183,7 +187,8 @@ __global__ void __launch_bounds__(256) k_adjoint(Agents<double> A, Work W, Batch
   const int2 sk = block_e(B);
   const ScnDesc d = desc_of(B, sk.x);
   const ScnState *ss = &B.sst[sk.x];
-  const int i = d.a0 + sk.y * ((int)blockDim.x / L) + (int)threadIdx.x / L;
+  const int lg = __builtin_ctz(L);  // L is a power of two
+  const int i = d.a0 + sk.y * ((int)blockDim.x >> lg) + ((int)threadIdx.x >> lg);
   const int q = threadIdx.x & (L - 1);
   const bool have = i < d.a1;
   const bool binned = (flags & CD_USE_AGENT_AGENT) && ss->valid && !ss->oob[0];

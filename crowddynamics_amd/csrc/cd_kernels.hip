@@ -994,7 +994,8 @@ struct LdsRow {
       pre[r + 1] = pre[r] + (S.off[kk + 3] - st[r]);
     }
     const int tot = pre[3];
-    const int b0 = (tot * q) / L, b1 = (tot * (q + 1)) / L;
+    const int lg = __builtin_ctz(L);  // L is a power of two
+    const int b0 = (tot * q) >> lg, b1 = (tot * (q + 1)) >> lg;
     for (int idx = b0; idx < b1; ++idx) {
       const int r = (idx >= pre[1] ? 1 : 0) + (idx >= pre[2] ? 1 : 0);
       const int e = (r == 0 ? st[0] : (r == 1 ? st[1] - pre[1] : st[2] - pre[2])) + idx;
@@ -1049,7 +1050,11 @@ struct MemRow {
       over = over || n[k] > W.K;
     }
     const int tot = pre[9];
-    const int b0 = (int)(((long long)tot * q) / L), b1 = (int)(((long long)tot * (q + 1)) / L);
+    // floor(tot q / L) by a shift (L is a power of two), in 64 bits: the
+    // counts of overfull cells are not bounded by the slots
+    const int lg = __builtin_ctz(L);
+    const int b0 = (int)(((long long)tot * q) >> lg);
+    const int b1 = (int)(((long long)tot * (q + 1)) >> lg);
     if (!over) {
       // slot-list position of candidate idx in cell k: so[k] + idx (empty
       // cells, which hold no candidate, keep 0; every real cell * K is below
@@ -1356,10 +1361,10 @@ __global__ void __launch_bounds__(256, MODEL == 0 ? 3 : 1) k_forces_direct(Agent
   if (!FUSED && !ok) return;
   const double dt = FUSED ? ss->dt : 0.0;
   stage_walls(fa, d, sseg, FUSED);
-  const int per = (int)blockDim.x / L;
+  const int per = (int)blockDim.x >> __builtin_ctz(L);  // L is a power of two
   // the agent: the next in index order, or in the processing order (an
   // entry outside the scenario, which a valid order never holds, serves none)
-  const int local = sk.y * per + (int)threadIdx.x / L;
+  const int local = sk.y * per + ((int)threadIdx.x >> __builtin_ctz(L));
   int i = d.a0 + local;
   if (ordered && i < d.a1) i = W.order[i];
   const bool have = i >= d.a0 && i < d.a1;
