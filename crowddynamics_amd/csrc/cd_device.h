@@ -156,6 +156,30 @@ struct Fix128 {
     lo += l;
     hi += (long long)h + (lo < l ? 1 : 0);
   }
+  // A float term: the same trunc(|x| 2^64) with x's sign as add((double)x).
+  // Below 2^32 it is formed by float steps that are all exact: the integer
+  // part, then the upper and lower 32 bits of the fraction, each a
+  // truncation, the subtraction of a value's own integer part and a scaling
+  // by 2^32.  Larger terms, inf and NaN take add(double).
+  __host__ __device__ __forceinline__ void add(float x) {
+    const float a = __builtin_fabsf(x);
+    if (!(a < 0x1p32f)) {
+      add((double)x);
+      return;
+    }
+    const float t2 = __builtin_truncf(a);
+    const float g1 = (a - t2) * 0x1p32f;
+    const float t1 = __builtin_truncf(g1);
+    const unsigned w0 = (unsigned)((g1 - t1) * 0x1p32f);
+    unsigned long long l = ((unsigned long long)(unsigned)t1 << 32) | w0;
+    unsigned long long h = (unsigned)t2;
+    if (x < 0.0f) {  // two's complement of (h, l)
+      h = ~h + (l == 0ull ? 1ull : 0ull);
+      l = ~l + 1ull;
+    }
+    lo += l;
+    hi += (long long)h + (lo < l ? 1 : 0);
+  }
   __host__ __device__ __forceinline__ void add(const Fix128 &o) {
     lo += o.lo;
     hi += o.hi + (lo < o.lo ? 1 : 0);

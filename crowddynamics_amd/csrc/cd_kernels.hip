@@ -722,8 +722,8 @@ __device__ __forceinline__ void pair_add_c(double2 xi, V2<T> vi, double ri, cons
   T px, py;
   if (pair_circular<T>((T)(xi.x - xj.x), (T)(xi.y - xj.y), vi.x - vj.x, vi.y - vj.y,
                        (T)(ri + rj), o, px, py)) {
-    ps.x.add((double)px);
-    ps.y.add((double)py);
+    ps.x.add(px);
+    ps.y.add(py);
   }
 }
 
@@ -1257,8 +1257,8 @@ __device__ __forceinline__ void agent_forces(const FusedArgs &fa, const ScnDesc 
   const bool walls = FUSED && (fa.flags & CD_USE_OBSTACLE);
   if (walls) {  // wall contacts join the same exact sums
     auto add = [&](T cx, T cy, double mm) {
-      ps.x.add((double)cx);
-      ps.y.add((double)cy);
+      ps.x.add(cx);
+      ps.y.add(cy);
       if (MODEL) ps.m.add(mm);
     };
     const int nw = d.w1 - d.w0;

@@ -1,10 +1,12 @@
 // Host check of the exact accumulator of the force kernels (Fix128 in
 // crowddynamics_amd/csrc/cd_device.h): exact round trips, independence of
-// the order of the terms, accuracy of sums, NaN for out-of-range terms.
+// the order of the terms, accuracy of sums, NaN for out-of-range terms, and
+// the float path adding exactly what the double path adds.
 // Prints "ok" and exits 0 on success.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <random>
 #include <vector>
 
@@ -76,6 +78,31 @@ int main() {
     }
     if (std::fabs((long double)a.value() - exact) > 1e-15L * scale + 1e-18L && fails++ < 5)
       std::printf("sum error %Lg\n", (long double)a.value() - exact);
+  }
+  // a float term (the fp32 mode's path) adds exactly what the same value as
+  // a double adds: random bit patterns over every float exponent, both
+  // signs, subnormals, the 2^32 switch to the double path, inf and NaN
+  for (int k = 0; k < 2000000; ++k) {
+    float x;
+    if (k < 64) {
+      const float special[] = {0.0f, -0.0f, 1.0f, -1.0f, 0x1p32f, -0x1p32f, 0x1.fffffep31f,
+                               -0x1.fffffep31f, 0x1p-64f, -0x1p-65f, 0x1p-149f, -0x1p-149f,
+                               0x1.fffffep127f, INFINITY, -INFINITY, NAN};
+      x = special[k % 16];
+    } else {
+      const unsigned bits = (unsigned)g();
+      std::memcpy(&x, &bits, sizeof(x));
+    }
+    cd::Fix128 a, b;
+    a.add(x);
+    b.add((double)x);
+    if ((a.hi != b.hi || a.lo != b.lo || a.bad != b.bad) && fails++ < 5)
+      std::printf("float term %a: %lld %llu %d vs %lld %llu %d\n", (double)x, a.hi, a.lo,
+                  (int)a.bad, b.hi, b.lo, (int)b.bad);
+    a.add(-x * 0.75f);  // and as the second term of a sum
+    b.add((double)(-x * 0.75f));
+    if ((a.hi != b.hi || a.lo != b.lo || a.bad != b.bad) && fails++ < 5)
+      std::printf("float sum %a\n", (double)x);
   }
   cd::Fix128 c;
   c.add(-0.5);
