@@ -123,10 +123,40 @@ struct Fix128 {
   unsigned long long lo;
   bool bad;
   __host__ __device__ __forceinline__ Fix128() : hi(0), lo(0ull), bad(false) {}
-  // Adds trunc(|x| 2^64) with x's sign: the mantissa shifted into place
-  // (|x| = m 2^(e - 1075) for biased exponent e, so |x| 2^64 = m 2^s with
-  // s = e - 1011; subnormals use e = 1).  |x| >= 2^62, inf and NaN set bad.
+  // Adds trunc(|x| 2^64) with x's sign.  Below 2^32 by exact
+  // floating-point steps: the integer part, then the upper and lower 32 bits
+  // of the fraction, each a truncation, the subtraction of a value's own
+  // integer part and a scaling by 2^32 (all exact in the value's own
+  // precision).  Otherwise by add_bits.
   __host__ __device__ __forceinline__ void add(double x) {
+    const double a = __builtin_fabs(x);
+    if (!(a < 0x1p32)) {
+      add_bits(x);
+      return;
+    }
+    const double t2 = __builtin_trunc(a);
+    const double g1 = (a - t2) * 0x1p32;
+    const double t1 = __builtin_trunc(g1);
+    add_mag((unsigned)t2, (unsigned)t1, (unsigned)((g1 - t1) * 0x1p32), x < 0.0);
+  }
+  // A float term: the same value as add((double)x), by the same steps in
+  // float arithmetic.
+  __host__ __device__ __forceinline__ void add(float x) {
+    const float a = __builtin_fabsf(x);
+    if (!(a < 0x1p32f)) {
+      add_bits((double)x);
+      return;
+    }
+    const float t2 = __builtin_truncf(a);
+    const float g1 = (a - t2) * 0x1p32f;
+    const float t1 = __builtin_truncf(g1);
+    add_mag((unsigned)t2, (unsigned)t1, (unsigned)((g1 - t1) * 0x1p32f), x < 0.0f);
+  }
+  // The bit-level conversion, valid for every double: the mantissa shifted
+  // into place (|x| = m 2^(e - 1075) for biased exponent e, so |x| 2^64 =
+  // m 2^s with s = e - 1011; subnormals use e = 1).  |x| >= 2^62, inf and
+  // NaN set bad.
+  __host__ __device__ __forceinline__ void add_bits(double x) {
     const unsigned long long bits = __builtin_bit_cast(unsigned long long, x);
     const int e = (int)((bits >> 52) & 0x7ffull);
     if (e >= 1085) {  // 2^62 = 2^(1085 - 1023)
@@ -149,31 +179,17 @@ struct Fix128 {
       l = 0ull;
       h = 0ull;
     }
-    if (bits >> 63) {  // two's complement of (h, l)
-      h = ~h + (l == 0ull ? 1ull : 0ull);
-      l = ~l + 1ull;
-    }
-    lo += l;
-    hi += (long long)h + (lo < l ? 1 : 0);
+    add_twos(h, l, (bits >> 63) != 0);
   }
-  // A float term: the same trunc(|x| 2^64) with x's sign as add((double)x).
-  // Below 2^32 it is formed by float steps that are all exact: the integer
-  // part, then the upper and lower 32 bits of the fraction, each a
-  // truncation, the subtraction of a value's own integer part and a scaling
-  // by 2^32.  Larger terms, inf and NaN take add(double).
-  __host__ __device__ __forceinline__ void add(float x) {
-    const float a = __builtin_fabsf(x);
-    if (!(a < 0x1p32f)) {
-      add((double)x);
-      return;
-    }
-    const float t2 = __builtin_truncf(a);
-    const float g1 = (a - t2) * 0x1p32f;
-    const float t1 = __builtin_truncf(g1);
-    const unsigned w0 = (unsigned)((g1 - t1) * 0x1p32f);
-    unsigned long long l = ((unsigned long long)(unsigned)t1 << 32) | w0;
-    unsigned long long h = (unsigned)t2;
-    if (x < 0.0f) {  // two's complement of (h, l)
+  // Adds the magnitude with 32-bit words w2 (2^0), w1 (2^-32), w0 (2^-64),
+  // negated when neg.
+  __host__ __device__ __forceinline__ void add_mag(unsigned w2, unsigned w1, unsigned w0,
+                                                   bool neg) {
+    add_twos(w2, ((unsigned long long)w1 << 32) | w0, neg);
+  }
+  __host__ __device__ __forceinline__ void add_twos(unsigned long long h, unsigned long long l,
+                                                    bool neg) {
+    if (neg) {  // two's complement of (h, l)
       h = ~h + (l == 0ull ? 1ull : 0ull);
       l = ~l + 1ull;
     }

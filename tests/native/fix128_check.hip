@@ -1,7 +1,8 @@
 // Host check of the exact accumulator of the force kernels (Fix128 in
 // crowddynamics_amd/csrc/cd_device.h): exact round trips, independence of
 // the order of the terms, accuracy of sums, NaN for out-of-range terms, and
-// the float path adding exactly what the double path adds.
+// the floating-point conversions (double and float terms below 2^32) adding
+// exactly what the bit-level conversion adds.
 // Prints "ok" and exits 0 on success.
 #include <algorithm>
 #include <cmath>
@@ -47,13 +48,16 @@ int main() {
       x = u(g) * std::ldexp(1.0, (int)(g() % 1140) - 1075);
     }
     if (!(std::fabs(x) < std::ldexp(1.0, 62))) continue;
-    cd::Fix128 a;
+    cd::Fix128 a, b;
     a.add(x);
+    b.add_bits(x);  // the bit-level conversion, over the whole range
     long long h;
     unsigned long long l;
     split_ref(x, h, l);
     if ((a.hi != h || a.lo != l) && fails++ < 5)
       std::printf("conversion %.17g: %lld %llu vs %lld %llu\n", x, a.hi, a.lo, h, l);
+    if ((b.hi != h || b.lo != l) && fails++ < 5)
+      std::printf("bit conversion %.17g: %lld %llu vs %lld %llu\n", x, b.hi, b.lo, h, l);
   }
   for (int k = 0; k < 200000; ++k) {  // |x| in [2^-11, 2^50]: the conversion is exact
     const double x = u(g) * std::ldexp(1.0, (int)(g() % 61) - 10);
@@ -95,12 +99,12 @@ int main() {
     }
     cd::Fix128 a, b;
     a.add(x);
-    b.add((double)x);
+    b.add_bits((double)x);
     if ((a.hi != b.hi || a.lo != b.lo || a.bad != b.bad) && fails++ < 5)
       std::printf("float term %a: %lld %llu %d vs %lld %llu %d\n", (double)x, a.hi, a.lo,
                   (int)a.bad, b.hi, b.lo, (int)b.bad);
     a.add(-x * 0.75f);  // and as the second term of a sum
-    b.add((double)(-x * 0.75f));
+    b.add_bits((double)(-x * 0.75f));
     if ((a.hi != b.hi || a.lo != b.lo || a.bad != b.bad) && fails++ < 5)
       std::printf("float sum %a\n", (double)x);
   }
