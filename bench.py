@@ -65,27 +65,29 @@ def pair_tally(arr, cell, segments):
 
 def step_begin_bytes(n, cells, real_bytes=8):
     """First kernel of the step, per agent.  Binning: read the position 16,
-    write the cell index and the slot id (4 + 4), read-modify-write the
-    cell's count 8.  Own terms: read mass, fluctuation sigma, tau_adj,
-    target speed and target 40, the four potential values of the bilinear
-    navigation lookup 32, force, velocity and target direction (2 reals
-    each); write force and target direction (2 reals each).  Snapshot for
-    the force kernel: write position 16 and velocity (2 reals).  120 + 12
-    reals = 216 B/agent in fp64, 168 B in fp32; plus clearing the previous
-    step's cell counts, 4 B per grid cell."""
-    return n * (32 + 40 + 32 + 16 + 12 * real_bytes) + 4 * cells
+    write the cell index, the slot and the id in the slot list (4 + 4 + 4),
+    read-modify-write the cell's count 8.  Own terms: read mass,
+    fluctuation sigma, tau_adj, target speed and target 40, the four
+    potential values of the bilinear navigation lookup 32, force, velocity
+    and target direction (2 reals each); write force and target direction
+    (2 reals each).  For the force kernel: write the snapshot (position 16,
+    velocity 2 reals) and the slot's neighbour record (position 16, velocity
+    2 reals, radius 8, read 8).  156 + 14 reals = 268 B/agent in fp64,
+    212 B in fp32; plus clearing the previous step's cell counts, 4 B per
+    grid cell."""
+    return n * (36 + 40 + 32 + 16 + 32 + 14 * real_bytes) + 4 * cells
 
 
 def forces_bytes(n, real_bytes=8):
     """Force kernel with the integrator (circular, fused), per agent: read
-    position 16, cell index 4, radius, mass and the five pair and contact
-    parameters 56, target speed 8, its id in a slot list 4, velocity, force
-    and previous force (2 reals each), and once as a neighbour its snapshot
-    (position 16, velocity 2 reals); write position 16, velocity, previous
-    force and the reset force (2 reals each).  120 + 14 reals = 232 B/agent
-    in fp64, 176 B in fp32 (neighbour data counted once: every agent is read
-    as a neighbour from cache)."""
-    return n * (16 + 4 + 56 + 8 + 4 + 16 + 16 + 14 * real_bytes)
+    position 16, cell index and own slot 4 + 4, radius, mass and the five
+    pair and contact parameters 56, target speed 8, velocity, force and
+    previous force (2 reals each), and once as a neighbour its record
+    (position 16, velocity 2 reals, radius 8); write position 16, velocity,
+    previous force and the reset force (2 reals each).  128 + 14 reals =
+    240 B/agent in fp64, 184 B in fp32 (neighbour data counted once: every
+    agent is read as a neighbour from cache)."""
+    return n * (16 + 8 + 56 + 8 + 16 + 8 + 16 + 14 * real_bytes)
 
 
 def measured_traffic(kernel, threads):

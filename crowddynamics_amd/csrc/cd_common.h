@@ -103,6 +103,7 @@ struct Layout {
   int64_t ell_ids, e_pos, e_vel, e_r, e_ls, e_rs, e_phi, e_rt, e_rs_r, e_rts;
   int64_t snap_pos, snap_vel, snap_ls, snap_rs, snap_phi;
   int64_t order, order_off;
+  int64_t rec;
   int64_t total;
 };
 
@@ -160,6 +161,9 @@ Layout make_layout(const cd_workspace_t *w) {
   // offsets it is built from
   L.order = take(4 * n);
   L.order_off = take(4 * cap);
+  // circular agents: the fused step's neighbour records, one per slot
+  // (NbrRec: 48 bytes in fp64, 32 in fp32)
+  L.rec = (w->model == CD_CIRCULAR) ? take(48 * slots) : -1;
   L.total = o;
   return L;
 }
@@ -231,6 +235,7 @@ struct Work {
   double2 *snap_ls, *snap_rs;
   double *snap_phi;
   int *order, *order_off;  // force kernel's processing order (cell order when order_every > 0)
+  void *rec;               // circular agents: NbrRec of each slot, written by the fused step's K_a
   int cap, K, ntiles;
   int order_every;
 };
@@ -274,6 +279,7 @@ Work carve(const cd_workspace_t *w) {
   W.snap_rs = L.snap_rs >= 0 ? (double2 *)(b + L.snap_rs) : nullptr;
   W.snap_phi = L.snap_phi >= 0 ? (double *)(b + L.snap_phi) : nullptr;
   W.order = (int *)(b + L.order);
+  W.rec = L.rec >= 0 ? (void *)(b + L.rec) : nullptr;
   W.order_off = (int *)(b + L.order_off);
   // only the agent-parallel force kernel follows the order
   W.order_every = (w->tile_x > 0 && w->tile_y > 0) ? 0 : (int)w->order_every;

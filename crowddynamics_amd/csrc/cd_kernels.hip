@@ -648,6 +648,19 @@ struct Nbr {
   const double *phi;
 };
 
+// A slot's neighbour record (circular agents, fused step): the data a
+// candidate needs, written by the first kernel beside the slot's id, so the
+// force kernel reads it at one address with no id hop.  The same values as
+// the snapshot.  48 bytes in fp64, 32 in fp32 (the workspace holds 48 per
+// slot).
+template <typename T>
+struct __attribute__((aligned(16))) NbrRec {
+  double2 pos;
+  V2<T> vel;
+  double r;
+};
+static_assert(sizeof(NbrRec<double>) == 48 && sizeof(NbrRec<float>) == 32, "NbrRec layout");
+
 template <typename T>
 __host__ __device__ __forceinline__ Nbr<T> nbr_agents(const Agents<T> &A) {
   Nbr<T> N;
@@ -874,6 +887,7 @@ __global__ void __launch_bounds__(256) k_step_begin(Agents<T> A, Work W, Batch B
   const bool ranks = W.order_every > 0 && step % (uint64_t)W.order_every == 0ull;
   V2<T> *snap_vel = (V2<T> *)W.snap_vel;
   auto serve = [&](int i, double2 pos, const OwnIn<T> &o) {
+    const double ri = (MODEL == 0 && bin) ? A.radius[i] : 0.0;  // for the neighbour record
     W.snap_pos[i] = pos;
     snap_vel[i] = o.vel;
     if (MODEL) {
@@ -891,7 +905,9 @@ __global__ void __launch_bounds__(256) k_step_begin(Agents<T> A, Work W, Batch B
         c = d.c0 + lc;
         W.cell_of[i] = c;
         k = atomicAdd(&count[c], 1);
-        if (ranks) W.rank[i] = k;
+        // the slot: for the processing order, and (circular agents) for the
+        // force kernel, which skips the agent's own record by it
+        if (ranks || MODEL == 0) W.rank[i] = k;
       }
     }
     NavFetch nf;
@@ -899,9 +915,16 @@ __global__ void __launch_bounds__(256) k_step_begin(Agents<T> A, Work W, Batch B
     if (fa.flags & CD_USE_NAV) nf = own_nav(fa, pos, o);
     own_apply<T, MODEL>(A, fa, d.seed, step, i, i - d.a0, o, nf);
     if (c >= 0) {
-      if (k < W.K)
+      if (k < W.K) {
         W.ell_ids[c * W.K + k] = i;
-      else if (k == W.K)  // the cell's list is full: its members are found by scanning cell_of
+        if constexpr (MODEL == 0) {
+          NbrRec<T> rr;
+          rr.pos = pos;
+          rr.vel = o.vel;
+          rr.r = ri;
+          ((NbrRec<T> *)W.rec)[(size_t)c * W.K + k] = rr;
+        }
+      } else if (k == W.K)  // the cell's list is full: its members are found by scanning cell_of
         atomicOr((unsigned long long *)&W.st->status, (unsigned long long)CD_STATUS_SLOTS_FULL);
     }
   };
@@ -1018,6 +1041,10 @@ struct MemRow {
   const int *count;  // the count buffer of this binning
   Nbr<T> N;
   int c0, nx, ny, lx, ly, a0, a1;
+  // circular agents of the fused step: the slots' neighbour records, and the
+  // agent's own slot in its cell (the stencil's centre cell); else null
+  const NbrRec<T> *rec;
+  int self_slot;
   template <class F>
   __device__ __forceinline__ void visit(int self, int j, F &fn) const {
     if (j == self) return;
@@ -1063,6 +1090,37 @@ struct MemRow {
 #pragma unroll
       for (int k = 0; k < 9; ++k)
         so[k] = n[k] ? (gr[k / 3] + k % 3) * W.K - pre[k] : 0;
+      if constexpr (MODEL == 0) {
+        if (rec) {
+          // neighbour records: one load per candidate and no id hop; the
+          // agent's own record (centre cell k = 4, its slot) is skipped by
+          // its candidate index
+          const int sidx = self_slot >= 0 ? pre[4] + self_slot : -1;
+          for (int b = b0; b < b1; b += 4) {
+            int s[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const int idx = b + u;
+              s[u] = -1;
+              if (idx < b1 && idx != sidx) {
+                int o = so[0];
+#pragma unroll
+                for (int k = 1; k < 9; ++k)
+                  if (idx >= pre[k]) o = so[k];
+                s[u] = o + idx;
+              }
+            }
+            NbrRec<T> rr[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+              if (s[u] >= 0) rr[u] = rec[s[u]];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+              if (s[u] >= 0) fn(-1, rr[u].pos, rr[u].vel, rr[u].r);
+          }
+          return;
+        }
+      }
       for (int b = b0; b < b1; b += 4) {
         int j[4];
 #pragma unroll
@@ -1374,13 +1432,17 @@ __global__ void __launch_bounds__(256, MODEL == 0 ? 3 : 1) k_forces_direct(Agent
   vel.x = T(0);
   vel.y = T(0);
   int c = d.c0;
+  int self_slot = -1;
   TCView me;
   Self<T> self;
   Tail<T> tail;
   if (have) {
     pos = A.pos[i];
     vel = A.vel[i];
-    if (ok) c = W.cell_of[i];
+    if (ok) {
+      c = W.cell_of[i];
+      if (MODEL == 0 && FUSED) self_slot = W.rank[i];
+    }
     if constexpr (MODEL == 1) me = tc_agent(A, i);
     self = self_of<T, MODEL>(A, i, pos);
     if (FUSED && lead) tail = tail_of(A, i);
@@ -1389,7 +1451,9 @@ __global__ void __launch_bounds__(256, MODEL == 0 ? 3 : 1) k_forces_direct(Agent
   const int lc = c - d.c0, lx = ny > 0 ? lc / ny : 0, ly = lc - lx * ny;
   const MemRow<T, MODEL> rows{A, W, W.count + (size_t)p * W.cap,
                               FUSED ? nbr_snapshot<T>(W) : nbr_agents(A),
-                              d.c0, nx, ny, lx, ly, d.a0, d.a1};
+                              d.c0, nx, ny, lx, ly, d.a0, d.a1,
+                              (MODEL == 0 && FUSED) ? (const NbrRec<T> *)W.rec : nullptr,
+                              self_slot};
   __syncthreads();  // walls staged
   V2<T> f;
   double tq;
@@ -1547,7 +1611,7 @@ __device__ void cell_from_memory(const Agents<T> &A, const Work &W, const FusedA
                                  int ly, const TileCtx<T> &cx, Red &red, bool pairs = true,
                                  int part = 0, int nb = 1) {
   const int gc = d.c0 + lx * ny + ly;
-  const MemRow<T, MODEL> rows{A, W, cx.count, cx.N, d.c0, nx, ny, lx, ly, d.a0, d.a1};
+  const MemRow<T, MODEL> rows{A, W, cx.count, cx.N, d.c0, nx, ny, lx, ly, d.a0, d.a1, nullptr, -1};
   for (int j0 = d.a0 + part * kTileGroups; j0 < d.a1; j0 += nb * kTileGroups) {
     const int i = j0 + (threadIdx.x >> 2);
     const bool have = i < d.a1 && (!pairs || W.cell_of[i] == gc);
