@@ -106,6 +106,19 @@ def measured_traffic(kernel, threads):
     return None
 
 
+def host_cpu():
+    """Model name and logical CPU count of the host the CPU baseline ran on."""
+    model = None
+    try:
+        for line in open('/proc/cpuinfo'):
+            if line.startswith('model name'):
+                model = line.split(':', 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return dict(model=model, logical_cpus=os.cpu_count())
+
+
 def cpu_baseline(scn, seconds, threads=None):
     """Oracle on the host: single-thread restatement of the reference's
     algorithm (threads=None) or its OpenMP owner-computes variant."""
@@ -477,7 +490,8 @@ def main():
         cpu = dict(value=v1, unit='agent-steps/s', cores=1, kind='port',
                    sample='%d steps of the same 20k-agent room from its initial state, '
                           'fp64, %.1f s' % (s1, e1),
-                   all_core=dict(value=vn, cores=nth, steps=sn))
+                   all_core=dict(value=vn, cores=nth, steps=sn),
+                   host=host_cpu())
 
     r = results[main_p]
     line = {
