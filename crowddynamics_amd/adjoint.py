@@ -33,7 +33,8 @@ class Rollout:
     """Fixed-dt rollout of `steps` steps from the records `array`."""
 
     def __init__(self, array, steps, dt=0.005, cell_size=3.6, checkpoint=25, seed=1,
-                 fluctuation=False, cell_capacity=None, bounds=None, device=None, lanes=0):
+                 fluctuation=False, cell_capacity=None, bounds=None, device=None, lanes=0,
+                 order_every=0):
         if 'position_ls' in array.dtype.names:
             raise InvalidValue('the adjoint covers circular agents')
         if steps < 1 or checkpoint < 1:
@@ -45,7 +46,7 @@ class Rollout:
         self.dt = float(dt)
         self.st = Stepper(array.copy(), cell_size=cell_size, dt_min=dt, dt_max=dt, seed=seed,
                           flags=flags, precision='f64', cell_capacity=cell_capacity,
-                          bounds=bounds, device=device, lanes=lanes)
+                          bounds=bounds, device=device, lanes=lanes, order_every=order_every)
         self.agents = self.st.agents
         self.n = self.agents.n
         dev = self.agents.device

@@ -188,9 +188,13 @@ __global__ void __launch_bounds__(256) k_adjoint(Agents<double> A, Work W, Batch
   const ScnDesc d = desc_of(B, sk.x);
   const ScnState *ss = &B.sst[sk.x];
   const int lg = __builtin_ctz(L);  // L is a power of two
-  const int i = d.a0 + sk.y * ((int)blockDim.x >> lg) + ((int)threadIdx.x >> lg);
+  // the agent: the next in index order, or in the forward steps' processing
+  // order (any permutation gives the same sums: each agent's lanes combine
+  // in a fixed order and agents share no accumulator)
+  int i = d.a0 + sk.y * ((int)blockDim.x >> lg) + ((int)threadIdx.x >> lg);
+  if (W.order_every > 0 && i < d.a1) i = W.order[i];
   const int q = threadIdx.x & (L - 1);
-  const bool have = i < d.a1;
+  const bool have = i >= d.a0 && i < d.a1;
   const bool binned = (flags & CD_USE_AGENT_AGENT) && ss->valid && !ss->oob[0];
   double acc[8];
 #pragma unroll

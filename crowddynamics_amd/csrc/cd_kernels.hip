@@ -935,8 +935,9 @@ __global__ void __launch_bounds__(256) k_step_begin(Agents<T> A, Work W, Batch B
   }
 }
 
-// Processing order of the agent-parallel force kernel: each scenario's
-// agents by cell, a counting sort of the step's binning (exclusive scan of
+// Processing order of the agent-parallel force kernel (and of the adjoint
+// kernel, which follows the forward steps' order): each scenario's agents
+// by cell, a counting sort of the step's binning (exclusive scan of
 // the scenario's cell counts, then each agent at its cell's offset plus its
 // slot rank).  A wave's agents then share their neighbour cells, so the
 // lanes that walk the same part of the stencil load the same candidates.

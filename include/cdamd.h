@@ -88,11 +88,11 @@ typedef struct cd_workspace {
   int32_t agent_lanes;    /* lanes per agent of the agent-parallel force kernel and the
                              adjoint: 4, 8, 16, 32 or 64 (0: 4); same forces either way
                              (the adjoint's lane sums are in fp64) */
-  int32_t order_every;    /* agent-parallel force kernel of the fused step: 0 = agents in
-                             index order; k > 0 = in cell order, rebuilt after the binning
-                             of every step whose counter is a multiple of k (a wave's
-                             agents then share neighbour cells, so its neighbour loads
-                             coalesce); same results either way */
+  int32_t order_every;    /* agent-parallel force kernel of the fused step, and the
+                             adjoint kernel: 0 = agents in index order; k > 0 = in cell
+                             order, rebuilt after the binning of every step whose counter
+                             is a multiple of k (a wave's agents then share neighbour
+                             cells); same results either way */
 } cd_workspace_t;
 
 /* One scenario of a batch: its agents are [agent_begin, agent_end) of the
