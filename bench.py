@@ -358,8 +358,8 @@ def main():
                          'known traffic for scaling hardware byte counters)')
     args = ap.parse_args()
     if args.gpus != 1:
-        print('bench.py measures one GPU: the per-step update runs on a single MI355X '
-              '(multi-GPU sharding is described in DESIGN.md, not launched).', flush=True)
+        print('bench.py measures one GPU: the per-step update runs on a single MI355X; '
+              'sharding a scenario over several GPUs is not built.', flush=True)
         sys.exit(2)
 
     import torch

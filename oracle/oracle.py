@@ -1,8 +1,8 @@
 """Python front end of the CPU oracle (oracle/cd_oracle.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker or the timed host baseline.  The
-product package never imports it.
+cpu_baseline leg of bench.py, as the reference the GPU results are compared
+with or as the timed host baseline.  The product package never imports it.
 
 Functions take the reference's structured agent records (field names of
 crowddynamics/simulation/agents.py) and work on contiguous per-field copies,
