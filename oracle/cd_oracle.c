@@ -2,8 +2,8 @@
  * cd_oracle.c -- CPU restatement (fp64, strict IEEE, single thread) of the
  * crowddynamics per-timestep agent update.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker the GPU path is compared
- * against and the host-CPU baseline that bench.py times.  Only tests/,
+ * TEST INFRASTRUCTURE ONLY.  This file is the reference the GPU path is
+ * compared against and the host-CPU baseline that bench.py times.  Only tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.  The
  * product path (crowddynamics_amd) never links, loads or falls back to it.
  *
