@@ -61,3 +61,12 @@ def test_hallway_pair_set_and_forces_consistent():
             acc[a] += out[0]
             acc[b] += out[1]
     assert np.array_equal(acc, ref['force'])
+
+
+def test_bench_pair_tally_counts_the_oracle_pairs():
+    """bench.py's pair tally (k-d tree under the binning's cell rule) counts
+    exactly the oracle's candidate pairs: the 20k room and a dense packing."""
+    import bench
+    for arr, cell in ((E.room().agents, 0.6), (E.rollout(n=3000, density=2.0).agents, 3.6)):
+        tally = bench.pair_tally(arr, cell, np.zeros(0, dtype=[('p0', 'f8', 2), ('p1', 'f8', 2)]))
+        assert tally['candidate_pairs'] == len(O.iter_pairs(arr['position'], cell))
