@@ -101,7 +101,7 @@ struct Layout {
   int64_t state, scn_desc, scn_state, acc, map_f, map_e, map_t, partials, tiles;
   int64_t cell_of, rank, pidx, count, offset;
   int64_t ell_ids, e_pos, e_vel, e_r, e_ls, e_rs, e_phi, e_rt, e_rs_r, e_rts;
-  int64_t snap_pos, snap_vel, snap_ls, snap_rs, snap_phi;
+  int64_t snap_pos, snap_vel, snap_ls, snap_rs, snap_off;
   int64_t order, order_off;
   int64_t rec;
   int64_t total;
@@ -153,9 +153,9 @@ Layout make_layout(const cd_workspace_t *w) {
   if (w->model == CD_THREE_CIRCLE) {
     L.snap_ls = take(16 * n);
     L.snap_rs = take(16 * n);
-    L.snap_phi = take(8 * n);
+    L.snap_off = take(16 * n);  // shoulder offsets
   } else {
-    L.snap_ls = L.snap_rs = L.snap_phi = -1;
+    L.snap_ls = L.snap_rs = L.snap_off = -1;
   }
   // processing order of the agent-parallel force kernel and the cell
   // offsets it is built from
@@ -233,7 +233,7 @@ struct Work {
   double2 *snap_pos;
   void *snap_vel;
   double2 *snap_ls, *snap_rs;
-  double *snap_phi;
+  double2 *snap_off;
   int *order, *order_off;  // force kernel's processing order (cell order when order_every > 0)
   void *rec;               // circular agents: NbrRec of each slot, written by the fused step's K_a
   int cap, K, ntiles;
@@ -277,7 +277,7 @@ Work carve(const cd_workspace_t *w) {
   W.snap_vel = (void *)(b + L.snap_vel);
   W.snap_ls = L.snap_ls >= 0 ? (double2 *)(b + L.snap_ls) : nullptr;
   W.snap_rs = L.snap_rs >= 0 ? (double2 *)(b + L.snap_rs) : nullptr;
-  W.snap_phi = L.snap_phi >= 0 ? (double *)(b + L.snap_phi) : nullptr;
+  W.snap_off = L.snap_off >= 0 ? (double2 *)(b + L.snap_off) : nullptr;
   W.order = (int *)(b + L.order);
   W.rec = L.rec >= 0 ? (void *)(b + L.rec) : nullptr;
   W.order_off = (int *)(b + L.order_off);

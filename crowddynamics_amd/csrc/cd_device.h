@@ -376,8 +376,16 @@ __device__ __forceinline__ void distance_circle_line(double x, double y, double 
 struct TCView {
   double px[3], py[3];
   double vx, vy;
-  double r_t, r_s, r_ts, phi;
+  double r_t, r_s;
+  double ox, oy;  // shoulder_offset(r_ts, phi)
 };
+
+// The offset of the left shoulder from the torso, (r_ts sin(phi), r_ts
+// (-cos(phi))), as force_social_three_circle forms it (core/motion/
+// power_law.py:338-350); a per-agent value, computed once per agent and step.
+__device__ __forceinline__ double2 shoulder_offset(double r_ts, double phi) {
+  return make_double2(r_ts * sin(phi), r_ts * (-cos(phi)));
+}
 
 // interaction_agent_agent_three_circle (core/interactions.py:73-104) in the
 // reference's argument order (A = agent i, B = agent j), returning only the
@@ -394,7 +402,7 @@ __device__ __forceinline__ bool pair_three_circle(const TCView &A, const TCView 
   // distance_three_circles; the parts chosen by the minimum are recorded as
   // they are selected (no dynamic indexing of the part arrays).
   double h_min = __builtin_nan("");
-  double nx = 0.0, ny = 0.0;
+  double minx = 0.0, miny = 0.0, mind = 0.0;  // x, y and d of the minimum
   double selx = 0.0, sely = 0.0, selr = 0.0;  // A.part[i_min], rA[i_min]
   double bugx = 0.0, bugy = 0.0, bugr = 0.0;  // A.part[j_min], rB[j_min]
 #pragma unroll
@@ -404,15 +412,11 @@ __device__ __forceinline__ bool pair_three_circle(const TCView &A, const TCView 
       double x = A.px[i] - B.px[j], y = A.py[i] - B.py[j];
       double d = hypot(x, y);
       double h = d - (rA[i] + rB[j]);
-      double ex = 0.0, ey = 0.0;
-      if (d != 0.0) {
-        ex = x / d;
-        ey = y / d;
-      }
       if (h < h_min || __builtin_isnan(h_min)) {
         h_min = h;
-        nx = ex;
-        ny = ey;
+        minx = x;
+        miny = y;
+        mind = d;
         selx = A.px[i];
         sely = A.py[i];
         selr = rA[i];
@@ -421,6 +425,13 @@ __device__ __forceinline__ bool pair_three_circle(const TCView &A, const TCView 
         bugr = rB[j];
       }
     }
+  }
+  // the normal of the minimum only (the reference divides every part pair's
+  // x, y by its d and keeps the minimum's: the same quotients)
+  double nx = 0.0, ny = 0.0;
+  if (mind != 0.0) {
+    nx = minx / mind;
+    ny = miny / mind;
   }
   if (!(h_min < kSightSoc)) return false;
   double rmx, rmy;
@@ -461,7 +472,7 @@ __device__ __forceinline__ bool pair_three_circle(const TCView &A, const TCView 
     if (!(__builtin_isnan(tau) || tau <= 0.0)) {
       double roix = 0.0, roiy = 0.0, rojx = 0.0, rojy = 0.0;
       if (ci == 1 || ci == 2) {
-        double ex = A.r_ts * sin(A.phi), ey = A.r_ts * (-cos(A.phi));
+        const double ex = A.ox, ey = A.oy;
         if (ci == 1) {
           roix += ex;
           roiy += ey;
@@ -471,7 +482,7 @@ __device__ __forceinline__ bool pair_three_circle(const TCView &A, const TCView 
         }
       }
       if (cj == 1 || cj == 2) {
-        double ex = B.r_ts * sin(B.phi), ey = B.r_ts * (-cos(B.phi));
+        const double ex = B.ox, ey = B.oy;
         if (cj == 1) {
           rojx += ex;
           rojy += ey;

@@ -580,7 +580,8 @@ struct TileLds {
   V2<T> vel[CAP];
   double r[CAP];
   double2 ls[XC], rs[XC];
-  double phi[XC], rt[XC], rsr[XC], rts[XC];
+  double rt[XC], rsr[XC];
+  double2 sho[XC];  // shoulder offsets
 };
 
 // Exclusive scan of one int per thread over the block; *total gets the sum.
@@ -632,8 +633,9 @@ __device__ __forceinline__ TCView tc_agent(const Agents<T> &A, int j) {
   t.vy = (double)v.y;
   t.r_t = A.r_t[j];
   t.r_s = A.r_s[j];
-  t.r_ts = A.r_ts[j];
-  t.phi = A.phi[j];
+  const double2 o = shoulder_offset(A.r_ts[j], A.phi[j]);
+  t.ox = o.x;
+  t.oy = o.y;
   return t;
 }
 
@@ -645,7 +647,8 @@ struct Nbr {
   const double2 *pos;
   const V2<T> *vel;
   const double2 *ls, *rs;
-  const double *phi;
+  const double *phi;    // the agent arrays' orientation (operator path) ...
+  const double2 *off;   // ... or the snapshot's shoulder offsets (fused step)
 };
 
 // A slot's neighbour record (circular agents, fused step): the data a
@@ -669,6 +672,7 @@ __host__ __device__ __forceinline__ Nbr<T> nbr_agents(const Agents<T> &A) {
   N.ls = A.ls;
   N.rs = A.rs;
   N.phi = A.phi;
+  N.off = nullptr;
   return N;
 }
 
@@ -679,7 +683,8 @@ __host__ __device__ __forceinline__ Nbr<T> nbr_snapshot(const Work &W) {
   N.vel = (const V2<T> *)W.snap_vel;
   N.ls = W.snap_ls;
   N.rs = W.snap_rs;
-  N.phi = W.snap_phi;
+  N.phi = nullptr;
+  N.off = W.snap_off;
   return N;
 }
 
@@ -698,8 +703,9 @@ __device__ __forceinline__ TCView tc_nbr(const Agents<T> &A, const Nbr<T> &N, in
   t.vy = (double)v.y;
   t.r_t = A.r_t[j];
   t.r_s = A.r_s[j];
-  t.r_ts = A.r_ts[j];
-  t.phi = N.phi[j];
+  const double2 o = N.off ? N.off[j] : shoulder_offset(A.r_ts[j], N.phi[j]);
+  t.ox = o.x;
+  t.oy = o.y;
   return t;
 }
 
@@ -718,8 +724,8 @@ __device__ __forceinline__ TCView tc_lds(const TileLds<T, MODEL> &S, int e) {
   t.vy = (double)v.y;
   t.r_t = S.rt[e];
   t.r_s = S.rsr[e];
-  t.r_ts = S.rts[e];
-  t.phi = S.phi[e];
+  t.ox = S.sho[e].x;
+  t.oy = S.sho[e].y;
   return t;
 }
 
@@ -893,7 +899,7 @@ __global__ void __launch_bounds__(256) k_step_begin(Agents<T> A, Work W, Batch B
     if (MODEL) {
       W.snap_ls[i] = A.ls[i];
       W.snap_rs[i] = A.rs[i];
-      W.snap_phi[i] = A.phi[i];
+      W.snap_off[i] = shoulder_offset(A.r_ts[i], A.phi[i]);
     }
     int c = -1, k = 0;
     if (bin) {
@@ -1554,10 +1560,9 @@ __device__ bool tile_region(const Agents<T> &A, const Work &W, const FusedArgs &
     if constexpr (MODEL == 1) {
       S.ls[e] = cx.N.ls[j];
       S.rs[e] = cx.N.rs[j];
-      S.phi[e] = cx.N.phi[j];
       S.rt[e] = A.r_t[j];
       S.rsr[e] = A.r_s[j];
-      S.rts[e] = A.r_ts[j];
+      S.sho[e] = cx.N.off ? cx.N.off[j] : shoulder_offset(A.r_ts[j], cx.N.phi[j]);
     }
   }
   if (tid == 0) {  // own agents: the inner cells, column by column
