@@ -617,3 +617,65 @@ def test_step_wall_contacts(hallway, corridor_small):
         for f in ('position', 'velocity', 'force_prev'):
             r = ref[f]
             assert_vec_close(out[f], r, np.hypot(*r.T) + 1.0, 1e-12, '%s %s' % (scn.name, f))
+
+
+def _many_walls(scn, n_extra, seed):
+    """`scn` with n_extra short segments (0.6 m, random) inside its bounds
+    after its own, and agents 0-29 centred 5 cm off the middle of the first
+    30 of them (a centre on a segment has no contact normal)."""
+    import dataclasses
+    rng = np.random.default_rng(seed)
+    x0, y0, x1, y1 = scn.bounds
+    c = rng.uniform((x0 + 0.5, y0 + 0.5), (x1 - 0.5, y1 - 0.5), (n_extra, 2))
+    a = rng.uniform(0.0, np.pi, n_extra)
+    h = 0.3 * np.stack([np.cos(a), np.sin(a)], 1)
+    extra = np.zeros(n_extra, dtype=core.obstacle_type_linear)
+    extra['p0'] = c - h
+    extra['p1'] = c + h
+    own = np.zeros(len(scn.segments), dtype=core.obstacle_type_linear)
+    own['p0'] = scn.segments['p0']
+    own['p1'] = scn.segments['p1']
+    arr = scn.agents.copy()
+    arr['position'][:30] = c[:30] + 0.05 * np.stack([-np.sin(a[:30]), np.cos(a[:30])], 1)
+    if 'position_ls' in arr.dtype.names:
+        shoulders_host(arr)
+    return dataclasses.replace(scn, agents=arr, segments=np.concatenate([own, extra]))
+
+
+@pytest.mark.parametrize('which,n_extra', [('hallway', 300), ('corridor', 360)])
+def test_step_many_walls(hallway, corridor_small, which, n_extra):
+    """More wall segments than the force kernel stages in LDS at once (tiles
+    of 256): 302 segments with circular agents, 404 with three-circle agents,
+    30 agents in contact with segments.  One step against the oracle; five
+    steps with the segments in the given and in reversed order (the wall
+    terms join exact sums), agent-parallel and (hallway) tiled, bit-identical."""
+    from crowddynamics_amd import _lib
+    from crowddynamics_amd.stepper import ALL_NODES
+    scn = _many_walls(hallway if which == 'hallway' else corridor_small, n_extra, 5)
+    say('%s with %d segments: one step against the oracle' % (which, len(scn.segments)))
+    ref = scn.agents.copy()
+    O.run(ref, _oracle_scenario(scn, fluct=False), 1)
+    st = Stepper(scn.agents.copy(), segments=scn.segments, nav=scn.nav, cell_size=scn.cell_size,
+                 dt_min=scn.dt_min, dt_max=scn.dt_max, seed=scn.seed,
+                 flags=ALL_NODES & ~_lib.CD_USE_FLUCT, bounds=scn.bounds)
+    st.step(1)
+    st.check()
+    out = st.download(scn.agents.copy())
+    assert np.all(np.hypot(*(ref['force_prev'][:30]).T) > 1.0)  # in contact
+    for f in ('position', 'velocity', 'force_prev'):
+        r = ref[f]
+        assert_vec_close(out[f], r, np.hypot(*r.T) + 1.0, 1e-12, '%s %s' % (which, f))
+    outs = []
+    runs = ((False, None), (True, None)) + (((True, (3, 2)),) if which == 'hallway' else ())
+    for rev, tile in runs:
+        say('%s: 5 steps, segments reversed %s, tile %s' % (which, rev, tile))
+        segs = scn.segments[::-1].copy() if rev else scn.segments
+        st = Stepper(scn.agents.copy(), segments=segs, nav=scn.nav, cell_size=scn.cell_size,
+                     dt_min=scn.dt_min, dt_max=scn.dt_max, seed=scn.seed, bounds=scn.bounds,
+                     tile=tile)
+        st.step(5)
+        st.check()
+        outs.append(st.download(scn.agents.copy()))
+    for out in outs[1:]:
+        for f in ('position', 'velocity', 'force_prev', 'target_direction'):
+            assert np.array_equal(out[f], outs[0][f]), (which, f)
