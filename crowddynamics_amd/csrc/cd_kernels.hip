@@ -804,15 +804,24 @@ __device__ __forceinline__ NavFetch own_nav(const FusedArgs &fa, double2 pos, co
 }
 
 // `local` is the agent's index within its scenario (the noise counter).
+// `noise`: the agent's fluctuation force x, y and torque computed elsewhere
+// (fluct_noise of the same inputs), or null to compute them here.
 template <typename T, int MODEL>
 __device__ __forceinline__ void own_apply(const Agents<T> &A, const FusedArgs &fa, uint64_t seed,
                                           uint64_t step, int i, int local, const OwnIn<T> &o,
-                                          const NavFetch &nf) {
+                                          const NavFetch &nf, const double *noise = nullptr) {
   V2<T> f = o.f;
   double tq = o.tq;
   if (fa.flags & CD_USE_FLUCT) {
     double nfx, nfy, nm;
-    fluct_noise(seed, step, local, o.mass, o.sig_f, MODEL != 0, o.inertia, o.sig_t, nfx, nfy, nm);
+    if (noise) {
+      nfx = noise[0];
+      nfy = noise[1];
+      nm = noise[2];
+    } else {
+      fluct_noise(seed, step, local, o.mass, o.sig_f, MODEL != 0, o.inertia, o.sig_t, nfx, nfy,
+                  nm);
+    }
     f.x += (T)nfx;
     f.y += (T)nfy;
     if (MODEL) tq += nm;
@@ -849,9 +858,14 @@ __device__ __forceinline__ void own_apply(const Agents<T> &A, const FusedArgs &f
 // loaded before the reduction is read, and its noise is computed while its
 // slot atomic and navigation map loads are in flight.  The counter is read
 // here only; the force kernel advances it.
-template <typename T, int MODEL>
+// NW (one scenario, fluctuation on): blocks of two waves, the first serving
+// 64 agents and the second computing their noise at the same time (the
+// same fluct_noise of the same inputs, handed over in LDS), so a lone wave
+// per SIMD does not also run the noise's long dependent chain.
+template <typename T, int MODEL, bool NW>
 __global__ void __launch_bounds__(256) k_step_begin(Agents<T> A, Work W, Batch B, FusedArgs fa,
                                                     double cell, int binning) {
+  __shared__ double s_noise[NW ? 3 * 64 : 1];
   const int2 sk = block_f(B);
   const ScnDesc d = desc_of(B, sk.x);
   ScnState *ss = &B.sst[sk.x];
@@ -859,20 +873,26 @@ __global__ void __launch_bounds__(256) k_step_begin(Agents<T> A, Work W, Batch B
   const int p = (int)(step & 1ull);
   int *count = W.count + (size_t)p * W.cap;
   // blocks of the scenario: its block map's, or the whole grid for a single
-  // scenario (launched as 64-thread blocks, one wave each, spread over all CUs)
-  const int stride = (B.map_f ? d.nf : (int)gridDim.x) * blockDim.x;
-  const int t0 = sk.y * blockDim.x + threadIdx.x;
+  // scenario (launched as 64-thread blocks, one wave each, spread over all
+  // CUs; with NW, 128-thread blocks)
+  const int nblk = B.map_f ? d.nf : (int)gridDim.x;
+  const int per = NW ? 64 : (int)blockDim.x;  // agents a block serves at once
+  const bool nwave = NW && threadIdx.x >= 64;   // the noise wave
+  const int lane = NW ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
+  const int stride = nblk * per;
+  const int t0 = sk.y * per + lane;
   const int i0 = d.a0 + t0;
   double2 p0 = make_double2(0.0, 0.0);
   OwnIn<T> o0;
-  if (i0 < d.a1) {
+  if (!nwave && i0 < d.a1) {
     p0 = A.pos[i0];
     o0 = own_load<T, MODEL>(A, fa, i0, p0);
   }
-  {  // counts of the previous step's binning
+  {  // counts of the previous step's binning (every thread of the block)
     int *other = W.count + (size_t)(1 - p) * W.cap + d.c0;
     const int clear = ss->C_clear[1 - p];
-    for (int c = t0; c < clear; c += stride) other[c] = 0;
+    for (int c = sk.y * (int)blockDim.x + (int)threadIdx.x; c < clear; c += nblk * (int)blockDim.x)
+      other[c] = 0;
   }
   const Red r = acc_read(W, sk.x, p);
   const Grid g = grid_of(r, cell, d.ccap);
@@ -892,8 +912,11 @@ __global__ void __launch_bounds__(256) k_step_begin(Agents<T> A, Work W, Batch B
   // steps that rebuild the force kernel's processing order keep the slot ranks
   const bool ranks = W.order_every > 0 && step % (uint64_t)W.order_every == 0ull;
   V2<T> *snap_vel = (V2<T> *)W.snap_vel;
-  auto serve = [&](int i, double2 pos, const OwnIn<T> &o) {
-    const double ri = (MODEL == 0 && bin) ? A.radius[i] : 0.0;  // for the neighbour record
+  // an agent's snapshot, binning and map loads (begin), then its own terms
+  // and slot list entry (finish)
+  auto begin = [&](int i, double2 pos, const OwnIn<T> &o, int &c, int &k, double &ri,
+                   NavFetch &nf) {
+    ri = (MODEL == 0 && bin) ? A.radius[i] : 0.0;  // for the neighbour record
     W.snap_pos[i] = pos;
     snap_vel[i] = o.vel;
     if (MODEL) {
@@ -901,7 +924,8 @@ __global__ void __launch_bounds__(256) k_step_begin(Agents<T> A, Work W, Batch B
       W.snap_rs[i] = A.rs[i];
       W.snap_off[i] = shoulder_offset(A.r_ts[i], A.phi[i]);
     }
-    int c = -1, k = 0;
+    c = -1;
+    k = 0;
     if (bin) {
       const int lc = cell_of_pos(pos, g, cell);
       if (lc < 0) {  // positions changed after the bounding box was reduced
@@ -916,10 +940,12 @@ __global__ void __launch_bounds__(256) k_step_begin(Agents<T> A, Work W, Batch B
         if (ranks || MODEL == 0) W.rank[i] = k;
       }
     }
-    NavFetch nf;
     nf.ok = false;
     if (fa.flags & CD_USE_NAV) nf = own_nav(fa, pos, o);
-    own_apply<T, MODEL>(A, fa, d.seed, step, i, i - d.a0, o, nf);
+  };
+  auto finish = [&](int i, double2 pos, const OwnIn<T> &o, int c, int k, double ri,
+                    const NavFetch &nf, const double *noise) {
+    own_apply<T, MODEL>(A, fa, d.seed, step, i, i - d.a0, o, nf, noise);
     if (c >= 0) {
       if (k < W.K) {
         W.ell_ids[c * W.K + k] = i;
@@ -934,10 +960,55 @@ __global__ void __launch_bounds__(256) k_step_begin(Agents<T> A, Work W, Batch B
         atomicOr((unsigned long long *)&W.st->status, (unsigned long long)CD_STATUS_SLOTS_FULL);
     }
   };
-  if (i0 < d.a1) serve(i0, p0, o0);
-  for (int i = i0 + stride; i < d.a1; i += stride) {
-    const double2 pos = A.pos[i];
-    serve(i, pos, own_load<T, MODEL>(A, fa, i, pos));
+  if constexpr (NW) {
+    // rounds of this block: the same for both waves (barriers inside)
+    const int first = d.a0 + sk.y * per;
+    const int rounds = first < d.a1 ? (d.a1 - first + stride - 1) / stride : 0;
+    for (int it = 0; it < rounds; ++it) {
+      const int i = i0 + it * stride;
+      const bool have = i < d.a1;
+      double2 pos = p0;
+      OwnIn<T> o = o0;
+      int c = -1, k = 0;
+      double ri = 0.0;
+      NavFetch nf;
+      nf.ok = false;
+      if (nwave) {
+        if (have) {
+          double nx, ny, nm;
+          fluct_noise(d.seed, step, i - d.a0, A.mass[i], A.sig_f[i], MODEL != 0,
+                      MODEL ? A.inertia[i] : 0.0, MODEL ? A.sig_t[i] : 0.0, nx, ny, nm);
+          s_noise[lane] = nx;
+          s_noise[64 + lane] = ny;
+          s_noise[128 + lane] = nm;
+        }
+      } else if (have) {
+        if (it) {
+          pos = A.pos[i];
+          o = own_load<T, MODEL>(A, fa, i, pos);
+        }
+        begin(i, pos, o, c, k, ri, nf);
+      }
+      __syncthreads();  // noise of this round in LDS
+      if (!nwave && have) {
+        const double noise[3] = {s_noise[lane], s_noise[64 + lane], s_noise[128 + lane]};
+        finish(i, pos, o, c, k, ri, nf, noise);
+      }
+      __syncthreads();  // LDS free for the next round
+    }
+  } else {
+    auto serve = [&](int i, double2 pos, const OwnIn<T> &o) {
+      int c, k;
+      double ri;
+      NavFetch nf;
+      begin(i, pos, o, c, k, ri, nf);
+      finish(i, pos, o, c, k, ri, nf, nullptr);
+    };
+    if (i0 < d.a1) serve(i0, p0, o0);
+    for (int i = i0 + stride; i < d.a1; i += stride) {
+      const double2 pos = A.pos[i];
+      serve(i, pos, own_load<T, MODEL>(A, fa, i, pos));
+    }
   }
 }
 
@@ -1801,16 +1872,26 @@ void launch_step_begin(const cd_agents_t *a, const cd_step_params_t *p, const Wo
                        const Batch &B, const FusedArgs &fa, const Launch &L, hipStream_t s) {
   const Agents<T> A = typed<T>(a);
   const int binning = (p->flags & CD_USE_AGENT_AGENT) ? 1 : 0;
+  // one scenario with fluctuation: a serving and a noise wave per 64 agents
+  const bool nw = !B.map_f && (p->flags & CD_USE_FLUCT);
   if (a->model == CD_THREE_CIRCLE) {
-    if constexpr (sizeof(T) == 8)
-      hipLaunchKernelGGL((k_step_begin<T, 1>), dim3(L.blocks_f), dim3(kBS), 0, s, A, W, B, fa,
-                         p->cell_size, binning);
+    if constexpr (sizeof(T) == 8) {
+      if (nw)
+        hipLaunchKernelGGL((k_step_begin<T, 1, true>), dim3(4 * L.blocks_f), dim3(128), 0, s, A,
+                           W, B, fa, p->cell_size, binning);
+      else
+        hipLaunchKernelGGL((k_step_begin<T, 1, false>), dim3(L.blocks_f), dim3(kBS), 0, s, A, W,
+                           B, fa, p->cell_size, binning);
+    }
   } else if (B.map_f) {
-    hipLaunchKernelGGL((k_step_begin<T, 0>), dim3(L.blocks_f), dim3(kBS), 0, s, A, W, B, fa,
-                       p->cell_size, binning);
+    hipLaunchKernelGGL((k_step_begin<T, 0, false>), dim3(L.blocks_f), dim3(kBS), 0, s, A, W, B,
+                       fa, p->cell_size, binning);
+  } else if (nw) {
+    hipLaunchKernelGGL((k_step_begin<T, 0, true>), dim3(4 * L.blocks_f), dim3(128), 0, s, A, W,
+                       B, fa, p->cell_size, binning);
   } else {  // one scenario: one-wave blocks, so the waves spread over all CUs
-    hipLaunchKernelGGL((k_step_begin<T, 0>), dim3(4 * L.blocks_f), dim3(64), 0, s, A, W, B, fa,
-                       p->cell_size, binning);
+    hipLaunchKernelGGL((k_step_begin<T, 0, false>), dim3(4 * L.blocks_f), dim3(64), 0, s, A, W,
+                       B, fa, p->cell_size, binning);
   }
   if (W.order_every > 0 && binning)
     hipLaunchKernelGGL(k_order_build, dim3(L.scenarios), dim3(kOrderThreads), 0, s, W, B);
