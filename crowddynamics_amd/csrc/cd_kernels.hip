@@ -804,20 +804,22 @@ __device__ __forceinline__ NavFetch own_nav(const FusedArgs &fa, double2 pos, co
 }
 
 // `local` is the agent's index within its scenario (the noise counter).
-// `noise`: the agent's fluctuation force x, y and torque computed elsewhere
-// (fluct_noise of the same inputs), or null to compute them here.
+// `pre`: the agent's fluctuation force x, y and torque (fluct_noise of the
+// same inputs), and whether the navigation lookup found a direction and
+// its x, y (nav_finish of the same fetch), computed elsewhere; or null to
+// compute them here.
 template <typename T, int MODEL>
 __device__ __forceinline__ void own_apply(const Agents<T> &A, const FusedArgs &fa, uint64_t seed,
                                           uint64_t step, int i, int local, const OwnIn<T> &o,
-                                          const NavFetch &nf, const double *noise = nullptr) {
+                                          const NavFetch &nf, const double *pre = nullptr) {
   V2<T> f = o.f;
   double tq = o.tq;
   if (fa.flags & CD_USE_FLUCT) {
     double nfx, nfy, nm;
-    if (noise) {
-      nfx = noise[0];
-      nfy = noise[1];
-      nm = noise[2];
+    if (pre) {
+      nfx = pre[0];
+      nfy = pre[1];
+      nm = pre[2];
     } else {
       fluct_noise(seed, step, local, o.mass, o.sig_f, MODEL != 0, o.inertia, o.sig_t, nfx, nfy,
                   nm);
@@ -827,8 +829,18 @@ __device__ __forceinline__ void own_apply(const Agents<T> &A, const FusedArgs &f
     if (MODEL) tq += nm;
   }
   V2<T> e = o.e;
-  double ex, ey;
-  if ((fa.flags & CD_USE_NAV) && nav_finish(nf, ex, ey)) {
+  double ex = 0.0, ey = 0.0;
+  bool found = false;
+  if (fa.flags & CD_USE_NAV) {
+    if (pre) {
+      found = pre[3] != 0.0;
+      ex = pre[4];
+      ey = pre[5];
+    } else {
+      found = nav_finish(nf, ex, ey);
+    }
+  }
+  if (found) {
     e.x = (T)ex;
     e.y = (T)ey;
     A.e0[i] = e;
@@ -858,14 +870,16 @@ __device__ __forceinline__ void own_apply(const Agents<T> &A, const FusedArgs &f
 // loaded before the reduction is read, and its noise is computed while its
 // slot atomic and navigation map loads are in flight.  The counter is read
 // here only; the force kernel advances it.
-// NW (one scenario, fluctuation on): blocks of two waves, the first serving
-// 64 agents and the second computing their noise at the same time (the
-// same fluct_noise of the same inputs, handed over in LDS), so a lone wave
-// per SIMD does not also run the noise's long dependent chain.
+// NW (one scenario, fluctuation or navigation on): blocks of two waves, the
+// first serving 64 agents and the second computing their noise and
+// navigation lookup at the same time (the same fluct_noise, nav_fetch and
+// nav_finish of the same inputs, handed over in LDS), so a lone wave per
+// SIMD does not also run their long dependent chains.
+constexpr int kPre = 6;  // handed-over values per agent
 template <typename T, int MODEL, bool NW>
 __global__ void __launch_bounds__(256) k_step_begin(Agents<T> A, Work W, Batch B, FusedArgs fa,
                                                     double cell, int binning) {
-  __shared__ double s_noise[NW ? 3 * 64 : 1];
+  __shared__ double s_pre[NW ? kPre * 64 : 1];
   const int2 sk = block_f(B);
   const ScnDesc d = desc_of(B, sk.x);
   ScnState *ss = &B.sst[sk.x];
@@ -941,7 +955,7 @@ __global__ void __launch_bounds__(256) k_step_begin(Agents<T> A, Work W, Batch B
       }
     }
     nf.ok = false;
-    if (fa.flags & CD_USE_NAV) nf = own_nav(fa, pos, o);
+    if (!NW && (fa.flags & CD_USE_NAV)) nf = own_nav(fa, pos, o);
   };
   auto finish = [&](int i, double2 pos, const OwnIn<T> &o, int c, int k, double ri,
                     const NavFetch &nf, const double *noise) {
@@ -975,12 +989,20 @@ __global__ void __launch_bounds__(256) k_step_begin(Agents<T> A, Work W, Batch B
       nf.ok = false;
       if (nwave) {
         if (have) {
-          double nx, ny, nm;
-          fluct_noise(d.seed, step, i - d.a0, A.mass[i], A.sig_f[i], MODEL != 0,
-                      MODEL ? A.inertia[i] : 0.0, MODEL ? A.sig_t[i] : 0.0, nx, ny, nm);
-          s_noise[lane] = nx;
-          s_noise[64 + lane] = ny;
-          s_noise[128 + lane] = nm;
+          double v[kPre] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+          NavFetch hf;
+          hf.ok = false;
+          if (fa.flags & CD_USE_NAV) {  // issued first: the noise runs while it loads
+            const double2 hp = A.pos[i];
+            hf = nav_fetch(hp.x, hp.y, A.target[i], fa.maps, fa.nav_T, fa.nav_rows, fa.nav_cols,
+                           fa.nav_minx, fa.nav_miny, fa.nav_step, fa.nav_mode);
+          }
+          if (fa.flags & CD_USE_FLUCT)
+            fluct_noise(d.seed, step, i - d.a0, A.mass[i], A.sig_f[i], MODEL != 0,
+                        MODEL ? A.inertia[i] : 0.0, MODEL ? A.sig_t[i] : 0.0, v[0], v[1], v[2]);
+          if (fa.flags & CD_USE_NAV) v[3] = nav_finish(hf, v[4], v[5]) ? 1.0 : 0.0;
+#pragma unroll
+          for (int u = 0; u < kPre; ++u) s_pre[u * 64 + lane] = v[u];
         }
       } else if (have) {
         if (it) {
@@ -989,10 +1011,12 @@ __global__ void __launch_bounds__(256) k_step_begin(Agents<T> A, Work W, Batch B
         }
         begin(i, pos, o, c, k, ri, nf);
       }
-      __syncthreads();  // noise of this round in LDS
+      __syncthreads();  // this round's values in LDS
       if (!nwave && have) {
-        const double noise[3] = {s_noise[lane], s_noise[64 + lane], s_noise[128 + lane]};
-        finish(i, pos, o, c, k, ri, nf, noise);
+        double v[kPre];
+#pragma unroll
+        for (int u = 0; u < kPre; ++u) v[u] = s_pre[u * 64 + lane];
+        finish(i, pos, o, c, k, ri, nf, v);
       }
       __syncthreads();  // LDS free for the next round
     }
@@ -1872,8 +1896,9 @@ void launch_step_begin(const cd_agents_t *a, const cd_step_params_t *p, const Wo
                        const Batch &B, const FusedArgs &fa, const Launch &L, hipStream_t s) {
   const Agents<T> A = typed<T>(a);
   const int binning = (p->flags & CD_USE_AGENT_AGENT) ? 1 : 0;
-  // one scenario with fluctuation: a serving and a noise wave per 64 agents
-  const bool nw = !B.map_f && (p->flags & CD_USE_FLUCT);
+  // one scenario with fluctuation or navigation: a serving and a helper
+  // wave per 64 agents
+  const bool nw = !B.map_f && (p->flags & (CD_USE_FLUCT | CD_USE_NAV));
   if (a->model == CD_THREE_CIRCLE) {
     if constexpr (sizeof(T) == 8) {
       if (nw)

@@ -333,11 +333,13 @@ def _stepper(scn, precision='f64', fluct=True):
 
 
 def test_step_one(hallway, room):
-    for scn in (hallway, room):
-        say('fused one step', scn.name)
+    """One fused step against the oracle: the hallway, and the room (its
+    navigation lookup) with and without fluctuation."""
+    for scn, fluct in ((hallway, True), (room, True), (room, False)):
+        say('fused one step', scn.name, 'fluctuation' if fluct else 'no fluctuation')
         ref = scn.agents.copy()
-        dts = O.run(ref, _oracle_scenario(scn), 1)
-        st = _stepper(scn)
+        dts = O.run(ref, _oracle_scenario(scn, fluct=fluct), 1)
+        st = _stepper(scn, fluct=fluct)
         st.step(1)
         out = st.download(scn.agents.copy())
         s = st.check()
