@@ -804,43 +804,22 @@ __device__ __forceinline__ NavFetch own_nav(const FusedArgs &fa, double2 pos, co
 }
 
 // `local` is the agent's index within its scenario (the noise counter).
-// `pre`: the agent's fluctuation force x, y and torque (fluct_noise of the
-// same inputs), and whether the navigation lookup found a direction and
-// its x, y (nav_finish of the same fetch), computed elsewhere; or null to
-// compute them here.
 template <typename T, int MODEL>
 __device__ __forceinline__ void own_apply(const Agents<T> &A, const FusedArgs &fa, uint64_t seed,
                                           uint64_t step, int i, int local, const OwnIn<T> &o,
-                                          const NavFetch &nf, const double *pre = nullptr) {
+                                          const NavFetch &nf) {
   V2<T> f = o.f;
   double tq = o.tq;
   if (fa.flags & CD_USE_FLUCT) {
     double nfx, nfy, nm;
-    if (pre) {
-      nfx = pre[0];
-      nfy = pre[1];
-      nm = pre[2];
-    } else {
-      fluct_noise(seed, step, local, o.mass, o.sig_f, MODEL != 0, o.inertia, o.sig_t, nfx, nfy,
-                  nm);
-    }
+    fluct_noise(seed, step, local, o.mass, o.sig_f, MODEL != 0, o.inertia, o.sig_t, nfx, nfy, nm);
     f.x += (T)nfx;
     f.y += (T)nfy;
     if (MODEL) tq += nm;
   }
   V2<T> e = o.e;
-  double ex = 0.0, ey = 0.0;
-  bool found = false;
-  if (fa.flags & CD_USE_NAV) {
-    if (pre) {
-      found = pre[3] != 0.0;
-      ex = pre[4];
-      ey = pre[5];
-    } else {
-      found = nav_finish(nf, ex, ey);
-    }
-  }
-  if (found) {
+  double ex, ey;
+  if ((fa.flags & CD_USE_NAV) && nav_finish(nf, ex, ey)) {
     e.x = (T)ex;
     e.y = (T)ey;
     A.e0[i] = e;
@@ -870,16 +849,15 @@ __device__ __forceinline__ void own_apply(const Agents<T> &A, const FusedArgs &f
 // loaded before the reduction is read, and its noise is computed while its
 // slot atomic and navigation map loads are in flight.  The counter is read
 // here only; the force kernel advances it.
-// NW (one scenario, fluctuation or navigation on): blocks of two waves, the
-// first serving 64 agents and the second computing their noise and
-// navigation lookup at the same time (the same fluct_noise, nav_fetch and
-// nav_finish of the same inputs, handed over in LDS), so a lone wave per
-// SIMD does not also run their long dependent chains.
-constexpr int kPre = 6;  // handed-over values per agent
-template <typename T, int MODEL, bool NW>
+// SPLIT (one scenario, fluctuation or navigation on): blocks of two waves
+// over the same 64 agents, the first binning them (snapshot, slot atomic,
+// slot list and record) and the second adding their own terms (noise,
+// navigation, adjusting).  The two parts share no data, so the waves run
+// side by side, and a lone wave per SIMD no longer runs both dependent
+// chains one after the other.
+template <typename T, int MODEL, bool SPLIT>
 __global__ void __launch_bounds__(256) k_step_begin(Agents<T> A, Work W, Batch B, FusedArgs fa,
                                                     double cell, int binning) {
-  __shared__ double s_pre[NW ? kPre * 64 : 1];
   const int2 sk = block_f(B);
   const ScnDesc d = desc_of(B, sk.x);
   ScnState *ss = &B.sst[sk.x];
@@ -888,19 +866,21 @@ __global__ void __launch_bounds__(256) k_step_begin(Agents<T> A, Work W, Batch B
   int *count = W.count + (size_t)p * W.cap;
   // blocks of the scenario: its block map's, or the whole grid for a single
   // scenario (launched as 64-thread blocks, one wave each, spread over all
-  // CUs; with NW, 128-thread blocks)
+  // CUs; 128-thread blocks with SPLIT)
   const int nblk = B.map_f ? d.nf : (int)gridDim.x;
-  const int per = NW ? 64 : (int)blockDim.x;  // agents a block serves at once
-  const bool nwave = NW && threadIdx.x >= 64;   // the noise wave
-  const int lane = NW ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
+  const int per = SPLIT ? 64 : (int)blockDim.x;  // agents of a block at once
+  const bool own_wave = SPLIT && threadIdx.x >= 64;
+  const bool bin_wave = !own_wave;
+  const int lane = SPLIT ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
   const int stride = nblk * per;
-  const int t0 = sk.y * per + lane;
-  const int i0 = d.a0 + t0;
+  const int i0 = d.a0 + sk.y * per + lane;
   double2 p0 = make_double2(0.0, 0.0);
   OwnIn<T> o0;
-  if (!nwave && i0 < d.a1) {
+  V2<T> v0;
+  if (i0 < d.a1) {
     p0 = A.pos[i0];
-    o0 = own_load<T, MODEL>(A, fa, i0, p0);
+    if (!SPLIT || own_wave) o0 = own_load<T, MODEL>(A, fa, i0, p0);
+    if (SPLIT && bin_wave) v0 = A.vel[i0];
   }
   {  // counts of the previous step's binning (every thread of the block)
     int *other = W.count + (size_t)(1 - p) * W.cap + d.c0;
@@ -908,11 +888,26 @@ __global__ void __launch_bounds__(256) k_step_begin(Agents<T> A, Work W, Batch B
     for (int c = sk.y * (int)blockDim.x + (int)threadIdx.x; c < clear; c += nblk * (int)blockDim.x)
       other[c] = 0;
   }
-  const Red r = acc_read(W, sk.x, p);
-  const Grid g = grid_of(r, cell, d.ccap);
   if (sk.y == 0)
     for (int sub = threadIdx.x; sub < kAccSub; sub += blockDim.x)
       acc_reset(acc_at(W, sk.x, 1 - p, sub));
+  // an agent's own terms (map loads issued before the noise is computed)
+  auto own = [&](int i, double2 pos, const OwnIn<T> &o) {
+    NavFetch nf;
+    nf.ok = false;
+    if (fa.flags & CD_USE_NAV) nf = own_nav(fa, pos, o);
+    own_apply<T, MODEL>(A, fa, d.seed, step, i, i - d.a0, o, nf);
+  };
+  if (own_wave) {  // needs no grid
+    if (i0 < d.a1) own(i0, p0, o0);
+    for (int i = i0 + stride; i < d.a1; i += stride) {
+      const double2 pos = A.pos[i];
+      own(i, pos, own_load<T, MODEL>(A, fa, i, pos));
+    }
+    return;
+  }
+  const Red r = acc_read(W, sk.x, p);
+  const Grid g = grid_of(r, cell, d.ccap);
   if (sk.y == 0 && threadIdx.x == 0) {
     publish_grid(W, ss, r, g, binning != 0, d.a1 > d.a0, p);
     if (!binning) ss->C_clear[p] = 0;
@@ -926,13 +921,12 @@ __global__ void __launch_bounds__(256) k_step_begin(Agents<T> A, Work W, Batch B
   // steps that rebuild the force kernel's processing order keep the slot ranks
   const bool ranks = W.order_every > 0 && step % (uint64_t)W.order_every == 0ull;
   V2<T> *snap_vel = (V2<T> *)W.snap_vel;
-  // an agent's snapshot, binning and map loads (begin), then its own terms
-  // and slot list entry (finish)
-  auto begin = [&](int i, double2 pos, const OwnIn<T> &o, int &c, int &k, double &ri,
-                   NavFetch &nf) {
+  // an agent's snapshot and slot atomic (bin_begin), then its slot list
+  // entry and neighbour record (bin_end)
+  auto bin_begin = [&](int i, double2 pos, V2<T> vel, int &c, int &k, double &ri) {
     ri = (MODEL == 0 && bin) ? A.radius[i] : 0.0;  // for the neighbour record
     W.snap_pos[i] = pos;
-    snap_vel[i] = o.vel;
+    snap_vel[i] = vel;
     if (MODEL) {
       W.snap_ls[i] = A.ls[i];
       W.snap_rs[i] = A.rs[i];
@@ -954,79 +948,39 @@ __global__ void __launch_bounds__(256) k_step_begin(Agents<T> A, Work W, Batch B
         if (ranks || MODEL == 0) W.rank[i] = k;
       }
     }
-    nf.ok = false;
-    if (!NW && (fa.flags & CD_USE_NAV)) nf = own_nav(fa, pos, o);
   };
-  auto finish = [&](int i, double2 pos, const OwnIn<T> &o, int c, int k, double ri,
-                    const NavFetch &nf, const double *noise) {
-    own_apply<T, MODEL>(A, fa, d.seed, step, i, i - d.a0, o, nf, noise);
-    if (c >= 0) {
-      if (k < W.K) {
-        W.ell_ids[c * W.K + k] = i;
-        if constexpr (MODEL == 0) {
-          NbrRec<T> rr;
-          rr.pos = pos;
-          rr.vel = o.vel;
-          rr.r = ri;
-          ((NbrRec<T> *)W.rec)[(size_t)c * W.K + k] = rr;
-        }
-      } else if (k == W.K)  // the cell's list is full: its members are found by scanning cell_of
-        atomicOr((unsigned long long *)&W.st->status, (unsigned long long)CD_STATUS_SLOTS_FULL);
+  auto bin_end = [&](int i, double2 pos, V2<T> vel, int c, int k, double ri) {
+    if (c < 0) return;
+    if (k < W.K) {
+      W.ell_ids[c * W.K + k] = i;
+      if constexpr (MODEL == 0) {
+        NbrRec<T> rr;
+        rr.pos = pos;
+        rr.vel = vel;
+        rr.r = ri;
+        ((NbrRec<T> *)W.rec)[(size_t)c * W.K + k] = rr;
+      }
+    } else if (k == W.K) {  // the cell's list is full: its members are found by scanning cell_of
+      atomicOr((unsigned long long *)&W.st->status, (unsigned long long)CD_STATUS_SLOTS_FULL);
     }
   };
-  if constexpr (NW) {
-    // rounds of this block: the same for both waves (barriers inside)
-    const int first = d.a0 + sk.y * per;
-    const int rounds = first < d.a1 ? (d.a1 - first + stride - 1) / stride : 0;
-    for (int it = 0; it < rounds; ++it) {
-      const int i = i0 + it * stride;
-      const bool have = i < d.a1;
-      double2 pos = p0;
-      OwnIn<T> o = o0;
-      int c = -1, k = 0;
-      double ri = 0.0;
-      NavFetch nf;
-      nf.ok = false;
-      if (nwave) {
-        if (have) {
-          double v[kPre] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-          NavFetch hf;
-          hf.ok = false;
-          if (fa.flags & CD_USE_NAV) {  // issued first: the noise runs while it loads
-            const double2 hp = A.pos[i];
-            hf = nav_fetch(hp.x, hp.y, A.target[i], fa.maps, fa.nav_T, fa.nav_rows, fa.nav_cols,
-                           fa.nav_minx, fa.nav_miny, fa.nav_step, fa.nav_mode);
-          }
-          if (fa.flags & CD_USE_FLUCT)
-            fluct_noise(d.seed, step, i - d.a0, A.mass[i], A.sig_f[i], MODEL != 0,
-                        MODEL ? A.inertia[i] : 0.0, MODEL ? A.sig_t[i] : 0.0, v[0], v[1], v[2]);
-          if (fa.flags & CD_USE_NAV) v[3] = nav_finish(hf, v[4], v[5]) ? 1.0 : 0.0;
-#pragma unroll
-          for (int u = 0; u < kPre; ++u) s_pre[u * 64 + lane] = v[u];
-        }
-      } else if (have) {
-        if (it) {
-          pos = A.pos[i];
-          o = own_load<T, MODEL>(A, fa, i, pos);
-        }
-        begin(i, pos, o, c, k, ri, nf);
-      }
-      __syncthreads();  // this round's values in LDS
-      if (!nwave && have) {
-        double v[kPre];
-#pragma unroll
-        for (int u = 0; u < kPre; ++u) v[u] = s_pre[u * 64 + lane];
-        finish(i, pos, o, c, k, ri, nf, v);
-      }
-      __syncthreads();  // LDS free for the next round
-    }
+  if constexpr (SPLIT) {
+    auto serve = [&](int i, double2 pos, V2<T> vel) {
+      int c, k;
+      double ri;
+      bin_begin(i, pos, vel, c, k, ri);
+      bin_end(i, pos, vel, c, k, ri);
+    };
+    if (i0 < d.a1) serve(i0, p0, v0);
+    for (int i = i0 + stride; i < d.a1; i += stride) serve(i, A.pos[i], A.vel[i]);
   } else {
+    // the own terms between the slot atomic and its use
     auto serve = [&](int i, double2 pos, const OwnIn<T> &o) {
       int c, k;
       double ri;
-      NavFetch nf;
-      begin(i, pos, o, c, k, ri, nf);
-      finish(i, pos, o, c, k, ri, nf, nullptr);
+      bin_begin(i, pos, o.vel, c, k, ri);
+      own(i, pos, o);
+      bin_end(i, pos, o.vel, c, k, ri);
     };
     if (i0 < d.a1) serve(i0, p0, o0);
     for (int i = i0 + stride; i < d.a1; i += stride) {
@@ -1896,7 +1850,7 @@ void launch_step_begin(const cd_agents_t *a, const cd_step_params_t *p, const Wo
                        const Batch &B, const FusedArgs &fa, const Launch &L, hipStream_t s) {
   const Agents<T> A = typed<T>(a);
   const int binning = (p->flags & CD_USE_AGENT_AGENT) ? 1 : 0;
-  // one scenario with fluctuation or navigation: a serving and a helper
+  // one scenario with fluctuation or navigation: a binning and an own-terms
   // wave per 64 agents
   const bool nw = !B.map_f && (p->flags & (CD_USE_FLUCT | CD_USE_NAV));
   if (a->model == CD_THREE_CIRCLE) {
