@@ -11,7 +11,8 @@
 //                         grid, dt; per agent: a snapshot of its state, the
 //                         step's own terms (fluctuation, navigation,
 //                         orientation, adjusting) and its insert into its
-//                         cell's slot list
+//                         cell's slot list (a single scenario: binning and
+//                         own terms on two waves per 64 agents)
 //   K_e  k_forces_direct  L = 4 to 64 lanes per agent: the pairs of its
 //        (k_forces_tiled) 3 x 3 cell stencil and the walls, summed exactly;
 //                         then Verlet and Reset of the agent, the block's
