@@ -163,3 +163,24 @@ def test_autotuned_cell_sizes_are_physics_neutral():
     assert best['cell_size'] in sizes[:2] and best['tile'] in ((0, 0), (2, 2))
     assert best['order_every'] in (0, 5)
     assert len(best['table']) == 6
+
+
+@pytest.mark.parametrize('precision', ['f64', 'f32'])
+def test_batch_of_one_room_equals_standalone(precision):
+    """A batch of one 5000-agent room (navigation and fluctuation on) equals
+    its standalone run bit for bit over 20 steps: the batched step bins and
+    adds the own terms on the same wave, the standalone step on two waves
+    per 64 agents."""
+    scn = E.room(n=5000, width=100.0, height=50.0)
+    steps = 20
+    stream = torch.cuda.Stream()
+    say('batch of one room,', scn.n, 'agents,', precision)
+    with torch.cuda.stream(stream):
+        ens = Ensemble([scn], precision=precision, cell_size=scn.cell_size)
+        ens.step(steps)
+        ens.check()
+        outs = ens.download_scenarios()
+        states = ens.states()
+    out, state = _standalone(scn, steps, precision, ens.cell_size, stream)
+    assert states[0].dt == state.dt and states[0].time_tot == state.time_tot
+    _compare(outs, [out], FIELDS)
