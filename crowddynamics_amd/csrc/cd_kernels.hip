@@ -850,11 +850,11 @@ __device__ __forceinline__ void own_apply(const Agents<T> &A, const FusedArgs &f
 // loaded before the reduction is read, and its noise is computed while its
 // slot atomic and navigation map loads are in flight.  The counter is read
 // here only; the force kernel advances it.
-// SPLIT (one scenario, fluctuation or navigation on): blocks of two waves
-// over the same 64 agents, the first binning them (snapshot, slot atomic,
-// slot list and record) and the second adding their own terms (noise,
-// navigation, adjusting).  The two parts share no data, so the waves run
-// side by side, and a lone wave per SIMD no longer runs both dependent
+// SPLIT (fluctuation or navigation on): the first half of each block bins
+// its agents (snapshot, slot atomic, slot list and record) and the second
+// half adds the same agents' own terms (noise, navigation, adjusting).  The
+// two parts share no data, so the halves run side by side, and a lone wave
+// per SIMD (one scenario: 128-thread blocks) no longer runs both dependent
 // chains one after the other.
 template <typename T, int MODEL, bool SPLIT>
 __global__ void __launch_bounds__(256) k_step_begin(Agents<T> A, Work W, Batch B, FusedArgs fa,
@@ -869,10 +869,10 @@ __global__ void __launch_bounds__(256) k_step_begin(Agents<T> A, Work W, Batch B
   // scenario (launched as 64-thread blocks, one wave each, spread over all
   // CUs; 128-thread blocks with SPLIT)
   const int nblk = B.map_f ? d.nf : (int)gridDim.x;
-  const int per = SPLIT ? 64 : (int)blockDim.x;  // agents of a block at once
-  const bool own_wave = SPLIT && threadIdx.x >= 64;
+  const int per = SPLIT ? (int)blockDim.x / 2 : (int)blockDim.x;  // agents of a block at once
+  const bool own_wave = SPLIT && (int)threadIdx.x >= per;
   const bool bin_wave = !own_wave;
-  const int lane = SPLIT ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
+  const int lane = own_wave ? (int)threadIdx.x - per : (int)threadIdx.x;
   const int stride = nblk * per;
   const int i0 = d.a0 + sk.y * per + lane;
   double2 p0 = make_double2(0.0, 0.0);
@@ -1851,24 +1851,28 @@ void launch_step_begin(const cd_agents_t *a, const cd_step_params_t *p, const Wo
                        const Batch &B, const FusedArgs &fa, const Launch &L, hipStream_t s) {
   const Agents<T> A = typed<T>(a);
   const int binning = (p->flags & CD_USE_AGENT_AGENT) ? 1 : 0;
-  // one scenario with fluctuation or navigation: a binning and an own-terms
-  // wave per 64 agents
-  const bool nw = !B.map_f && (p->flags & (CD_USE_FLUCT | CD_USE_NAV));
+  // with fluctuation or navigation: a binning and an own-terms half per
+  // block over the same agents (one scenario: 128-thread blocks, 64 agents
+  // each, so the waves spread over all CUs; a batch: the block map's
+  // 256-thread blocks)
+  const bool split = (p->flags & (CD_USE_FLUCT | CD_USE_NAV)) != 0;
+  const bool one = !B.map_f;
+  const dim3 grid_s(one ? 4 * L.blocks_f : L.blocks_f), block_s(one ? 128 : kBS);
   if (a->model == CD_THREE_CIRCLE) {
     if constexpr (sizeof(T) == 8) {
-      if (nw)
-        hipLaunchKernelGGL((k_step_begin<T, 1, true>), dim3(4 * L.blocks_f), dim3(128), 0, s, A,
-                           W, B, fa, p->cell_size, binning);
+      if (split)
+        hipLaunchKernelGGL((k_step_begin<T, 1, true>), grid_s, block_s, 0, s, A, W, B, fa,
+                           p->cell_size, binning);
       else
         hipLaunchKernelGGL((k_step_begin<T, 1, false>), dim3(L.blocks_f), dim3(kBS), 0, s, A, W,
                            B, fa, p->cell_size, binning);
     }
-  } else if (B.map_f) {
+  } else if (split) {
+    hipLaunchKernelGGL((k_step_begin<T, 0, true>), grid_s, block_s, 0, s, A, W, B, fa,
+                       p->cell_size, binning);
+  } else if (!one) {
     hipLaunchKernelGGL((k_step_begin<T, 0, false>), dim3(L.blocks_f), dim3(kBS), 0, s, A, W, B,
                        fa, p->cell_size, binning);
-  } else if (nw) {
-    hipLaunchKernelGGL((k_step_begin<T, 0, true>), dim3(4 * L.blocks_f), dim3(128), 0, s, A, W,
-                       B, fa, p->cell_size, binning);
   } else {  // one scenario: one-wave blocks, so the waves spread over all CUs
     hipLaunchKernelGGL((k_step_begin<T, 0, false>), dim3(4 * L.blocks_f), dim3(64), 0, s, A, W,
                        B, fa, p->cell_size, binning);
