@@ -225,12 +225,31 @@ def bench_corridor(steps, stream):
     from crowddynamics_amd.stepper import Stepper
     log('corridor: 50k three-circle agents')
     scn = E.corridor()
+    G = 50
+
+    def stepper(lanes):
+        return Stepper(scn.agents.copy(), segments=scn.segments, cell_size=scn.cell_size,
+                       dt_min=scn.dt_min, dt_max=scn.dt_max, seed=scn.seed, precision='f64',
+                       bounds=scn.bounds, lanes=lanes)
+
     with torch.cuda.stream(stream):
-        st = Stepper(scn.agents.copy(), segments=scn.segments, cell_size=scn.cell_size,
-                     dt_min=scn.dt_min, dt_max=scn.dt_max, seed=scn.seed, precision='f64',
-                     bounds=scn.bounds)
+        # lanes per agent: 2 or 4, by one graph of G steps each (same results)
+        lane_ms = {}
+        for lanes in (2, 4):
+            st = stepper(lanes)
+            st.step(10)
+            graph = st.capture(G)
+            graph.launch()
+            stream.synchronize()
+            t0 = time.perf_counter()
+            graph.launch()
+            stream.synchronize()
+            lane_ms[lanes] = 1e3 * (time.perf_counter() - t0) / G
+            del graph, st
+        lanes = min(lane_ms, key=lane_ms.get)
+        log('corridor: ms per step by lanes %s' % lane_ms)
+        st = stepper(lanes)
         st.step(10)
-        G = 50
         graph = st.capture(G)
         reps = max(1, steps // G)
         stream.synchronize()
@@ -245,8 +264,8 @@ def bench_corridor(steps, stream):
     v = scn.n * reps * G / el
     log('corridor: %.3e agent-steps/s' % v)
     return dict(agents=scn.n, steps=reps * G, value=v, unit='agent-steps/s',
-                ms_per_step=1e3 * el / (reps * G), precision='f64',
-                kernel_ms={k: kms[i] / 20 for k, i in KERNELS.items()})
+                ms_per_step=1e3 * el / (reps * G), precision='f64', lanes=lanes,
+                lane_ms=lane_ms, kernel_ms={k: kms[i] / 20 for k, i in KERNELS.items()})
 
 
 def bench_adjoint(n, steps, checkpoint):

@@ -179,7 +179,7 @@ __device__ __forceinline__ void stencil_vjp(const Agents<double> &A, const Work 
   }
 }
 
-// L lanes per agent (4 to 64) as in the forward force kernel: lane q
+// L lanes per agent (2 to 64) as in the forward force kernel: lane q
 // takes part q of the stencil's candidates; lane 0 adds the agent's own
 // (adjusting) terms and writes the results.
 __global__ void __launch_bounds__(256) k_adjoint(Agents<double> A, Work W, Batch B, AdjView J,

@@ -94,7 +94,7 @@ inline int64_t max_blocks_f(int64_t n, int64_t S) {
   return std::max<int64_t>(kSingleParts, n / kBS + S + 1);
 }
 inline int64_t max_blocks_e(int64_t n, int64_t S) { return n / kMinForceAgents + S + 1; }
-// lanes per agent of the agent-parallel force kernel (4, 8, 16, 32 or 64; 0: 4)
+// lanes per agent of the agent-parallel force kernel (2, 4, 8, 16, 32 or 64; 0: 4)
 inline int agent_lanes(const cd_workspace_t *w) { return w->agent_lanes > 0 ? (int)w->agent_lanes : 4; }
 
 struct Layout {
@@ -755,7 +755,7 @@ bool ws_sizes_ok(const cd_workspace_t *w) {
          (w->force_threads == 0 || w->force_threads == 64 || w->force_threads == 128 ||
           w->force_threads == 256) &&
          w->tile_x >= 0 && w->tile_x <= 16 && w->tile_y >= 0 && w->tile_y <= 16 &&
-         (w->agent_lanes == 0 || w->agent_lanes == 4 || w->agent_lanes == 8 ||
+         (w->agent_lanes == 0 || w->agent_lanes == 2 || w->agent_lanes == 4 || w->agent_lanes == 8 ||
           w->agent_lanes == 16 || w->agent_lanes == 32 || w->agent_lanes == 64) &&
          w->order_every >= 0 &&
          (w->model == CD_CIRCULAR || w->model == CD_THREE_CIRCLE);

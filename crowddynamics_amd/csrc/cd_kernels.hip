@@ -13,7 +13,7 @@
 //                         orientation, adjusting) and its insert into its
 //                         cell's slot list (a single scenario: binning and
 //                         own terms on two waves per 64 agents)
-//   K_e  k_forces_direct  L = 4 to 64 lanes per agent: the pairs of its
+//   K_e  k_forces_direct  L = 2 to 64 lanes per agent: the pairs of its
 //        (k_forces_tiled) 3 x 3 cell stencil and the walls, summed exactly;
 //                         then Verlet and Reset of the agent, the block's
 //                         reduction of the new state merged into the
@@ -514,7 +514,7 @@ __global__ void __launch_bounds__(256) k_walls(Agents<T> A, int model, const dou
 // K_e: forces (owner-computes over the 3x3 cell stencil)
 // ---------------------------------------------------------------------------
 //
-// L = 4, 8, 16, 32 or 64 lanes serve one agent: the candidates of its 3 x 3 cell
+// L = 2, 4, 8, 16, 32 or 64 lanes serve one agent: the candidates of its 3 x 3 cell
 // stencil, taken as one sequence, are split evenly over the lanes, and so
 // are the wall segments.  (The agent's own terms were added by the step's
 // first kernel.)  Pair and wall terms are
@@ -1341,7 +1341,7 @@ __device__ __forceinline__ Self<T> self_of(const Agents<T> &A, int i, double2 po
   return s;
 }
 
-// Pair and wall forces of agent i, served by L lanes (L = 4 to 64; q =
+// Pair and wall forces of agent i, served by L lanes (L = 2 to 64; q =
 // lane in the group): lane q takes part q of the stencil's candidates
 // (when `pairs`) and every L-th wall from q (fused step with walls).  Pair
 // and wall terms are summed exactly, so results do not depend on L.  Every

@@ -86,7 +86,7 @@ typedef struct cd_workspace {
                              LDS tile, each 1..16 (same results either way) */
   int32_t blocks_t;       /* set by cd_batch_prepare */
   int32_t agent_lanes;    /* lanes per agent of the agent-parallel force kernel and the
-                             adjoint: 4, 8, 16, 32 or 64 (0: 4); same forces either way
+                             adjoint: 2, 4, 8, 16, 32 or 64 (0: 4); same forces either way
                              (the adjoint's lane sums are in fp64) */
   int32_t order_every;    /* agent-parallel force kernel of the fused step, and the
                              adjoint kernel: 0 = agents in index order; k > 0 = in cell

@@ -198,11 +198,12 @@ def test_adjoint_central_fd_persistent_contacts():
 
 
 def test_adjoint_lanes_per_agent():
-    """16, 32 and 64 lanes per agent (the stencil's candidates split in more
-    parts) give the gradients of 4 lanes up to summation order."""
+    """2, 16, 32 and 64 lanes per agent (the stencil's candidates split in
+    fewer or more parts) give the gradients of 4 lanes up to summation
+    order."""
     arr = AC.packed(40, seed=8)
     out = []
-    for lanes in (4, 16, 32, 64):
+    for lanes in (4, 2, 16, 32, 64):
         R = Rollout(arr, steps=20, dt=DT, cell_size=CELL, checkpoint=6, lanes=lanes)
         Xhat = _targets(R, arr, seed=4)
         out.append(_adjoint_grads(R, arr, Xhat)[1])

@@ -493,12 +493,12 @@ def test_step_processing_order(room, corridor_small, hallway):
 
 
 def test_step_wide_lane_groups(hallway, corridor_small):
-    """32 and 64 lanes per agent (one agent per half wave or per wave) give
-    the 4-lane step bit for bit: hallway (circular) and corridor
-    (three-circle), 5 fused steps, fp64."""
+    """2, 32 and 64 lanes per agent (32 agents per wave, one agent per half
+    wave or per wave) give the 4-lane step bit for bit: hallway (circular)
+    and corridor (three-circle), 5 fused steps, fp64."""
     for scn in (hallway, corridor_small):
         outs = []
-        for lanes in (4, 32, 64):
+        for lanes in (4, 2, 32, 64):
             say('%s 5 steps, %d lanes per agent' % (scn.name, lanes))
             st = Stepper(scn.agents.copy(), segments=scn.segments, nav=scn.nav,
                          cell_size=scn.cell_size, dt_min=scn.dt_min, dt_max=scn.dt_max,
