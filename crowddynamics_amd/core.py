@@ -25,6 +25,21 @@ def _check_agents(agents):
         raise InvalidType('expected DeviceAgents')
 
 
+def _on_agents_device(t, agents, what):
+    """`t` as a contiguous fp64 tensor on the agents' device.  A host
+    pointer or a tensor of another dtype must never reach a kernel (the
+    kernels read fp64 device memory), so other tensors are converted here;
+    anything that is not a tensor of real numbers raises InvalidValue."""
+    if not isinstance(t, torch.Tensor):
+        raise InvalidValue('%s must be a torch.Tensor' % what)
+    if t.is_complex() or t.dtype == torch.bool:
+        raise InvalidValue('%s must hold real numbers, not %s' % (what, t.dtype))
+    dev = agents.device
+    if dev.type == 'cuda' and dev.index is None:
+        dev = torch.device('cuda', torch.cuda.current_device())
+    return t.to(device=dev, dtype=torch.float64).contiguous()
+
+
 def segments_to_device(obstacles, device='cuda'):
     """obstacle_type_linear[W] (or [W, 4]) -> fp64 [W, 4] device tensor."""
     if isinstance(obstacles, torch.Tensor):
@@ -61,7 +76,13 @@ def agent_obstacle(agents, obstacles):
     is a [W, 4] fp64 device tensor (segments_to_device)."""
     _check_agents(agents)
     if not isinstance(obstacles, torch.Tensor):
-        obstacles = segments_to_device(obstacles)
+        obstacles = segments_to_device(obstacles, agents.device)
+    if obstacles.dim() == 3 and tuple(obstacles.shape[1:]) == (2, 2):
+        obstacles = obstacles.reshape(-1, 4)
+    if obstacles.dim() != 2 or obstacles.shape[1] != 4:
+        raise InvalidValue('obstacles must be [W, 4] (p0x, p0y, p1x, p1y), got %s'
+                           % (tuple(obstacles.shape),))
+    obstacles = _on_agents_device(obstacles, agents, 'obstacles')
     n_seg = obstacles.shape[0]
     ptr = C.c_void_p(obstacles.data_ptr() if n_seg else 0)
     _lib.check(_lib.lib().cd_agent_obstacle(agents.ref(), ptr, n_seg, stream_handle()),
@@ -97,11 +118,17 @@ def navigation(agents, maps, minx, miny, step, mode=0):
     maps read at the truncated grid index (reference behaviour).  mode 1:
     `maps` is [T, rows, cols] potentials read by bilinear gradient."""
     _check_agents(agents)
+    if mode not in (0, 1):
+        raise InvalidValue('navigation mode must be 0 (nearest node) or 1 (bilinear)')
+    if not isinstance(maps, torch.Tensor):
+        maps = torch.as_tensor(np.asarray(maps, dtype=np.float64))
     if mode == 0 and (maps.dim() != 4 or maps.shape[-1] != 2):
         raise InvalidValue('direction maps must be [T, rows, cols, 2]')
     if mode == 1 and maps.dim() != 3:
         raise InvalidValue('potentials must be [T, rows, cols]')
-    maps = maps.contiguous()
+    if min(maps.shape[:3]) < 1 or not step > 0:
+        raise InvalidValue('navigation maps must be non-empty with a positive step')
+    maps = _on_agents_device(maps, agents, 'navigation maps')
     T_, rows, cols = maps.shape[:3]
     _lib.check(_lib.lib().cd_navigation(agents.ref(), C.c_void_p(maps.data_ptr()), T_, rows,
                                         cols, float(minx), float(miny), float(step), int(mode),

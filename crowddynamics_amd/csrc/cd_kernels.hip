@@ -939,6 +939,9 @@ __global__ void __launch_bounds__(256) k_step_begin(Agents<T> A, Work W, Batch B
       const int lc = cell_of_pos(pos, g, cell);
       if (lc < 0) {  // positions changed after the bounding box was reduced
         atomicOr(&ss->oob[p], 1);
+        // the force kernel skips this scenario's pairs for the step: report it
+        atomicOr((unsigned long long *)&W.st->status,
+                 (unsigned long long)CD_STATUS_GRID_OVERFLOW);
         W.cell_of[i] = -1;
       } else {
         c = d.c0 + lc;

@@ -21,16 +21,32 @@ Parity pinning of each oracle part (what checks it besides the GPU):
                            -> 0, head-on -> non-zero), 2 kN truncation
   power law (three-circle) tau = min over the part pairs by polynomial roots
                            where the reference's selection rule reduces to it;
-                           the negative-tau lock-in quirk by a constructed case
+                           the negative-tau lock-in quirk by a constructed case;
+                           the shoulder-offset branch (r_ts != 0) by the
+                           identity grad = -grad tau(Xp) - 3 r_off / d with
+                           grad tau by central differences of numpy roots
+                           (the 3 r_off / d term, from the offset's sign and
+                           its factor 2, is reference-defined), rotation
+                           equivariance and phi -> phi + pi relabelling
+                           invariance (tests/test_oracle_pins.py)
   contact                  physical special cases (pure compression mu*delta,
                            pure sliding kappa*delta*v_t, pure approach c*v_n)
                            and Newton's third law with equal parameters
   binning                  numpy stable argsort by cell; pair set equals the
                            brute-force set {i<j : |dlx|<=1 and |dly|<=1}
-  adjust / integrator      closed-form constant-force trajectories of the
-                           reference's Verlet ordering; dt in [dt_min, dt_max];
-                           the reference's validation runs (TestMovement >= 10 m,
-                           TestAgentInteraction >= 8 m)
+  integrator               closed-form constant-force and constant-torque
+                           trajectories of the reference's Verlet ordering; dt
+                           in [dt_min, dt_max]; the reference's validation runs
+                           (TestMovement >= 10 m, TestAgentInteraction >= 8 m)
+  force_adjust             one agent from rest against the closed form of the
+                           Verlet recurrence w_{k+1} = w_k - dt/(2 tau)(w_k +
+                           w_{k-1}) by its characteristic roots (position,
+                           velocity, previous force to 1e-12)
+  torque_adjust            the rotational recurrence solved by eigen-
+                           decomposition, across +-pi (short way) and with
+                           omega0 != 0 and omega0 = 0
+  orientation              bit-equal to math.atan2 on the axes, quadrants and
+                           signed zeros
   navigation               exact lattice positions (nearest node) and exact
                            bilinear potentials (gradient mode)
   fluctuation              Philox4x32-10 against its published known-answer

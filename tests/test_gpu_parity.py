@@ -575,6 +575,36 @@ def test_step_grid_overflow_reported(hallway, order_every):
         assert np.array_equal(outs[-1][f], outs[0][f])
 
 
+def test_step_stale_box_reported(hallway):
+    """Positions edited on the host between steps without prepare(): an agent
+    now lies outside the grid of the bounding box the previous step reduced.
+    The step's first kernel cannot bin it, the force kernel skips the pairs
+    (walls and integration still run) and the condition is reported as
+    GridOverflow.  After prepare() the run continues cleanly."""
+    from crowddynamics_amd.stepper import GridOverflow
+    say('fused step after a host edit outside the reduced bounding box')
+    st = Stepper(hallway.agents.copy(), segments=hallway.segments,
+                 cell_size=hallway.cell_size, dt_min=hallway.dt_min, dt_max=hallway.dt_max,
+                 seed=hallway.seed, bounds=hallway.bounds)
+    st.step(1)
+    st.check()
+    arr = st.download(hallway.agents.copy())
+    arr['position'][5] = arr['position'][5] + np.array([40.0, 0.0])
+    st.agents.upload(arr, ['position'])
+    st.step(1)
+    with pytest.raises(GridOverflow):
+        st.check()
+    out = st.download(hallway.agents.copy())
+    assert np.all(np.isfinite(out['position']))
+    assert st.state().step == 2
+    out['position'][5] = out['position'][5] - np.array([40.0, 0.0])
+    st.agents.upload(out, ['position'])
+    st.ws.clear_status()
+    st.prepare(2)
+    st.step(1)
+    st.check()
+
+
 @pytest.mark.parametrize('tile', [(1, 1), (3, 2)])
 def test_step_tiled_equals_direct(hallway, tile):
     """Ten fused steps with the tiled force kernel (which integrates the
