@@ -21,6 +21,9 @@ CD_ERR_WORKSPACE = -3
 CD_ERR_UNSUPPORTED = -4
 CD_ERR_HIP = -5
 
+#: bytes of the library's device scratch array of the allocation-free entry point
+NOALLOC_SCRATCH_BYTES = 64 << 20
+
 CD_STATUS_GRID_OVERFLOW = 1
 CD_STATUS_NONFINITE = 2
 CD_STATUS_SLOTS_FULL = 4
@@ -102,6 +105,8 @@ SIGNATURES = {
     'cd_clear_status': (C.c_int, [_W, _vp]),
     'cd_add_to_cells': (C.c_int, [_P(CDAgents), C.c_double, _W, _vp]),
     'cd_agent_agent_block_list': (C.c_int, [_P(CDAgents), C.c_double, _W, _vp]),
+    'cd_agent_agent_block_list_noalloc': (C.c_int, [_P(CDAgents), C.c_double, C.c_int64,
+                                                    C.c_int64, _vp, _vp]),
     'cd_agent_obstacle': (C.c_int, [_P(CDAgents), _vp, C.c_int64, _vp]),
     'cd_force_adjust_agents': (C.c_int, [_P(CDAgents), _vp]),
     'cd_torque_adjust_agents': (C.c_int, [_P(CDAgents), _vp]),

@@ -1988,6 +1988,41 @@ int build_batch(const cd_scenario_t *sc, int S, const cd_workspace_t *w, int64_t
   return CD_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free AgentAgentInteractions: the workspace of
+// cd_agent_agent_block_list lives in `scratch`, a device array the library
+// holds on every GPU it runs on until the process ends.  Host code reaches
+// the array only through the address hipGetSymbolAddress returns for the
+// current device (kept per device below); the array's own name is not a
+// device address on the host.
+// ---------------------------------------------------------------------------
+
+constexpr int64_t kScratchBytes = CD_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's copy of `scratch` (looked up once per device).
+int scratch_ptr(void **out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return CD_ERR_HIP;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return CD_ERR_HIP;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return CD_OK;
+}
+
+// The call's status bits to the caller's word (0 when st is null: no agents).
+__global__ void k_status_out(const DevState *st, int64_t *out) {
+  *out = st ? (int64_t)st->status : 0;
+}
+
 __global__ void k_clear_status(DevState *st) { st->status = 0; }
 __global__ void k_set_step(DevState *st, unsigned long long step) { st->step = step; }
 __global__ void k_philox(const uint32_t *in, uint32_t *out) {
@@ -2096,6 +2131,51 @@ int cd_agent_agent_block_list(const cd_agents_t *a, double cell_size, const cd_w
   if (a->n == 0) return CD_OK;
   if (a->real == CD_F64) return do_agent_agent<double>(a, cell_size, w, (hipStream_t)stream);
   return do_agent_agent<float>(a, cell_size, w, (hipStream_t)stream);
+}
+
+int cd_agent_agent_block_list_noalloc(const cd_agents_t *a, double cell_size,
+                                      int64_t cell_capacity, int64_t cell_slots,
+                                      int64_t *status_out, cd_stream_t stream) {
+  int rc = check_agents(a);
+  if (rc) return rc;
+  if (!(cell_size > 0.0)) return CD_ERR_ARG;
+  cd_workspace_t w;
+  memset(&w, 0, sizeof(w));
+  w.n = a->n;
+  w.cell_capacity = cell_capacity;
+  w.cell_slots = cell_slots;
+  w.model = a->model;
+  if (!ws_sizes_ok(&w)) return CD_ERR_ARG;
+  if (make_layout(&w).total > kScratchBytes) return CD_ERR_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  if (a->n == 0) {
+    if (status_out) {
+      hipLaunchKernelGGL(k_status_out, dim3(1), dim3(1), 0, s, (const DevState *)nullptr,
+                         status_out);
+      return check_launch();
+    }
+    return CD_OK;
+  }
+  void *p = nullptr;
+  if ((rc = scratch_ptr(&p))) return rc;
+  w.ptr = p;
+  w.bytes = kScratchBytes;
+  // the previous call on this GPU left its state, counts and scan tiles
+  // (at the offsets of its own sizes): clear them for this call's layout
+  const Work W = carve(&w);
+  const Layout L = make_layout(&w);
+  hipLaunchKernelGGL(k_ws_init, dim3(std::min(grid_for(w.cell_capacity + 2, kBS), 4096)),
+                     dim3(kBS), 0, s, W, (ScnState *)((char *)p + L.scn_state), 1);
+  if ((rc = check_launch())) return rc;
+  rc = (a->real == CD_F64) ? do_agent_agent<double>(a, cell_size, &w, s)
+                           : do_agent_agent<float>(a, cell_size, &w, s);
+  if (rc) return rc;
+  if (status_out) {
+    hipLaunchKernelGGL(k_status_out, dim3(1), dim3(1), 0, s, (const DevState *)W.st,
+                       status_out);
+    return check_launch();
+  }
+  return CD_OK;
 }
 
 int cd_agent_obstacle(const cd_agents_t *a, const double *segments, int64_t W,

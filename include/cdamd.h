@@ -161,6 +161,23 @@ int cd_add_to_cells(const cd_agents_t *agents, double cell_size, const cd_worksp
 int cd_agent_agent_block_list(const cd_agents_t *agents, double cell_size,
                               const cd_workspace_t *ws, cd_stream_t stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): cd_agent_agent_block_list with its workspace in a
+ * library-owned device array of CD_NOALLOC_SCRATCH_BYTES, held on every GPU
+ * the library runs on until the process ends.  The call allocates nothing.
+ * cell_capacity and cell_slots size the grid as in cd_workspace_t; sizes
+ * whose workspace (cd_workspace_bytes with max_scenarios 0) exceeds the
+ * array return CD_ERR_WORKSPACE.  Run it with the stream's device current.
+ * The call's CD_STATUS_* bits go to *status_out (a device int64, may be
+ * NULL), written stream-ordered at the end of the call.  Every call on one
+ * GPU shares its array: calls that overlap on one GPU (other streams, other
+ * host threads) race and must be serialised by the caller. */
+#define CD_NOALLOC_SCRATCH_BYTES (64ll << 20)
+int cd_agent_agent_block_list_noalloc(const cd_agents_t *agents, double cell_size,
+                                      int64_t cell_capacity, int64_t cell_slots,
+                                      int64_t *status_out, cd_stream_t stream);
+
 /* agent_obstacle(agents, obstacles) (crowddynamics/core/interactions.py:208-214):
  * segments is obstacle_type_linear[W] = {p0: f8[2], p1: f8[2]}, [W][4] fp64. */
 int cd_agent_obstacle(const cd_agents_t *agents, const double *segments, int64_t n_segments,
