@@ -146,16 +146,95 @@ def cpu_baseline(scn, seconds, threads=None):
     return scn.n * steps / el, steps, el
 
 
+def pair_tally_three_circle(arr, cell, segments):
+    """pair_tally for three-circle agents, by this build's count of the
+    three-circle pair (same convention: each add/sub/mul/div/sqrt/exp/hypot
+    is one op): 9 part distances (4 ops each) and the sight test, 38 ops
+    when rejected (h >= 3 at the closest part pair); the time to collision
+    of 9 part pairs (16 ops each), the gradient, magnitude, shoulder offset
+    and truncation, 230 ops when social; 280 with contact and torque.
+    Walls: 3 parts x the circle-segment distance, 66 ops, 96 with contact."""
+    from scipy.spatial import cKDTree
+    p = arr['position']
+    l = np.floor((p - p.min(0)) / cell).astype(np.int64)
+    pairs = cKDTree(p).query_pairs(r=2.0 * np.sqrt(2.0) * cell * (1.0 + 1e-9),
+                                   output_type='ndarray').reshape(-1, 2)
+    pairs = pairs[np.all(np.abs(l[pairs[:, 0]] - l[pairs[:, 1]]) <= 1, axis=1)]
+    i, j = pairs[:, 0], pairs[:, 1]
+    parts = (arr['position'], arr['position_ls'], arr['position_rs'])
+    radii = (arr['r_t'], arr['r_s'], arr['r_s'])
+    h = np.full(len(pairs), np.inf)
+    for a in range(3):
+        for b in range(3):
+            d = np.hypot(*(parts[a][i] - parts[b][j]).T) - (radii[a][i] + radii[b][j])
+            h = np.minimum(h, d)
+    n_rej = int(np.sum(h >= 3.0))
+    n_con = int(np.sum(h < 0.0))
+    n_soc = len(h) - n_rej - n_con
+    seg = np.concatenate([segments['p0'], segments['p1']], 1) if len(segments) else np.zeros((0, 4))
+    from crowddynamics_amd.examples import _seg_dist
+    w_con = 0
+    if len(seg):
+        dmin = np.full((len(p), len(seg)), np.inf)
+        for a in range(3):
+            dmin = np.minimum(dmin, _seg_dist(parts[a][:, 0], parts[a][:, 1], seg)
+                              - radii[a][:, None])
+        w_con = int(np.sum(dmin < 0))
+    w_all = len(p) * len(seg)
+    flops = 38 * n_rej + 230 * n_soc + 280 * n_con + 66 * (w_all - w_con) + 96 * w_con
+    return dict(candidate_pairs=len(h), social_pairs=n_soc, contact_pairs=n_con,
+                wall_pairs=w_all, wall_contacts=w_con, flops=flops)
+
+
+def _window(st, graph, G, reps, stream, restore):
+    """Per-kernel GPU time and throughput over the same steps: the window of
+    reps * G steps is first run eagerly with HIP events around every kernel
+    (cd_step_timed), then the state is restored and the same window is
+    replayed from the graph and timed on the host.  Returns (kernel ms per
+    step by name, elapsed seconds of the replay)."""
+    import torch
+    kms = st.timed(reps * G)
+    restore()
+    stream.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        graph.launch()
+    stream.synchronize()
+    el = time.perf_counter() - t0
+    torch.cuda.synchronize()
+    return {k: kms[i] / (reps * G) for k, i in KERNELS.items()}, el
+
+
+def _snapshot(st, array):
+    """A restore() for `st`: its agents' records and step counter now."""
+    arr = st.download(array.copy())
+    step = st.state().step
+
+    def restore():
+        st.agents.upload(arr)
+        st.prepare(step)
+    return restore
+
+
+def _force_roofline(n, ms, bytes_, flops, cand):
+    return dict(bound='hbm', unit='GB/s', ms=ms, bytes=bytes_,
+                achieved=bytes_ / (ms * 1e-3) / 1e9, peak=HBM_PEAK_GBS,
+                frac=bytes_ / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                flops=flops, flops_achieved_tflops=flops / (ms * 1e-3) / 1e12,
+                flops_frac=flops / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+                candidate_pairs_per_s=cand / (ms * 1e-3))
+
+
 def bench_ensemble(n_scenarios, steps, loop_steps, stream, tune=True, order_every=10):
     """Batched ensemble of uneven rooms (fp32, fp64 positions): autotuned
     batched steps replayed from a HIP graph versus a per-scenario loop of
     plain GPU steps (one Stepper per scenario, same cell size and
-    precision)."""
+    precision), also replayed from a HIP graph."""
     import torch
     from crowddynamics_amd import examples as E
     from crowddynamics_amd.autotune import neutral_cell_sizes, max_body_radius, tune_ensemble
     from crowddynamics_amd.ensemble import Ensemble
-    from crowddynamics_amd.stepper import Stepper
+    from crowddynamics_amd.stepper import Stepper, capture_many
     log('ensemble: %d scenarios' % n_scenarios)
     scns = E.ensemble(n_scenarios)
     n = sum(s.n for s in scns)
@@ -173,64 +252,91 @@ def bench_ensemble(n_scenarios, steps, loop_steps, stream, tune=True, order_ever
                        lanes=tuned['lanes'], order_every=tuned['order_every'])
         ens.step(10)
         G = 50
-        graph = ens.capture(G)
         reps = max(1, steps // G)
-        stream.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            graph.launch()
-        stream.synchronize()
-        el = time.perf_counter() - t0
+        arr0 = np.concatenate([s.agents for s in scns])
+        restore = _snapshot(ens, arr0)
+        tally0 = _ensemble_tally(ens, scns, arr0, tuned['cell_size'])
+        graph = ens.capture(G)
+        restore()
+        kms, el = _window(ens, graph, G, reps, stream, restore)
         ens.check()
-        kms = ens.timed(20)
+        tally1 = _ensemble_tally(ens, scns, arr0, tuned['cell_size'])
         del graph
         batched = n * reps * G / el
         del ens
-        log('ensemble batched: %.3e agent-steps/s (%.3f ms/step)' % (batched, 1e3 * el / (reps * G)))
+        log('ensemble batched: %.3e agent-steps/s (%.3f ms/step), kernels %s' % (
+            batched, 1e3 * el / (reps * G), {k: round(v, 4) for k, v in kms.items()}))
         steppers = [Stepper(s.agents.copy(), segments=s.segments, cell_size=tuned['cell_size'],
                             dt_min=s.dt_min, dt_max=s.dt_max, seed=s.seed, precision='f32',
                             bounds=s.bounds) for s in scns]
         for st in steppers:
             st.step(2)
+        GL = 5
+        loop_graph = capture_many(steppers, GL)
+        loop_graph.launch()
         stream.synchronize()
+        loop_reps = max(1, loop_steps // GL)
         t0 = time.perf_counter()
-        for _ in range(loop_steps):
-            for st in steppers:
-                st.step(1)
+        for _ in range(loop_reps):
+            loop_graph.launch()
         stream.synchronize()
         el_loop = time.perf_counter() - t0
-        loop = n * loop_steps / el_loop
+        loop = n * loop_reps * GL / el_loop
         for st in steppers:
             st.check()
-        del steppers
-    log('ensemble per-scenario loop: %.3e agent-steps/s' % loop)
+        del loop_graph, steppers
+    log('ensemble per-scenario loop (graph): %.3e agent-steps/s' % loop)
+    flops = 0.5 * (tally0['flops'] + tally1['flops'])
+    cand = 0.5 * (tally0['candidate_pairs'] + tally1['candidate_pairs'])
+    roof = _force_roofline(n, kms['forces'], forces_bytes(n, 4), flops, cand)
     return dict(scenarios=n_scenarios, agents=n, precision='f32 (f64 positions and dt)',
                 steps=reps * G, value=batched, unit='agent-steps/s',
                 ms_per_step=1e3 * el / (reps * G),
-                kernel_ms={k: kms[i] / 20 for k, i in KERNELS.items()},
+                kernel_ms=kms,
+                kernel_ms_note='HIP events around each kernel over the same %d steps '
+                               '(eager), from the same state as the graph replay' % (reps * G),
+                roofline_forces=roof,
+                pair_tally=dict(first_step=tally0, last_step=tally1),
                 autotuned=dict(cell_size=tuned['cell_size'],
                                tile=tuned['tile'], lanes=tuned['lanes'],
                                order_every=tuned['order_every'],
                                candidates=[dict(ms=t[0], cell_size=t[1], kernel=t[2])
                                            for t in tuned['table']]),
-                per_scenario_loop=dict(value=loop, steps=loop_steps,
-                                       ms_per_step=1e3 * el_loop / loop_steps),
+                per_scenario_loop=dict(value=loop, steps=loop_reps * GL,
+                                       ms_per_step=1e3 * el_loop / (loop_reps * GL),
+                                       note='one Stepper per scenario, one graph of %d rounds '
+                                            'of a step of every scenario' % GL),
                 speedup_vs_loop=batched / loop)
 
 
+def _ensemble_tally(ens, scns, arr0, cell):
+    """Pair tally of the batch's current state, summed over the scenarios."""
+    arr = ens.download(arr0.copy())
+    tot = {}
+    for k, s in enumerate(scns):
+        t = pair_tally(arr[ens.offsets[k]:ens.offsets[k + 1]], cell, s.segments)
+        for key, v in t.items():
+            tot[key] = tot.get(key, 0) + v
+    return tot
+
+
 def bench_corridor(steps, stream):
-    """50k ThreeCircle agents, 300x150 m corridor with 200 obstacles, fp64."""
+    """50k ThreeCircle agents, 300x150 m corridor with 200 obstacles, fp64,
+    target orientation from the velocity."""
     import torch
+    from crowddynamics_amd import _lib
     from crowddynamics_amd import examples as E
-    from crowddynamics_amd.stepper import Stepper
+    from crowddynamics_amd.stepper import ALL_NODES, Stepper
     log('corridor: 50k three-circle agents')
     scn = E.corridor()
     G = 50
+    flags = ALL_NODES | (_lib.CD_USE_ORIENT_VELOCITY if scn.extra.get('orient') == 'velocity'
+                         else 0)
 
     def stepper(lanes):
         return Stepper(scn.agents.copy(), segments=scn.segments, cell_size=scn.cell_size,
                        dt_min=scn.dt_min, dt_max=scn.dt_max, seed=scn.seed, precision='f64',
-                       bounds=scn.bounds, lanes=lanes)
+                       bounds=scn.bounds, lanes=lanes, flags=flags)
 
     with torch.cuda.stream(stream):
         # lanes per agent: 2 or 4, by one graph of G steps each (same results)
@@ -250,22 +356,149 @@ def bench_corridor(steps, stream):
         log('corridor: ms per step by lanes %s' % lane_ms)
         st = stepper(lanes)
         st.step(10)
-        graph = st.capture(G)
         reps = max(1, steps // G)
-        stream.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            graph.launch()
-        stream.synchronize()
-        el = time.perf_counter() - t0
+        restore = _snapshot(st, scn.agents)
+        tally0 = pair_tally_three_circle(st.download(scn.agents.copy()), scn.cell_size,
+                                         scn.segments)
+        graph = st.capture(G)
+        restore()
+        kms, el = _window(st, graph, G, reps, stream, restore)
         st.check()
-        kms = st.timed(20)
+        tally1 = pair_tally_three_circle(st.download(scn.agents.copy()), scn.cell_size,
+                                         scn.segments)
         del graph
     v = scn.n * reps * G / el
-    log('corridor: %.3e agent-steps/s' % v)
+    log('corridor: %.3e agent-steps/s, kernels %s' % (v, {k: round(x, 4) for k, x in kms.items()}))
+    flops = 0.5 * (tally0['flops'] + tally1['flops'])
+    cand = 0.5 * (tally0['candidate_pairs'] + tally1['candidate_pairs'])
+    # three-circle force kernel bytes per agent: the circular model's plus the
+    # shoulders' snapshot read as a neighbour (2 x 16 + offset 16), r_t, r_s,
+    # r_ts, inertia, orientation, angular velocity, torque, previous torque,
+    # tau_rot, target orientation and target angular velocity read (11 x 8),
+    # orientation, angular velocity, torques and both shoulders written
+    # (4 x 8 + 2 x 16)
+    fb = forces_bytes(scn.n, 8) + scn.n * (48 + 88 + 64)
+    roof = _force_roofline(scn.n, kms['forces'], fb, flops, cand)
     return dict(agents=scn.n, steps=reps * G, value=v, unit='agent-steps/s',
                 ms_per_step=1e3 * el / (reps * G), precision='f64', lanes=lanes,
-                lane_ms=lane_ms, kernel_ms={k: kms[i] / 20 for k, i in KERNELS.items()})
+                orientation='angle(velocity)' if flags & _lib.CD_USE_ORIENT_VELOCITY
+                else 'angle(target_direction)',
+                lane_ms=lane_ms, kernel_ms=kms,
+                kernel_ms_note='HIP events around each kernel over the same %d steps '
+                               '(eager), from the same state as the graph replay' % (reps * G),
+                roofline_forces=roof, pair_tally=dict(first_step=tally0, last_step=tally1))
+
+
+def _adjoint_window(arr, scn, steps, checkpoint, lanes, seed=0):
+    """A rollout of `steps` steps of the differentiable workload, its target
+    trajectory (tau_0 and kappa x 1.05, desired speeds x (1 + 0.05 N(0, 1)))
+    and the parameter constructor."""
+    import torch
+    from crowddynamics_amd.adjoint import Rollout, rollout_positions
+    dev, f64 = 'cuda', torch.float64
+    n = len(arr)
+
+    def params(grad=False):
+        ps = [torch.tensor(3.0), torch.tensor(1.2e5), torch.tensor(4e4), torch.tensor(500.0),
+              torch.as_tensor(np.array(arr['target_velocity'], dtype=np.float64))]
+        return [p.to(device=dev, dtype=f64).requires_grad_(grad) for p in ps]
+
+    R = Rollout(arr, steps=steps, dt=scn.dt_max, cell_size=scn.cell_size,
+                checkpoint=checkpoint, bounds=scn.bounds, lanes=lanes)
+    rng = np.random.default_rng(seed)
+    q = params()
+    q[0] = q[0] * 1.05
+    q[2] = q[2] * 1.05
+    q[4] = q[4] * torch.as_tensor(1.0 + 0.05 * rng.standard_normal(n), device=dev)
+    with torch.no_grad():
+        Xhat = rollout_positions(R, *q)
+    return R, Xhat, params
+
+
+def _loss(X, Xhat):
+    return ((X[1:] - Xhat[1:]) ** 2).sum(-1).mean()
+
+
+def _gradient(R, Xhat, params):
+    """Loss and the adjoint gradient (tau_0, mu, kappa, damping as global
+    scalars; the per-agent desired speeds)."""
+    from crowddynamics_amd.adjoint import rollout_positions
+    p = params(True)
+    L = _loss(rollout_positions(R, *p), Xhat)
+    L.backward()
+    return float(L.detach()), [t.grad.detach() for t in p]
+
+
+def _gradient_summary(g):
+    import torch
+    return dict(tau_0=float(g[0]), mu=float(g[1]), kappa=float(g[2]), damping=float(g[3]),
+                v0_l2_norm=float(torch.linalg.vector_norm(g[4])),
+                v0_max_abs=float(g[4].abs().max()))
+
+
+def adjoint_checks(arr, scn, lanes, windows=(10, 25, 50, 100), hs=(1e-4, 1e-5, 1e-6, 1e-7, 1e-8),
+                   checkpoint=25):
+    """On the differentiable workload, for rollouts of each length in
+    `windows`: the dot-product test of the adjoint against the forward-mode
+    tangent (both exact derivatives of the same rollout: they agree to
+    rounding whatever the gradient's size), central differences of the loss
+    in tau_0 at the steps `hs`, the smallest agent-agent gap over the window
+    (negative: contacts), and the adjoint gradient."""
+    import torch
+    from crowddynamics_amd.adjoint import rollout_positions
+    out = []
+    for W in windows:
+        R, Xhat, params = _adjoint_window(arr, scn, W, min(checkpoint, W), lanes)
+        L, g = _gradient(R, Xhat, params)
+        # dot-product test along a random tangent of all parameters and of
+        # the initial position and velocity
+        n = len(arr)
+        R.set_parameters(*params())
+        X = R.forward().clone().requires_grad_(True)
+        _loss(X, Xhat).backward()
+        gX = X.grad.detach()
+        gp, lx0, lv0 = R.backward(gX)
+        rng = np.random.default_rng(W)
+        dp = torch.as_tensor(rng.standard_normal((n, 5)) *
+                             (1e-3 * np.array([3.0, 1.2e5, 4e4, 500.0, 1.15])), device='cuda')
+        dx0 = torch.as_tensor(rng.standard_normal((n, 2)) * 1e-3, device='cuda')
+        dv0 = torch.as_tensor(rng.standard_normal((n, 2)) * 1e-3, device='cuda')
+        dX = R.tangent(dp, dx0, dv0)
+        terms = gX * dX
+        lhs = float(terms.sum())
+        rhs = float((gp * dp).sum() + (lx0 * dx0).sum() + (lv0 * dv0).sum())
+        dot_rel = abs(lhs - rhs) / max(abs(lhs), float(terms.abs().sum()), 1e-300)
+        # central differences in tau_0
+        fd = {}
+        for h in hs:
+            vals = []
+            for sgn in (1.0, -1.0):
+                p = params()
+                p[0] = p[0] + sgn * h
+                with torch.no_grad():
+                    vals.append(float(_loss(rollout_positions(R, *p), Xhat)))
+            fd['%g' % h] = (vals[0] - vals[1]) / (2 * h)
+        with torch.no_grad():
+            Xc = rollout_positions(R, *params())
+            rr = torch.as_tensor(np.array(arr['radius']), device='cuda')
+            gap = float('inf')
+            for k in range(0, W + 1, max(1, W // 10)):
+                d = torch.cdist(Xc[k], Xc[k]) - (rr[:, None] + rr[None, :])
+                d.fill_diagonal_(float('inf'))
+                gap = min(gap, float(d.min()))
+        adj = float(g[0])
+        floor = 100 * np.finfo(np.float64).eps * abs(L) / 1e-6
+        agree = abs(fd['1e-06'] - adj) <= 1e-4 * abs(adj) + floor
+        rec = dict(steps=W, loss=L, dot_product=dict(tangent=lhs, adjoint=rhs,
+                                                      rel_diff=dot_rel),
+                   tau_0_adjoint=adj, tau_0_central_fd=fd, fd_agrees_at_h_1e_6=bool(agree),
+                   min_gap_m=gap, gradient=_gradient_summary(g))
+        log('adjoint window %d steps: dot-product rel diff %.2e, dL/dtau_0 adjoint %.6e, '
+            'fd(1e-6) %.6e (%s), min gap %.3f m' % (W, dot_rel, adj, fd['1e-06'],
+                                                  'agrees' if agree else 'differs', gap))
+        out.append(rec)
+        del R
+    return out
 
 
 def bench_adjoint(n, steps, checkpoint):
@@ -273,77 +506,77 @@ def bench_adjoint(n, steps, checkpoint):
     mean squared distance to a target trajectory made by the same method
     with tau_0 and kappa scaled by 1.05 and desired speeds by 1 + 0.05 N(0,1);
     gradients for tau_0, mu, kappa, damping and the n desired speeds from
-    the adjoint with a checkpoint every `checkpoint` steps."""
+    the adjoint with a checkpoint every `checkpoint` steps.  Reports the
+    timing of the whole `steps`-step gradient, the dot-product and
+    finite-difference checks on windows of 10 to 100 steps, and the gradient
+    of the longest window whose central difference (h = 1e-6) agrees."""
     import torch
     from crowddynamics_amd import examples as E
-    from crowddynamics_amd.adjoint import Rollout, rollout_positions
+    from crowddynamics_amd.adjoint import rollout_positions
     log('adjoint: n=%d, %d steps, checkpoint %d' % (n, steps, checkpoint))
     scn = E.rollout(n=n, steps=steps)
     arr = scn.agents
-    dev = 'cuda'
-    f64 = torch.float64
-
-    def params(grad=False):
-        ps = [torch.tensor(3.0), torch.tensor(1.2e5), torch.tensor(4e4), torch.tensor(500.0),
-              torch.as_tensor(np.array(arr['target_velocity'], dtype=np.float64))]
-        return [p.to(device=dev, dtype=f64).requires_grad_(grad) for p in ps]
-
-    def rollout(k, lanes):
-        return Rollout(arr, steps=k, dt=scn.dt_max, cell_size=scn.cell_size,
-                       checkpoint=checkpoint, bounds=scn.bounds, lanes=lanes)
 
     # lanes per agent: timed on a short rollout (forward + backward)
     lane_ms = {}
     for lanes in (4, 8, 16, 32, 64):
-        Rt = rollout(2 * checkpoint, lanes)
-        p = params(True)
+        Rt, Xt, pt = _adjoint_window(arr, scn, 2 * checkpoint, checkpoint, lanes)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        (rollout_positions(Rt, *p) ** 2).sum().backward()
+        _gradient(Rt, Xt, pt)
         torch.cuda.synchronize()
         lane_ms[lanes] = 1e3 * (time.perf_counter() - t0) / (2 * checkpoint)
         del Rt
     lanes = min(lane_ms, key=lane_ms.get)
     log('adjoint lanes per agent %d (%s ms/step)' % (lanes, lane_ms))
-    R = rollout(steps, lanes)
-
-    rng = np.random.default_rng(0)
-    q = params()
-    q[0] = q[0] * 1.05
-    q[2] = q[2] * 1.05
-    q[4] = q[4] * torch.as_tensor(1.0 + 0.05 * rng.standard_normal(n), device=dev)
-    with torch.no_grad():
-        Xhat = rollout_positions(R, *q)
+    R, Xhat, params = _adjoint_window(arr, scn, steps, checkpoint, lanes)
     # one untimed pass of the whole gradient (torch's loss and reduction
     # kernels load on first use), then the timed one
-    p = params(True)
-    ((rollout_positions(R, *p)[1:] - Xhat[1:]) ** 2).sum(-1).mean().backward()
+    _gradient(R, Xhat, params)
     p = params(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     X = rollout_positions(R, *p)
-    L = ((X[1:] - Xhat[1:]) ** 2).sum(-1).mean()
+    L = _loss(X, Xhat)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     L.backward()
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     fwd, bwd = t1 - t0, t2 - t1
-    n_par = 4 + n
+    # one central difference (two forward rollouts of the same length), timed
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    for sgn in (1.0, -1.0):
+        q = params()
+        q[0] = q[0] + sgn * 1e-6
+        with torch.no_grad():
+            float(_loss(rollout_positions(R, *q), Xhat))
+    torch.cuda.synchronize()
+    fd_one = time.perf_counter() - t3
+    del R
     log('adjoint: forward %.3f ms/step, backward %.3f ms/step' % (1e3 * fwd / steps,
                                                                    1e3 * bwd / steps))
+    checks = adjoint_checks(arr, scn, lanes)
+    good = [c for c in checks if c['fd_agrees_at_h_1e_6']]
+    best = max(good, key=lambda c: c['steps']) if good else None
     return dict(agents=n, steps=steps, checkpoint=checkpoint, dt=scn.dt_max, lanes=lanes,
-                loss=float(L.detach()), forward_ms_per_step=1e3 * fwd / steps,
+                lane_ms=lane_ms, loss=float(L.detach()),
+                forward_ms_per_step=1e3 * fwd / steps,
                 backward_ms_per_step=1e3 * bwd / steps,
                 adjoint_step_ms=1e3 * (fwd + bwd) / steps,
                 gradient_s=fwd + bwd,
-                grad_norms=dict(tau_0=float(p[0].grad), mu=float(p[1].grad),
-                                kappa=float(p[2].grad), damping=float(p[3].grad),
-                                v0_l2=float(torch.linalg.vector_norm(p[4].grad))),
-                fd_equivalent=dict(
-                    note='central differences need 2 forward rollouts per parameter',
-                    parameters=n_par, seconds=2 * n_par * fwd,
-                    speedup_of_adjoint=2 * n_par * fwd / (fwd + bwd)))
+                one_central_difference_s=fd_one,
+                gradient_full_window=dict(
+                    steps=steps, values=_gradient_summary([t.grad for t in p]),
+                    note='not confirmed by finite differences at this horizon: the '
+                         'time-to-collision force grows without bound near grazing '
+                         'pairs, so the loss is not smooth at h = 1e-6 over long windows'),
+                checks=checks,
+                fd_confirmed_window=(dict(steps=best['steps'], gradient=best['gradient'],
+                                          loss=best['loss'],
+                                          tau_0_central_fd=best['tau_0_central_fd']['1e-06'])
+                                     if best else None))
 
 
 def main():
@@ -461,11 +694,13 @@ def main():
     rb = 8 if main_p == 'f64' else 4
     per_kernel = {}
     tname = 'double' if main_p == 'f64' else 'float'
-    knames = {'step_begin': 'k_step_begin<%s, 0>' % tname,
+    knames = {'step_begin': 'k_step_begin<%s, 0, true>' % tname,
               'forces': 'k_forces_direct<%s, 0, true>' % tname}
-    # grid threads of each launch (the traffic summary is keyed by them)
-    f_threads = min(-(-scn.n // 256), 512) * 256  # blocks_f of one scenario
-    grid = {'step_begin': f_threads, 'forces': -(-scn.n // (256 // kern[2])) * 256}
+    # grid threads of each launch (the traffic summary is keyed by them): the
+    # first kernel of one scenario with fluctuation or navigation runs
+    # 4 x blocks_f blocks of 128 threads (binning and own-terms waves)
+    blocks_f = min(-(-scn.n // 256), 512)
+    grid = {'step_begin': 4 * blocks_f * 128, 'forces': -(-scn.n // (256 // kern[2])) * 256}
     sb = step_begin_bytes(scn.n, cells, rb)
     per_kernel['step_begin'] = dict(bound='hbm', unit='GB/s', ms=km['step_begin'], bytes=sb,
                                     achieved=sb / (km['step_begin'] * 1e-3) / 1e9,

@@ -34,6 +34,7 @@ CD_USE_ADJUST = 4
 CD_USE_AGENT_AGENT = 8
 CD_USE_OBSTACLE = 16
 CD_USE_RESET = 32
+CD_USE_ORIENT_VELOCITY = 64
 
 _vp = C.c_void_p
 
@@ -74,6 +75,10 @@ class CDAdjoint(C.Structure):
                 ('lam_vt', _vp), ('grad', _vp)]
 
 
+class CDTangent(C.Structure):
+    _fields_ = [('dx', _vp), ('dv', _vp), ('dg', _vp), ('df', _vp), ('dp', _vp)]
+
+
 class CDState(C.Structure):
     _fields_ = [('dt', C.c_double), ('time_tot', C.c_double),
                 ('origin', C.c_double * 2), ('shape', C.c_int64 * 2),
@@ -92,7 +97,9 @@ class CDStepParams(C.Structure):
                 ('flags', C.c_int32), ('nav_mode', C.c_int32), ('seed', C.c_uint64),
                 ('segments', _vp), ('n_segments', C.c_int64), ('nav_maps', _vp),
                 ('nav_targets', C.c_int64), ('nav_rows', C.c_int64), ('nav_cols', C.c_int64),
-                ('nav_minx', C.c_double), ('nav_miny', C.c_double), ('nav_step', C.c_double)]
+                ('nav_minx', C.c_double), ('nav_miny', C.c_double), ('nav_step', C.c_double),
+                ('wall_cells', _vp), ('wall_nx', C.c_int64), ('wall_ny', C.c_int64),
+                ('wall_minx', C.c_double), ('wall_miny', C.c_double), ('wall_cell', C.c_double)]
 
 
 #: every symbol include/cdamd.h declares, with (restype, argtypes)
@@ -116,6 +123,8 @@ SIGNATURES = {
     'cd_force_fluctuation': (C.c_int, [_P(CDAgents), C.c_uint64, C.c_uint64, _vp]),
     'cd_velocity_verlet_integrator': (C.c_int, [_P(CDAgents), C.c_double, C.c_double, _W, _vp]),
     'cd_reset': (C.c_int, [_P(CDAgents), _vp]),
+    'cd_wall_grid_build': (C.c_int64, [_vp, C.c_int64, C.c_double, C.c_double, _vp,
+                                       C.c_int64, _P(CDStepParams)]),
     'cd_step_prepare': (C.c_int, [_P(CDAgents), _P(CDStepParams), _W, C.c_uint64, _vp]),
     'cd_step': (C.c_int, [_P(CDAgents), _P(CDStepParams), _W, _vp]),
     'cd_step_tape': (C.c_int, [_P(CDAgents), _P(CDStepParams), _W, _vp, _vp, _vp]),
@@ -125,6 +134,7 @@ SIGNATURES = {
     'cd_batch_read_states': (C.c_int, [_W, _P(CDState), C.c_int32, _vp]),
     'cd_adjoint_step': (C.c_int, [_P(CDAgents), _P(CDStepParams), _vp, _P(CDAdjoint), _W,
                                   _vp]),
+    'cd_tangent_step': (C.c_int, [_P(CDAgents), _P(CDStepParams), _P(CDTangent), _W, _vp]),
     'cd_rollout_record': (C.c_int, [_P(CDAgents), _P(CDStepParams), _W, C.c_int64, C.c_int64,
                                     _vp, _vp, _vp, _vp, _vp]),
     'cd_adjoint_segment': (C.c_int, [_P(CDAgents), _P(CDStepParams), _W, C.c_int64, C.c_int64,
@@ -137,6 +147,7 @@ SIGNATURES = {
     'cd_graph_end': (C.c_int, [_vp, _P(_vp)]),
     'cd_graph_launch': (C.c_int, [_vp, _vp]),
     'cd_graph_destroy': (C.c_int, [_vp]),
+    'cd_hypot': (C.c_int, [_vp, _vp, C.c_int64, _vp]),
     'cd_philox4x32': (C.c_int, [_vp, _vp, _vp]),
     'cd_version': (C.c_char_p, []),
 }

@@ -22,7 +22,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .device import stream_handle
+from .device import Workspace, stream_handle
 from .exceptions import InvalidValue
 from .stepper import Stepper
 
@@ -126,6 +126,50 @@ class Rollout:
                 C.byref(self.adj_struct), s_h), 'adjoint_segment')
         self.st.check()
         return self.grad.clone(), self.adj['lam_x'].clone(), self.adj['lam_v'].clone()
+
+    def tangent(self, dp, dx0=None, dv0=None):
+        """Forward-mode derivative of the trajectory: dX [steps + 1, n, 2],
+        the derivative of the positions along the parameter tangent dp
+        ([n, 5]: tau_0, mu, kappa, damping, target_velocity of each agent)
+        and the initial tangents dx0, dv0 ([n, 2], default zero) of
+        position and velocity.  Runs the rollout from the initial records,
+        one cd_tangent_step (from state k, binned in a workspace of its
+        own) before each fused step k -> k + 1."""
+        dev = self.agents.device
+        f64 = torch.float64
+        n = self.n
+        T = {k: torch.zeros((n, 2), dtype=f64, device=dev) for k in ('dx', 'dv', 'dg', 'df')}
+        if dx0 is not None:
+            T['dx'].copy_(torch.as_tensor(dx0, dtype=f64, device=dev))
+        if dv0 is not None:
+            T['dv'].copy_(torch.as_tensor(dv0, dtype=f64, device=dev))
+        dp = torch.as_tensor(dp, dtype=f64, device=dev).contiguous()
+        if dp.shape != (n, 5):
+            raise InvalidValue('dp must be [n, 5]')
+        tan = _lib.CDTangent()
+        for k, t in T.items():
+            setattr(tan, k, C.c_void_p(t.data_ptr()))
+        tan.dp = C.c_void_p(dp.data_ptr())
+        if getattr(self, 'tan_ws', None) is None:
+            w = self.st.ws
+            self.tan_ws = Workspace(n, w.cell_capacity, 0, cell_slots=w.cell_slots,
+                                    device=dev, lanes=w.desc.agent_lanes)
+        dX = torch.empty((self.steps + 1, n, 2), dtype=f64, device=dev)
+        dX[0] = T['dx']
+        self._restore_initial()
+        self.st.prepare(0)
+        L = _lib.lib()
+        s_h = stream_handle()
+        par = C.byref(self.st.params)
+        for k in range(self.steps):
+            _lib.check(L.cd_tangent_step(self.agents.ref(), par, C.byref(tan),
+                                         self.tan_ws.ref(), s_h), 'tangent_step')
+            _lib.check(L.cd_step(self.agents.ref(), par, self.st.ws.ref(), s_h), 'step')
+            dX[k + 1] = T['dx']
+        self.st.check()
+        if self.tan_ws.state().status & ~_lib.CD_STATUS_SLOTS_FULL:
+            raise InvalidValue('tangent binning: status %d' % self.tan_ws.state().status)
+        return dX
 
     def adjoint_step(self, grad_x=None):
         """One cd_adjoint_step from the agents' current position and

@@ -94,15 +94,14 @@ __device__ __forceinline__ Owner<double> owner_at(const Agents<double> &A, int j
   return o;
 }
 
-// Pair adjoints of agent i with part q (of L equal parts) of the candidates
-// of its 3 x 3 stencil, taken as one sequence (rows dx = -1, 0, +1, each
-// row's cells dy = -1, 0, +1, each cell's slots in agent order).
-__device__ __forceinline__ void stencil_vjp(const Agents<double> &A, const Work &W,
-                                            const AdjView &J, int c0, int nx, int ny, int lx,
-                                            int ly, int q, int L, int a0, int a1, int i,
-                                            double2 xi, double2 vi, double ri,
-                                            const Owner<double> &oi, double2 lfi,
-                                            double acc[8]) {
+// Calls fn(j, x_j, v_j, r_j) for part q (of L equal parts) of the
+// candidates of agent i's 3 x 3 stencil, taken as one sequence (rows
+// dx = -1, 0, +1, each row's cells dy = -1, 0, +1, each cell's slots in
+// agent order), skipping i itself.
+template <class Fn>
+__device__ __forceinline__ void stencil_each(const Agents<double> &A, const Work &W, int c0,
+                                             int nx, int ny, int lx, int ly, int q, int L,
+                                             int a0, int a1, int i, Fn &&fn) {
   int gr[3], n[9];
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
@@ -140,9 +139,7 @@ __device__ __forceinline__ void stencil_vjp(const Agents<double> &A, const Work 
       const size_t e = (size_t)c * W.K + (idx - base);
       const int j = W.ell_ids[e];
       if (j == i) continue;
-      const double2 xj = W.e_pos[e], vj = e_vel[e];
-      pair_vjp(xi.x - xj.x, xi.y - xj.y, vi.x - vj.x, vi.y - vj.y, ri + W.e_r[e], oi,
-               owner_at(A, j), lfi, J.lam_f[j], acc);
+      fn(j, W.e_pos[e], e_vel[e], W.e_r[e]);
     }
     return;
   }
@@ -159,9 +156,7 @@ __device__ __forceinline__ void stencil_vjp(const Agents<double> &A, const Work 
         const size_t e = (size_t)c * W.K + s2;
         const int j = W.ell_ids[e];
         if (j == i) continue;
-        const double2 xj = W.e_pos[e], vj = e_vel[e];
-        pair_vjp(xi.x - xj.x, xi.y - xj.y, vi.x - vj.x, vi.y - vj.y, ri + W.e_r[e], oi,
-                 owner_at(A, j), lfi, J.lam_f[j], acc);
+        fn(j, W.e_pos[e], e_vel[e], W.e_r[e]);
       }
       continue;
     }
@@ -169,14 +164,27 @@ __device__ __forceinline__ void stencil_vjp(const Agents<double> &A, const Work 
     for (int j = a0; j < a1 && r < s1; ++j) {
       if (W.cell_of[j] != c) continue;
       if (r >= s0 && j != i) {
-        const double2 xj = A.pos[j];
         const V2<double> vj = A.vel[j];
-        pair_vjp(xi.x - xj.x, xi.y - xj.y, vi.x - vj.x, vi.y - vj.y, ri + A.radius[j], oi,
-                 owner_at(A, j), lfi, J.lam_f[j], acc);
+        fn(j, A.pos[j], make_double2(vj.x, vj.y), A.radius[j]);
       }
       ++r;
     }
   }
+}
+
+// Pair adjoints of agent i with part q (of L equal parts) of its stencil's
+// candidates.
+__device__ __forceinline__ void stencil_vjp(const Agents<double> &A, const Work &W,
+                                            const AdjView &J, int c0, int nx, int ny, int lx,
+                                            int ly, int q, int L, int a0, int a1, int i,
+                                            double2 xi, double2 vi, double ri,
+                                            const Owner<double> &oi, double2 lfi,
+                                            double acc[8]) {
+  stencil_each(A, W, c0, nx, ny, lx, ly, q, L, a0, a1, i,
+               [&](int j, double2 xj, double2 vj, double rj) {
+                 pair_vjp(xi.x - xj.x, xi.y - xj.y, vi.x - vj.x, vi.y - vj.y, ri + rj, oi,
+                          owner_at(A, j), lfi, J.lam_f[j], acc);
+               });
 }
 
 // L lanes per agent (2 to 64) as in the forward force kernel: lane q
@@ -250,6 +258,103 @@ __global__ void __launch_bounds__(256) k_adjoint(Agents<double> A, Work W, Batch
   g[4] += g_v0;
 }
 
+// ---------------------------------------------------------------------------
+// Tangent (forward-mode) step: the directional derivative of the same step
+// along a tangent of the state (dx, dv, dg = d force_prev) and of the
+// parameters (dp).  With the adjoint it forms the dot-product test
+// <J t, u> = <t, J^T u> of the whole rollout map, which holds at any
+// horizon for exact derivatives of the same floating-point program.
+//   dF_k     = (dF/dx) dx_k + (dF/dv) dv_k + (dF/dp) dp
+//   dv_{k+1} = dv_k + (dG_k + dF_k) dt / (2 m)
+//   dx_{k+1} = dx_k + dv_{k+1} dt + dF_k dt^2 / (2 m)
+//   dG_{k+1} = dF_k
+// Agent i's pair terms are f(x_i - x_j, v_i - v_j; p_i), evaluated on
+// one-direction dual numbers seeded with (dx_i - dx_j, dv_i - dv_j) and
+// i's parameter tangents, so every branch is the forward step's.
+// ---------------------------------------------------------------------------
+
+struct TanView {
+  double2 *dx, *dv, *dg;  // tangent state, [n]
+  double2 *df;            // dF_k of the current step, [n]
+  const double *dp;       // [n][5]: tau_0, mu, kappa, damping, target_velocity
+};
+
+using D1 = Dual<1>;
+
+__device__ __forceinline__ D1 d1(double v, double t) {
+  D1 r(v);
+  r.d[0] = t;
+  return r;
+}
+
+__global__ void __launch_bounds__(256) k_tangent(Agents<double> A, Work W, Batch B, TanView T,
+                                                 int flags, int L) {
+  const int2 sk = block_e(B);
+  const ScnDesc d = desc_of(B, sk.x);
+  const ScnState *ss = &B.sst[sk.x];
+  const int lg = __builtin_ctz(L);
+  int i = d.a0 + sk.y * ((int)blockDim.x >> lg) + ((int)threadIdx.x >> lg);
+  if (W.order_every > 0 && i < d.a1) i = W.order[i];
+  const int q = threadIdx.x & (L - 1);
+  const bool have = i >= d.a0 && i < d.a1;
+  const bool binned = (flags & CD_USE_AGENT_AGENT) && ss->valid && !ss->oob[0];
+  double acc[2] = {0.0, 0.0};
+  if (have && binned) {
+    const double2 xi = A.pos[i];
+    const V2<double> v = A.vel[i];
+    const double2 vi = make_double2(v.x, v.y);
+    const double2 dxi = T.dx[i], dvi = T.dv[i];
+    const double ri = A.radius[i];
+    const double *dpi = T.dp + (size_t)i * 5;
+    const Owner<D1> oi{D1((-A.mass[i]) * A.k_soc[i]), d1(A.tau_0[i], dpi[0]),
+                       d1(A.mu[i], dpi[1]), d1(A.kappa[i], dpi[2]), d1(A.damping[i], dpi[3])};
+    const int c = W.cell_of[i] - d.c0;
+    const int ny = ss->ny, nx = ss->nx;
+    const int lx = c / ny, ly = c - lx * ny;
+    stencil_each(A, W, d.c0, nx, ny, lx, ly, q, L, d.a0, d.a1, i,
+                 [&](int j, double2 xj, double2 vj, double rj) {
+                   const double2 dxj = T.dx[j], dvj = T.dv[j];
+                   D1 fx, fy;
+                   if (pair_circular<D1>(d1(xi.x - xj.x, dxi.x - dxj.x),
+                                         d1(xi.y - xj.y, dxi.y - dxj.y),
+                                         d1(vi.x - vj.x, dvi.x - dvj.x),
+                                         d1(vi.y - vj.y, dvi.y - dvj.y), D1(ri + rj), oi, fx,
+                                         fy)) {
+                     acc[0] += fx.d[0];
+                     acc[1] += fy.d[0];
+                   }
+                 });
+  }
+  for (int mask = 1; mask < L; mask <<= 1) {
+    acc[0] += __shfl_xor(acc[0], mask, 64);
+    acc[1] += __shfl_xor(acc[1], mask, 64);
+  }
+  if (!have || q != 0) return;
+  double2 df = make_double2(acc[0], acc[1]);
+  if (flags & CD_USE_ADJUST) {
+    // force_adjust (core/motion/adjusting.py:16-51): F = (m / tau_adj) (v0 e0 - v)
+    const double s = A.mass[i] / A.tau_adj[i];
+    const V2<double> e = A.e0[i];
+    const double2 dv = T.dv[i];
+    const double dv0 = T.dp[(size_t)i * 5 + 4];
+    df.x += s * (dv0 * e.x - dv.x);
+    df.y += s * (dv0 * e.y - dv.y);
+  }
+  T.df[i] = df;
+}
+
+__global__ void __launch_bounds__(256) k_tan_post(int n, const double *mass, double dt,
+                                                  TanView T) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double c = dt / (2.0 * mass[i]);
+  const double2 df = T.df[i], dg = T.dg[i], dx = T.dx[i], dv = T.dv[i];
+  const double2 v = make_double2(dv.x + c * (dg.x + df.x), dv.y + c * (dg.y + df.y));
+  T.dv[i] = v;
+  T.dx[i] = make_double2(dx.x + dt * v.x + (dt * c) * df.x, dx.y + dt * v.y + (dt * c) * df.y);
+  T.dg[i] = df;
+}
+
 }  // namespace
 
 extern "C" {
@@ -293,6 +398,44 @@ int cd_adjoint_step(const cd_agents_t *a, const cd_step_params_t *p, const doubl
   }
   hipLaunchKernelGGL(k_adjoint, dim3(L.blocks_e), dim3(L.threads_e), 0, s, A, W, B, J,
                      (int)p->flags, L.lanes);
+  return check_launch();
+}
+
+int cd_tangent_step(const cd_agents_t *a, const cd_step_params_t *p, const cd_tangent_t *tan,
+                    const cd_workspace_t *w, cd_stream_t stream) {
+  int rc = check_step(a, p, w);
+  if (rc) return rc;
+  if (!tan || (a->n > 0 && (!tan->dx || !tan->dv || !tan->dg || !tan->df || !tan->dp)))
+    return CD_ERR_ARG;
+  if (a->model != CD_CIRCULAR || a->real != CD_F64) return CD_ERR_UNSUPPORTED;
+  const int allowed = CD_USE_FLUCT | CD_USE_ADJUST | CD_USE_AGENT_AGENT | CD_USE_RESET;
+  if ((p->flags & ~allowed) || !(p->flags & CD_USE_RESET)) return CD_ERR_UNSUPPORTED;
+  if (!(p->dt_max > 0.0) || p->dt_min != p->dt_max) return CD_ERR_UNSUPPORTED;
+  if (w->n_scenarios > 0) return CD_ERR_UNSUPPORTED;
+  if (a->n == 0) return CD_OK;
+  hipStream_t s = (hipStream_t)stream;
+  const Agents<double> A = typed<double>(a);
+  const Work W = carve(w);
+  const Batch B = single_batch(w, 0, p->seed, p->dt_min, p->dt_max);
+  const Launch L = single_launch(w);
+  TanView T;
+  T.dx = (double2 *)tan->dx;
+  T.dv = (double2 *)tan->dv;
+  T.dg = (double2 *)tan->dg;
+  T.df = (double2 *)tan->df;
+  T.dp = tan->dp;
+  if (p->flags & CD_USE_AGENT_AGENT) {
+    if ((rc = launch_reduce(A, W, B, L.blocks_f, true, s))) return rc;
+    hipLaunchKernelGGL(k_bin_insert, dim3(L.blocks_f), dim3(kBS), 0, s, (const double2 *)A.pos,
+                       p->cell_size, W, B, 1);
+    hipLaunchKernelGGL(k_cell_gather<double>, dim3(grid_for((int64_t)W.cap * 64, kBS)),
+                       dim3(kBS), 0, s, A, W);
+    if ((rc = check_launch())) return rc;
+  }
+  hipLaunchKernelGGL(k_tangent, dim3(L.blocks_e), dim3(L.threads_e), 0, s, A, W, B, T,
+                     (int)p->flags, L.lanes);
+  hipLaunchKernelGGL(k_tan_post, dim3(grid_for(a->n, kBS)), dim3(kBS), 0, s, (int)a->n, A.mass,
+                     p->dt_max, T);
   return check_launch();
 }
 

@@ -23,6 +23,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <mutex>
 #include <vector>
 
@@ -814,6 +815,9 @@ int check_step(const cd_agents_t *a, const cd_step_params_t *p, const cd_workspa
   if ((p->flags & CD_USE_NAV) &&
       (!p->nav_maps || p->nav_rows < 2 || p->nav_cols < 2 || p->nav_targets < 0 ||
        !(p->nav_step > 0.0) || (p->nav_mode != 0 && p->nav_mode != 1)))
+    return CD_ERR_ARG;
+  if (p->wall_cells && (p->wall_nx < 1 || p->wall_ny < 1 ||
+                        p->wall_nx * p->wall_ny > ((int64_t)1 << 26) || !(p->wall_cell > 0.0)))
     return CD_ERR_ARG;
   return check_ws(a, w);
 }

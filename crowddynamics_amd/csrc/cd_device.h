@@ -25,7 +25,53 @@ struct __attribute__((aligned(2 * sizeof(T)))) V2 {
 // hardware's approximate reciprocal, square root and exponential (a few
 // ulp each), well inside the mode's stated error bound against the fp64
 // oracle.
-__device__ __forceinline__ double dhypot(double a, double b) { return hypot(a, b); }
+// hypot as the reference's C library computes it (np.hypot, the `length`
+// of core/vector2D.py:113, reaches the C library's hypot): sqrt(ax^2 + ay^2)
+// followed by one correction step from C. F. Borges, "An Improved Algorithm
+// for hypot_ref(a, b)" (2019), in its variant without fused multiply-add (the
+// library is built with -ffp-contract=off), inputs scaled by powers of two
+// where squaring them would overflow or underflow.  That library hypot is
+// faithful but not correctly rounded (about one result in 400 is the other
+// neighbour of the exact value); this sequence returns the same bits, so
+// distances that tie exactly (those of equally oriented three-circle agents)
+// resolve as in the reference, and dt's max |v| is the reference's.
+__device__ __forceinline__ double hypot_kernel(double ax, double ay) {
+  const double h = sqrt(ax * ax + ay * ay);
+  double t1, t2;
+  if (h <= 2.0 * ay) {
+    const double delta = h - ay;
+    t1 = ax * (2.0 * delta - ax);
+    t2 = (delta - 2.0 * (ax - ay)) * delta;
+  } else {
+    const double delta = h - ax;
+    t1 = 2.0 * delta * (ax - 2.0 * ay);
+    t2 = (4.0 * delta - ay) * ay + delta * delta;
+  }
+  return h - (t1 + t2) / (2.0 * h);
+}
+
+__device__ __forceinline__ double hypot_ref(double x, double y) {
+  if (!__builtin_isfinite(x) || !__builtin_isfinite(y)) {
+    if (__builtin_isinf(x) || __builtin_isinf(y)) return __builtin_inf();
+    return x + y;  // NaN
+  }
+  x = fabs(x);
+  y = fabs(y);
+  const double ax = x < y ? y : x, ay = x < y ? x : y;
+  constexpr double kScale = 0x1p-600, kLarge = 0x1p+511, kTiny = 0x1p-511, kEps = 0x1p-54;
+  if (ax > kLarge) {
+    if (ay <= ax * kEps) return ax + ay;
+    return hypot_kernel(ax * kScale, ay * kScale) / kScale;
+  }
+  if (ay < kTiny) {
+    if (ax >= ay / kEps) return ax + ay;
+    return hypot_kernel(ax / kScale, ay / kScale) * kScale;
+  }
+  if (ay <= ax * kEps) return ax + ay;
+  return hypot_kernel(ax, ay);
+}
+
+__device__ __forceinline__ double dhypot(double a, double b) { return hypot_ref(a, b); }
 __device__ __forceinline__ float dhypot(float a, float b) { return __builtin_amdgcn_sqrtf(a * a + b * b); }
 __device__ __forceinline__ double dsqrt(double a) { return sqrt(a); }
 __device__ __forceinline__ float dsqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
@@ -335,7 +381,7 @@ __device__ __forceinline__ double np_sign(double v) {
 // length l_w of the segment p0 -> p1 (returned as {tx, ty, l_w, 0}).
 __device__ __forceinline__ double4 seg_frame(double p0x, double p0y, double p1x, double p1y) {
   const double dx = p1x - p0x, dy = p1y - p0y;
-  const double l_w = hypot(dx, dy);
+  const double l_w = hypot_ref(dx, dy);
   return make_double4(dx / l_w, dy / l_w, l_w, 0.0);
 }
 
@@ -354,11 +400,11 @@ __device__ __forceinline__ void distance_circle_line(double x, double y, double 
   double l_t = -(tx * q1x + ty * q1y) - (tx * q0x + ty * q0y);
   double d_iw;
   if (l_t > l_w) {
-    d_iw = hypot(q0x, q0y);
+    d_iw = hypot_ref(q0x, q0y);
     nx = q0x / d_iw;
     ny = q0y / d_iw;
   } else if (l_t < -l_w) {
-    d_iw = hypot(q1x, q1y);
+    d_iw = hypot_ref(q1x, q1y);
     nx = q1x / d_iw;
     ny = q1y / d_iw;
   } else {
@@ -410,7 +456,7 @@ __device__ __forceinline__ bool pair_three_circle(const TCView &A, const TCView 
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       double x = A.px[i] - B.px[j], y = A.py[i] - B.py[j];
-      double d = hypot(x, y);
+      double d = hypot_ref(x, y);
       double h = d - (rA[i] + rB[j]);
       if (h < h_min || __builtin_isnan(h_min)) {
         h_min = h;

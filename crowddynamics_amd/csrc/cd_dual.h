@@ -125,7 +125,7 @@ __device__ __forceinline__ Dual<K> dexp(const Dual<K> &a) {
 template <int K>
 __device__ __forceinline__ Dual<K> dhypot(const Dual<K> &a, const Dual<K> &b) {
   Dual<K> r;
-  r.v = hypot(a.v, b.v);
+  r.v = hypot_ref(a.v, b.v);
 #pragma unroll
   for (int k = 0; k < K; ++k) r.d[k] = (a.v * a.d[k] + b.v * b.d[k]) / r.v;
   return r;

@@ -293,7 +293,7 @@ __device__ __forceinline__ bool nav_finish(const NavFetch &n, double &ex, double
   }
   const double gx = ((n.p01 - n.p00) * (1.0 - n.fy) + (n.p11 - n.p10) * n.fy) / n.step;
   const double gy = ((n.p10 - n.p00) * (1.0 - n.fx) + (n.p11 - n.p01) * n.fx) / n.step;
-  const double g = hypot(gx, gy);
+  const double g = hypot_ref(gx, gy);
   if (!(g > 0.0)) return false;
   ex = -gx / g;
   ey = -gy / g;
@@ -545,6 +545,11 @@ struct FusedArgs {
   const double4 *segs;
   double2 *tape_x;  // when set, the step also writes its new positions here ([n])
   void *tape_v;     // ... and its new velocities
+  // wall grid (cd_wall_grid_build) or null: offsets of the nx * ny cells,
+  // then the segment indices
+  const int *wall_cells;
+  int wall_nx, wall_ny;
+  double wall_minx, wall_miny, wall_cell;
 };
 
 constexpr int kTileThreads = 256;
@@ -826,7 +831,12 @@ __device__ __forceinline__ void own_apply(const Agents<T> &A, const FusedArgs &f
     A.e0[i] = e;
   }
   if (fa.flags & CD_USE_ADJUST) {
-    if (MODEL) A.phi0[i] = atan2((double)e.y, (double)e.x);
+    if (MODEL) {
+      // orient_towards_target_direction (core/steering/orientation.py:17-21),
+      // or, with CD_USE_ORIENT_VELOCITY, toward the velocity the step starts from
+      A.phi0[i] = (fa.flags & CD_USE_ORIENT_VELOCITY) ? atan2((double)o.vel.y, (double)o.vel.x)
+                                                       : atan2((double)e.y, (double)e.x);
+    }
     // force_adjust (core/motion/adjusting.py:16-51), as adjust_force
     const T s = (T)(o.mass / o.tau_adj);
     const T v0 = (T)o.v0;
@@ -1260,7 +1270,7 @@ __device__ __forceinline__ void stage_segs(const double4 *src, int cnt, double4 
 // work, so the loads overlap it).
 __device__ __forceinline__ void stage_walls(const FusedArgs &fa, const ScnDesc &d, double4 *sseg,
                                             bool fused) {
-  if (!fused || !(fa.flags & CD_USE_OBSTACLE)) return;
+  if (!fused || !(fa.flags & CD_USE_OBSTACLE) || fa.wall_cells) return;
   stage_segs(fa.segs + d.w0, min(kWallTile, d.w1 - d.w0), sseg);
 }
 
@@ -1379,21 +1389,41 @@ __device__ __forceinline__ void agent_forces(const FusedArgs &fa, const ScnDesc 
       ps.y.add(cy);
       if (MODEL) ps.m.add(mm);
     };
-    const int nw = d.w1 - d.w0;
-    const double4 *sfrm = sseg + kWallTile, *sbox = sseg + 2 * kWallTile;
-    if (have)
-      walls_each(self.w, MODEL, pos, vel, sseg, sfrm, min(kWallTile, nw), add, q, L, sbox);
-    for (int w0 = kWallTile; w0 < nw; w0 += kWallTile) {  // more segments than staged
-      const int cnt = min(kWallTile, nw - w0);
-      __syncthreads();
-      stage_segs(fa.segs + d.w0 + w0, cnt, sseg);
-      __syncthreads();
-      if (have) walls_each(self.w, MODEL, pos, vel, sseg, sfrm, cnt, add, q, L, sbox);
-    }
-    if (nw > kWallTile) {  // restore the first tile for the next agents
-      __syncthreads();
-      stage_walls(fa, d, sseg, true);
-      __syncthreads();
+    if (fa.wall_cells) {
+      // binned walls: only the segments listed for the agent's wall cell
+      // (a superset of those within its reach; outside the grid, none)
+      if (have) {
+        const double qx = (pos.x - fa.wall_minx) / fa.wall_cell;
+        const double qy = (pos.y - fa.wall_miny) / fa.wall_cell;
+        if (qx >= 0.0 && qy >= 0.0 && qx < (double)fa.wall_nx && qy < (double)fa.wall_ny) {
+          const int c = (int)qx * fa.wall_ny + (int)qy;
+          const int k0 = fa.wall_cells[c], k1 = fa.wall_cells[c + 1];
+          const int *ids = fa.wall_cells + (fa.wall_nx * fa.wall_ny + 1);
+          for (int k = k0 + q; k < k1; k += L) {
+            const double4 sg = fa.segs[d.w0 + ids[k]];
+            const double4 bx = make_double4(fmin(sg.x, sg.z), fmin(sg.y, sg.w),
+                                            fmax(sg.x, sg.z), fmax(sg.y, sg.w));
+            walls_each(self.w, MODEL, pos, vel, &sg, nullptr, 1, add, 0, 1, &bx);
+          }
+        }
+      }
+    } else {
+      const int nw = d.w1 - d.w0;
+      const double4 *sfrm = sseg + kWallTile, *sbox = sseg + 2 * kWallTile;
+      if (have)
+        walls_each(self.w, MODEL, pos, vel, sseg, sfrm, min(kWallTile, nw), add, q, L, sbox);
+      for (int w0 = kWallTile; w0 < nw; w0 += kWallTile) {  // more segments than staged
+        const int cnt = min(kWallTile, nw - w0);
+        __syncthreads();
+        stage_segs(fa.segs + d.w0 + w0, cnt, sseg);
+        __syncthreads();
+        if (have) walls_each(self.w, MODEL, pos, vel, sseg, sfrm, cnt, add, q, L, sbox);
+      }
+      if (nw > kWallTile) {  // restore the first tile for the next agents
+        __syncthreads();
+        stage_walls(fa, d, sseg, true);
+        __syncthreads();
+      }
     }
   }
   // exact sums over the L lanes (butterfly: every lane ends with the totals)
@@ -1845,6 +1875,12 @@ FusedArgs fused_args(const cd_step_params_t *p) {
   fa.segs = (const double4 *)p->segments;
   fa.tape_x = nullptr;
   fa.tape_v = nullptr;
+  fa.wall_cells = p->wall_cells;
+  fa.wall_nx = (int)p->wall_nx;
+  fa.wall_ny = (int)p->wall_ny;
+  fa.wall_minx = p->wall_minx;
+  fa.wall_miny = p->wall_miny;
+  fa.wall_cell = p->wall_cell;
   return fa;
 }
 
@@ -2025,6 +2061,10 @@ __global__ void k_status_out(const DevState *st, int64_t *out) {
 
 __global__ void k_clear_status(DevState *st) { st->status = 0; }
 __global__ void k_set_step(DevState *st, unsigned long long step) { st->step = step; }
+__global__ void __launch_bounds__(256) k_hypot(const double2 *xy, double *out, int64_t n) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) out[i] = hypot_ref(xy[i].x, xy[i].y);
+}
 __global__ void k_philox(const uint32_t *in, uint32_t *out) {
   uint32_t c[4] = {in[0], in[1], in[2], in[3]};
   philox4x32(c, in[4], in[5]);
@@ -2295,6 +2335,67 @@ int cd_reset(const cd_agents_t *a, cd_stream_t stream) {
   return check_launch();
 }
 
+int64_t cd_wall_grid_build(const double *segs, int64_t W, double reach, double cell,
+                           int32_t *out, int64_t capacity, cd_step_params_t *p) {
+  if (W <= 0 || W > 0x7fffffff || !segs || !(reach >= 0.0) || !(cell > 0.0) || !p ||
+      capacity < 0)
+    return CD_ERR_ARG;
+  double bx0 = HUGE_VAL, by0 = HUGE_VAL, bx1 = -HUGE_VAL, by1 = -HUGE_VAL;
+  for (int64_t w = 0; w < W; ++w) {
+    const double *s = segs + 4 * w;
+    for (int k = 0; k < 4; ++k)
+      if (!std::isfinite(s[k])) return CD_ERR_ARG;
+    bx0 = std::min(bx0, std::min(s[0], s[2]));
+    by0 = std::min(by0, std::min(s[1], s[3]));
+    bx1 = std::max(bx1, std::max(s[0], s[2]));
+    by1 = std::max(by1, std::max(s[1], s[3]));
+  }
+  // the force kernel skips a segment whose box lies farther than the
+  // agent's reach + 1e-6 from its centre; a further 1e-6 covers rounding
+  const double g = reach + 2e-6;
+  const double minx = bx0 - g, miny = by0 - g;
+  const double sx = (bx1 + g - minx) / cell, sy = (by1 + g - miny) / cell;
+  if (!(sx < 1e8 && sy < 1e8)) return CD_ERR_ARG;
+  const int64_t nx = (int64_t)sx + 1, ny = (int64_t)sy + 1;
+  if (nx * ny > ((int64_t)1 << 26)) return CD_ERR_ARG;
+  auto range = [&](double lo, double hi, double o, int64_t m, int64_t &a, int64_t &b) {
+    a = std::max<int64_t>(0, std::min<int64_t>(m - 1, (int64_t)std::floor((lo - o) / cell)));
+    b = std::max<int64_t>(0, std::min<int64_t>(m - 1, (int64_t)std::floor((hi - o) / cell)));
+  };
+  std::vector<int64_t> off(nx * ny + 1, 0);
+  for (int pass = 0; pass < 2; ++pass) {
+    std::vector<int64_t> fill;
+    if (pass == 1) {
+      for (int64_t c = 0; c < nx * ny; ++c) off[c + 1] += off[c];
+      const int64_t total = nx * ny + 1 + off[nx * ny];
+      if (total > 0x7fffffff) return CD_ERR_ARG;
+      p->wall_nx = nx;
+      p->wall_ny = ny;
+      p->wall_minx = minx;
+      p->wall_miny = miny;
+      p->wall_cell = cell;
+      if (!out || capacity < total) return total;
+      for (int64_t c = 0; c <= nx * ny; ++c) out[c] = (int32_t)off[c];
+      fill.assign(off.begin(), off.end() - 1);
+    }
+    for (int64_t w = 0; w < W; ++w) {  // cells in segment order: ids ascend
+      const double *s = segs + 4 * w;
+      int64_t ix0, ix1, iy0, iy1;
+      range(std::min(s[0], s[2]) - g, std::max(s[0], s[2]) + g, minx, nx, ix0, ix1);
+      range(std::min(s[1], s[3]) - g, std::max(s[1], s[3]) + g, miny, ny, iy0, iy1);
+      for (int64_t ix = ix0; ix <= ix1; ++ix)
+        for (int64_t iy = iy0; iy <= iy1; ++iy) {
+          const int64_t c = ix * ny + iy;
+          if (pass == 0)
+            off[c + 1] += 1;
+          else
+            out[nx * ny + 1 + fill[c]++] = (int32_t)w;
+        }
+    }
+  }
+  return nx * ny + 1 + off[nx * ny];
+}
+
 int cd_step_prepare(const cd_agents_t *a, const cd_step_params_t *p, const cd_workspace_t *w,
                     uint64_t first_step, cd_stream_t stream) {
   int rc = check_step(a, p, w);
@@ -2499,6 +2600,14 @@ int cd_graph_launch(void *graph_exec, cd_stream_t stream) {
 int cd_graph_destroy(void *graph_exec) {
   if (!graph_exec) return CD_ERR_ARG;
   return hipGraphExecDestroy((hipGraphExec_t)graph_exec) == hipSuccess ? CD_OK : CD_ERR_HIP;
+}
+
+int cd_hypot(const double *xy, double *out, int64_t n, cd_stream_t stream) {
+  if (n < 0 || (n > 0 && (!xy || !out)) || n > ((int64_t)1 << 40)) return CD_ERR_ARG;
+  if (n == 0) return CD_OK;
+  hipLaunchKernelGGL(k_hypot, dim3((unsigned)((n + kBS - 1) / kBS)), dim3(kBS), 0,
+                     (hipStream_t)stream, (const double2 *)xy, out, n);
+  return check_launch();
 }
 
 int cd_philox4x32(const uint32_t *in, uint32_t *out, cd_stream_t stream) {

@@ -265,7 +265,8 @@ def room(n=20000, seed=0, width=200.0, height=100.0, exit_width=1.2, steps=500,
 def corridor(n=50000, seed=0, width=300.0, height=150.0, n_obstacles=200, steps=300,
              cell_size=0.6):
     """ThreeCircle agents in a corridor with random line obstacles; desired
-    direction +x, target orientation from the direction (Orientation node)."""
+    direction +x; target orientation from the velocity (extra['orient'];
+    the reference's Orientation node takes it from the desired direction)."""
     rng = np.random.default_rng(seed)
     arr = empty_agents(n, 'three_circle')
     arr['r_t'] = 0.15
@@ -299,8 +300,11 @@ def corridor(n=50000, seed=0, width=300.0, height=150.0, n_obstacles=200, steps=
     arr['velocity'] = arr['target_velocity'][:, None] * arr['target_direction']
     arr['target'] = 0
     shoulders_host(arr)
+    # the workload aligns each body with its velocity (target_orientation =
+    # angle(v)); the reference's Orientation node aligns it with the desired
+    # direction (core/steering/orientation.py:17-21)
     return Scenario('corridor', arr, segs, cell_size, 0.001, 0.01, steps, seed=1,
-                    bounds=(0.0, 0.0, width, height))
+                    bounds=(0.0, 0.0, width, height), extra=dict(orient='velocity'))
 
 
 def ensemble(n_scenarios=256, seed=0, n_min=500, n_max=8000, rho=(0.5, 3.0), steps=1000,

@@ -52,11 +52,50 @@ def make_params(cell_size, dt_min, dt_max, flags, seed, segments, maps, nav):
     return p
 
 
+#: the fused step bins the walls into cells of this side (m) when a scenario
+#: has at least WALL_GRID_MIN_SEGMENTS segments (results are the same either way)
+WALL_GRID_CELL = 1.0
+WALL_GRID_MIN_SEGMENTS = 32
+
+
+def wall_reach(array):
+    """Largest distance from an agent's centre at which a wall contact can
+    occur: the radius of circular agents, r_ts + max(r_t, r_s) of
+    three-circle agents."""
+    if len(array) == 0:
+        return 0.0
+    if 'position_ls' in array.dtype.names:
+        return float(np.max(array['r_ts'] + np.maximum(array['r_t'], array['r_s'])))
+    return float(np.max(array['radius']))
+
+
+def build_wall_grid(params, segments, reach, cell):
+    """Bins the segments ([W, 4] tensor) into cells of side `cell`
+    (cd_wall_grid_build) and points `params` at the device copy, which the
+    caller keeps alive as long as the parameters."""
+    L = _lib.lib()
+    seg = np.ascontiguousarray(segments.detach().cpu().numpy(), dtype=np.float64)
+    ptr = C.c_void_p(seg.ctypes.data)
+    need = L.cd_wall_grid_build(ptr, len(seg), float(reach), float(cell), None, 0,
+                                C.byref(params))
+    if need < 0:
+        _lib.check(int(need), 'wall_grid_build')
+    out = np.zeros(int(need), dtype=np.int32)
+    got = L.cd_wall_grid_build(ptr, len(seg), float(reach), float(cell),
+                               C.c_void_p(out.ctypes.data), int(need), C.byref(params))
+    if got != need:
+        _lib.check(int(got) if got < 0 else _lib.CD_ERR_ARG, 'wall_grid_build')
+    dev = torch.from_numpy(out).to(segments.device)
+    params.wall_cells = C.c_void_p(dev.data_ptr())
+    return dev
+
+
 class Stepper:
     def __init__(self, array, segments=None, nav=None, cell_size=3.6, dt_min=0.01,
                  dt_max=0.01, seed=1, flags=ALL_NODES, precision='f64', cell_capacity=None,
                  bounds=None, first_step=0, device=None, device_agents=None,
-                 cell_slots=None, force_threads=0, tile=None, lanes=0, order_every=0):
+                 cell_slots=None, force_threads=0, tile=None, lanes=0, order_every=0,
+                 wall_grid='auto'):
         if device_agents is not None:
             self.agents = device_agents
         else:
@@ -90,6 +129,12 @@ class Stepper:
                             order_every=order_every)
         self.params = make_params(cell_size, dt_min, dt_max, flags, seed, self.segments,
                                   self.maps, nav)
+        self.wall_cells = None
+        if wall_grid == 'auto':
+            wall_grid = WALL_GRID_CELL if len(self.segments) >= WALL_GRID_MIN_SEGMENTS else None
+        if wall_grid and (flags & _lib.CD_USE_OBSTACLE):
+            self.wall_cells = build_wall_grid(self.params, self.segments, wall_reach(array),
+                                              float(wall_grid))
         self.flags = flags
         self.prepare(first_step)
 
@@ -171,6 +216,29 @@ class Stepper:
         torch.cuda.current_stream().synchronize()
         self.agents.download(array)
         return array
+
+
+def capture_many(steppers, k):
+    """One HIP graph of k rounds of one step of every stepper in turn (a
+    per-scenario loop of plain steps, launched on the current stream)."""
+    if torch.cuda.current_stream() == torch.cuda.default_stream():
+        raise InvalidValue('capture needs a non-default stream (use torch.cuda.stream)')
+    L = _lib.lib()
+    s = stream_handle()
+    _lib.check(L.cd_graph_begin(s), 'graph_begin')
+    try:
+        for _ in range(k):
+            for st in steppers:
+                st.step(1)
+    except Exception:
+        ex = C.c_void_p()
+        L.cd_graph_end(s, C.byref(ex))
+        if ex.value:
+            L.cd_graph_destroy(ex)
+        raise
+    ex = C.c_void_p()
+    _lib.check(L.cd_graph_end(s, C.byref(ex)), 'graph_end')
+    return Graph(ex)
 
 
 class Graph:

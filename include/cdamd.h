@@ -226,7 +226,11 @@ enum {
   CD_USE_ADJUST = 4,        /* Adjusting (+ Orientation for three-circle) */
   CD_USE_AGENT_AGENT = 8,
   CD_USE_OBSTACLE = 16,
-  CD_USE_RESET = 32
+  CD_USE_RESET = 32,
+  /* Orientation: target_orientation = angle(velocity) instead of the
+   * reference's angle(target_direction) (core/steering/orientation.py:17-21);
+   * three-circle agents, fused step only */
+  CD_USE_ORIENT_VELOCITY = 64
 };
 
 typedef struct cd_step_params {
@@ -239,7 +243,33 @@ typedef struct cd_step_params {
   const double *nav_maps;
   int64_t nav_targets, nav_rows, nav_cols;
   double nav_minx, nav_miny, nav_step;
+  /* Optional wall grid of the fused step of a single scenario (device
+   * memory written by cd_wall_grid_build and copied to the device): the
+   * segments binned into square cells, so that an agent visits only the
+   * segments within its reach.  NULL: every agent scans every segment.
+   * Same results either way. */
+  const int32_t *wall_cells;
+  int64_t wall_nx, wall_ny;
+  double wall_minx, wall_miny, wall_cell;
 } cd_step_params_t;
+
+/* Host only (no device call): bins segments ([W][4] fp64, HOST memory) into
+ * square cells of side `cell` over their bounding box grown by `reach`, the
+ * largest distance from an agent's centre at which a wall contact can occur
+ * (circular: the radius; three-circle: r_ts + max(r_t, r_s)).  A cell lists
+ * every segment whose bounding box lies within reach of it, so an agent in
+ * the cell is within reach of no other segment, and an agent outside the grid
+ * of none.  Writes into out[0 .. capacity) the offsets of the nx * ny cells
+ * ([nx * ny + 1], flat index ix * ny + iy) followed by the segment indices,
+ * and into *params the grid fields (wall_nx, wall_ny, wall_minx, wall_miny,
+ * wall_cell; the caller sets wall_cells to the device copy of out).  Returns
+ * the number of int32 the grid needs (call with out NULL to size it), or a
+ * negative CD_ERR_* code.  Replaces the per-step O(N W) scan of
+ * agent_obstacle (crowddynamics/core/interactions.py:169-186) inside the
+ * fused step. */
+int64_t cd_wall_grid_build(const double *segments, int64_t n_segments, double reach,
+                           double cell, int32_t *out, int64_t capacity,
+                           cd_step_params_t *params);
 
 /* One MultiAgentSimulation.update() (crowddynamics/simulation/multiagent.py:51-55)
  * of the tree Reset << (Integrator << (Fluctuation, Adjusting << (Navigation,
@@ -298,6 +328,23 @@ int cd_adjoint_step(const cd_agents_t *agents, const cd_step_params_t *params,
                     const double *grad_x, const cd_adjoint_t *adj, const cd_workspace_t *ws,
                     cd_stream_t stream);
 
+/* Tangent (forward-mode) step, the scope of cd_adjoint_step: from the
+ * state x_k, v_k in `agents`, maps the tangent (dx, dv, dg) of (position,
+ * velocity, force_prev) at step k to step k+1 in place, along the parameter
+ * tangent dp ([n][5]: tau_0, mu, kappa, damping, target_velocity); df
+ * ([n][2]) receives the tangent of the step's force.  With cd_adjoint_step
+ * it gives the dot-product test <J t, u> = <t, J^T u> of a rollout.
+ * Stream-ordered, no allocation.  No reference counterpart (the reference
+ * has no derivatives of interactions.py:51-70, adjusting.py:16-51 or
+ * integrator.py:164-176). */
+typedef struct cd_tangent {
+  double *dx, *dv, *dg; /* [n][2] tangent state, in/out */
+  double *df;           /* [n][2] out: tangent of the step's force */
+  const double *dp;     /* [n][5] parameter tangent */
+} cd_tangent_t;
+int cd_tangent_step(const cd_agents_t *agents, const cd_step_params_t *params,
+                    const cd_tangent_t *tan, const cd_workspace_t *ws, cd_stream_t stream);
+
 /* Differentiable rollout drivers (circular, fp64, the cd_adjoint_step scope).
  * cd_rollout_record: from the agents' current state (after cd_step_prepare),
  * `steps` fused steps; positions to x_out [steps + 1][n][2] (x_out[0] the
@@ -352,6 +399,11 @@ int cd_graph_destroy(void *graph_exec);
 /* Philox4x32-10 on the device for one counter (testing the noise stream):
  * in = {ctr0..3, key0, key1}, out = 4 words. */
 int cd_philox4x32(const uint32_t *in_dev, uint32_t *out_dev, cd_stream_t stream);
+/* The device's length of (x, y) for n pairs xy ([n][2] fp64, device) into
+ * out ([n], device): the sequence of the C library's hypot that np.hypot
+ * reaches (core/vector2D.py:113), bit for bit (testing the lengths every
+ * distance, normal and max |v| of the step takes). */
+int cd_hypot(const double *xy, double *out, int64_t n, cd_stream_t stream);
 const char *cd_version(void);
 
 #ifdef __cplusplus

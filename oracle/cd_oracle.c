@@ -664,6 +664,14 @@ void orc_orient_towards_target_direction(orc_agents *A) {
     A->target_orientation[i] = atan2(A->target_direction[2 * i + 1], A->target_direction[2 * i]);
 }
 
+/* The same with the velocity as the direction (no reference counterpart:
+ * a workload that aligns the body with its motion instead of its desired
+ * direction). */
+void orc_orient_towards_velocity(orc_agents *A) {
+  for (int64_t i = 0; i < A->n; ++i)
+    A->target_orientation[i] = atan2(A->velocity[2 * i + 1], A->velocity[2 * i]);
+}
+
 /* Navigation.update (simulation/logic.py:149-165) with meshgrid().indicer
  * (core/steering/quickest_path.py:41-44) and getdefault/is_inside
  * (core/steering/navigation.py:59-78).  `maps` holds n_targets direction
@@ -895,7 +903,7 @@ void orc_fluctuation(orc_agents *A, uint64_t seed, uint64_t s) {
 typedef struct orc_step_cfg {
   double cell_size, dt_min, dt_max;
   int32_t use_fluct, use_nav, nav_mode, use_adjust, use_agent_agent, use_obstacle, use_reset;
-  int32_t _pad;
+  int32_t orient_velocity; /* target_orientation = angle(velocity) (a workload's choice) */
   uint64_t seed;
   const double *segments;
   int64_t n_segments;
@@ -919,7 +927,12 @@ double orc_step(orc_agents *A, const orc_step_cfg *cfg, uint64_t step_index) {
       orc_navigation_bilinear(A, cfg->nav_maps, cfg->nav_targets, cfg->nav_rows,
                               cfg->nav_cols, cfg->nav_minx, cfg->nav_miny, cfg->nav_step);
   }
-  if (A->model == 1 && cfg->use_adjust) orc_orient_towards_target_direction(A);
+  if (A->model == 1 && cfg->use_adjust) {
+    if (cfg->orient_velocity)
+      orc_orient_towards_velocity(A);
+    else
+      orc_orient_towards_target_direction(A);
+  }
   if (cfg->use_adjust) {
     orc_force_adjust_agents(A);
     if (A->model == 1) orc_torque_adjust_agents(A);

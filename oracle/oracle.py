@@ -97,7 +97,8 @@ class StepConfig(C.Structure):
     _fields_ = [('cell_size', C.c_double), ('dt_min', C.c_double), ('dt_max', C.c_double),
                 ('use_fluct', C.c_int32), ('use_nav', C.c_int32), ('nav_mode', C.c_int32),
                 ('use_adjust', C.c_int32), ('use_agent_agent', C.c_int32),
-                ('use_obstacle', C.c_int32), ('use_reset', C.c_int32), ('_pad', C.c_int32),
+                ('use_obstacle', C.c_int32), ('use_reset', C.c_int32),
+                ('orient_velocity', C.c_int32),
                 ('seed', C.c_uint64), ('segments', _dp), ('n_segments', C.c_int64),
                 ('nav_maps', _dp), ('nav_targets', C.c_int64), ('nav_rows', C.c_int64),
                 ('nav_cols', C.c_int64), ('nav_minx', C.c_double), ('nav_miny', C.c_double),
@@ -348,7 +349,7 @@ class Scenario:
     def __init__(self, cell_size=3.6, dt_min=0.01, dt_max=0.01, seed=0,
                  segments=None, nav_maps=None, nav_mode=0, nav_origin=(0.0, 0.0),
                  nav_step=0.1, fluct=True, adjust=True, agent_agent=True,
-                 obstacle=True, reset=True, navigation=True):
+                 obstacle=True, reset=True, navigation=True, orient_velocity=False):
         self.segments = (np.ascontiguousarray(_segments_array(segments))
                          if segments is not None and len(segments) else np.zeros((0, 4)))
         self.nav_maps = (np.ascontiguousarray(nav_maps, dtype=np.float64)
@@ -362,6 +363,7 @@ class Scenario:
         cfg.use_agent_agent = int(agent_agent)
         cfg.use_obstacle = int(obstacle)
         cfg.use_reset = int(reset)
+        cfg.orient_velocity = int(orient_velocity)
         cfg.seed = seed
         cfg.segments = _ptr(self.segments)
         cfg.n_segments = len(self.segments)

@@ -1,0 +1,211 @@
+"""Parity at the full sizes of the benchmark workloads.
+
+* corridor50k: 50 000 three-circle agents in a 300 x 150 m corridor with 200
+  random line obstacles plus 4 walls, cell 0.6 m, fp64 (orientation and
+  torque integration): one fused step, then three more, against the oracle.
+* ensemble256: the 256-scenario batch (agent counts log-uniform in
+  [500, 8000], 760k agents, densities U[0.5, 3]), advanced as one batch.
+  The smallest, the median and the largest scenario are checked against the
+  oracle at every one of 5 steps, in fp64 (trajectory) and in the fp32 mode
+  (one oracle step from the GPU's state, with the fp32 bound), and against
+  their standalone GPU runs bit for bit.
+"""
+import numpy as np
+import pytest
+import torch
+
+from crowddynamics_amd import _lib
+from crowddynamics_amd import examples as E
+from crowddynamics_amd.autotune import max_body_radius, neutral_cell_sizes
+from crowddynamics_amd.ensemble import Ensemble
+from crowddynamics_amd.stepper import ALL_NODES, Stepper
+from oracle import oracle as O
+from tests.helpers import assert_vec_close
+
+pytestmark = pytest.mark.gpu
+
+TC_FIELDS = ('position', 'velocity', 'force_prev', 'orientation', 'angular_velocity',
+             'torque_prev', 'position_ls', 'position_rs')
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def _oracle_scenario(scn, cell_size=None, fluct=True, orient_velocity=False):
+    nv = scn.nav
+    return O.Scenario(cell_size=cell_size or scn.cell_size, dt_min=scn.dt_min,
+                      dt_max=scn.dt_max, seed=scn.seed, segments=scn.segments,
+                      nav_maps=nv['maps'] if nv else None, nav_mode=nv['mode'] if nv else 0,
+                      nav_origin=(nv['minx'], nv['miny']) if nv else (0.0, 0.0),
+                      nav_step=nv['step'] if nv else 0.1, fluct=fluct,
+                      orient_velocity=orient_velocity)
+
+
+def _compare(out, ref, fields, rtol, what):
+    for f in fields:
+        if f not in ref.dtype.names:
+            continue
+        r, g = ref[f], out[f]
+        if r.ndim == 2:
+            scale = np.hypot(*r.T) + 1.0
+        else:
+            scale = np.abs(r) + 1.0
+        assert_vec_close(g, r, scale, rtol, '%s %s' % (what, f))
+
+
+# ------------------------------------------------------------- corridor50k --
+
+@pytest.fixture(scope='module')
+def corridor50k():
+    return E.corridor()
+
+
+@pytest.mark.parametrize('orient', ['velocity', 'direction'])
+def test_corridor50k_three_circle_fp64(corridor50k, orient):
+    """Both orientation modes: target_orientation = angle(velocity), as the
+    workload asks, and the reference's angle(target_direction)."""
+    scn = corridor50k
+    assert scn.n == 50000 and len(scn.segments) == 204
+    assert scn.extra['orient'] == 'velocity'
+    vel = orient == 'velocity'
+    flags = ALL_NODES | (_lib.CD_USE_ORIENT_VELOCITY if vel else 0)
+    say('corridor50k (orientation from %s): one fused step against the oracle' % orient)
+    ref = scn.agents.copy()
+    osc = _oracle_scenario(scn, orient_velocity=vel)
+    dts = O.run(ref, osc, 1)
+    st = Stepper(scn.agents.copy(), segments=scn.segments, cell_size=scn.cell_size,
+                 dt_min=scn.dt_min, dt_max=scn.dt_max, seed=scn.seed, precision='f64',
+                 bounds=scn.bounds, flags=flags)
+    st.step(1)
+    s = st.check()
+    assert s.dt == dts[0]
+    out = st.download(scn.agents.copy())
+    _compare(out, ref, TC_FIELDS + ('target_orientation',), 1e-13, 'corridor50k step 1')
+    assert np.all(out['force'] == 0.0) and np.all(out['torque'] == 0.0)
+    # the obstacles are in play: some agents touch a wall segment
+    say('corridor50k: three more steps, trajectory against the oracle')
+    dts3 = O.run(ref, osc, 3, first_step=1)
+    st.step(3)
+    s = st.check()
+    assert s.step == 4
+    assert abs(s.time_tot - (dts[0] + dts3.sum())) < 1e-15
+    out = st.download(scn.agents.copy())
+    _compare(out, ref, TC_FIELDS + ('target_orientation',), 1e-11, 'corridor50k step 4')
+    if vel:  # the modes differ once velocities turn away from the desired direction
+        assert np.any(out['target_orientation'] != 0.0)
+    err = np.abs(out['position'] - ref['position']).max()
+    say('corridor50k: max |dx| after 4 steps %.3e m' % err)
+
+
+# ------------------------------------------------------------- ensemble256 --
+
+@pytest.fixture(scope='module')
+def ensemble256():
+    scns = E.ensemble(256)
+    cell = neutral_cell_sizes(max_body_radius(scns))[0]
+    order = sorted(range(len(scns)), key=lambda k: scns[k].n)
+    picks = (order[0], order[len(order) // 2], order[-1])
+    return scns, cell, picks
+
+
+def _one_step_bounds(before, scn, cell, step):
+    """One oracle step from the state `before` and the per-agent error
+    scales of its results: the step's force sums terms whose magnitudes add
+    up to S (pair terms) + |F| + 1 N, so two summation orders (or fp32
+    arithmetic) differ by a multiple of that; velocity and position inherit
+    it through dt / (2 m) and dt (plus their own magnitudes' rounding)."""
+    ref = before.copy()
+    O.run(ref, _oracle_scenario(scn, cell), 1, first_step=step)
+    probe = before.copy()
+    probe['force'] = 0.0
+    S, _ = O.agent_agent_scale(probe, cell)
+    fmag = S + np.hypot(*ref['force_prev'].T) + 1.0
+    h = scn.dt_max
+    vmag = fmag * h / (2.0 * ref['mass']) + np.hypot(*ref['velocity'].T) + 1.0
+    xmag = vmag * h + np.hypot(*ref['position'].T) + 1.0
+    return ref, dict(force_prev=fmag, velocity=vmag, position=xmag)
+
+
+#: fp32 mode: share of agents whose step is within 1e-4 of its error scale.
+#: Rooms packed to 3 agents/m^2 hold many pairs near grazing, where the
+#: time-to-collision discriminant b^2 - ac cancels and fp32's relative error
+#: grows: measured per step, 99.79-100 % of the agents are within 1e-4 and
+#: all within 3e-3 (99.9 % within 1e-4 in the 20k room at 1 agent/m^2).
+FP32_FRAC = 0.995
+
+
+def _rel(out, ref, scales, f):
+    return np.hypot(*(out[f] - ref[f]).T) / scales[f]
+
+
+def test_ensemble256_fp64_against_oracle(ensemble256):
+    """fp64 batch: at each of 5 steps, the smallest, median and largest
+    scenario equal one oracle step from the GPU's state to 1e-12 of their
+    error scales (dense rooms are contact-heavy, so trajectories are compared
+    step by step), and after 5 steps the trajectories agree to 1e-9 m."""
+    scns, cell, picks = ensemble256
+    say('ensemble256 fp64: %d agents, cell %.3f, scenarios %s' % (
+        sum(s.n for s in scns), cell, [(k, scns[k].n) for k in picks]))
+    ens = Ensemble(scns, precision='f64', cell_size=cell, lanes=4, order_every=10)
+    traj = {k: scns[k].agents.copy() for k in picks}
+    for step in range(5):
+        before = ens.download_scenarios()
+        ens.step(1)
+        after = ens.download_scenarios()
+        for k in picks:
+            ref, sc = _one_step_bounds(before[k], scns[k], cell, step)
+            for f in ('position', 'velocity', 'force_prev'):
+                rel = _rel(after[k], ref, sc, f)
+                assert rel.max() <= 1e-12, (step, k, f, rel.max())
+            O.run(traj[k], _oracle_scenario(scns[k], cell), 1, first_step=step)
+    st = ens.check()
+    assert st.step == 5
+    outs = ens.download_scenarios()
+    for k in picks:
+        err = np.abs(outs[k]['position'] - traj[k]['position']).max()
+        say('  scenario %d (n=%d): max |dx| after 5 steps %.3e m' % (k, scns[k].n, err))
+        assert err < 1e-9
+
+
+def test_ensemble256_fp32_against_oracle_and_standalone(ensemble256):
+    """fp32 mode (fp64 positions and dt), the configuration the batch is
+    run in: at each of 5 steps, one oracle step (fp64) from the GPU's state;
+    within 1e-4 of the error scales for 99.5 % of the agents and 1e-2 for
+    all (the fp32 mode's bound in dense crowds); then the same scenarios run alone
+    give the batch's results bit for bit."""
+    scns, cell, picks = ensemble256
+    say('ensemble256 fp32: one oracle step per step from the GPU state')
+    ens = Ensemble(scns, precision='f32', cell_size=cell, lanes=4, order_every=10)
+    stats = []
+    for step in range(5):
+        before = ens.download_scenarios()
+        ens.step(1)
+        after = ens.download_scenarios()
+        for k in picks:
+            ref, sc = _one_step_bounds(before[k], scns[k], cell, step)
+            for f in ('position', 'velocity', 'force_prev'):
+                rel = _rel(after[k], ref, sc, f)
+                fr = [float(np.mean(rel <= t)) for t in (1e-6, 1e-5, 1e-4, 1e-3)]
+                say('  step %d scenario %d %s: within 1e-6/1e-5/1e-4/1e-3: %s, max %.2e'
+                    % (step, k, f, ' '.join('%.5f' % x for x in fr), rel.max()))
+                stats.append((step, k, f, fr[2], float(rel.max())))
+    for step, k, f, frac, mx in stats:
+        assert frac >= FP32_FRAC, (step, k, f, frac)
+        assert mx <= 1e-2, (step, k, f, mx)
+    st = ens.check()
+    outs = ens.download_scenarios()
+    states = ens.states()
+    # the same scenarios run alone: bit for bit
+    for k in picks:
+        s = scns[k]
+        alone = Stepper(s.agents.copy(), segments=s.segments, cell_size=cell,
+                        dt_min=s.dt_min, dt_max=s.dt_max, seed=s.seed, precision='f32',
+                        bounds=s.bounds)
+        alone.step(5)
+        sa = alone.check()
+        out = alone.download(s.agents.copy())
+        for f in ('position', 'velocity', 'force', 'force_prev', 'target_direction'):
+            assert np.array_equal(out[f], outs[k][f]), (k, f)
+        assert sa.dt == states[k].dt and sa.time_tot == states[k].time_tot
+    assert st.status & ~_lib.CD_STATUS_SLOTS_FULL == 0

@@ -698,13 +698,17 @@ def test_step_many_walls(hallway, corridor_small, which, n_extra):
         r = ref[f]
         assert_vec_close(out[f], r, np.hypot(*r.T) + 1.0, 1e-12, '%s %s' % (which, f))
     outs = []
-    runs = ((False, None), (True, None)) + (((True, (3, 2)),) if which == 'hallway' else ())
-    for rev, tile in runs:
-        say('%s: 5 steps, segments reversed %s, tile %s' % (which, rev, tile))
+    # the segments binned into wall cells (the default at this many
+    # segments: 1 m and 0.4 m cells) and scanned whole (wall_grid None)
+    runs = ((False, None, 'auto'), (True, None, 'auto'), (False, None, None),
+            (True, None, 0.4)) + (((True, (3, 2), 'auto'),) if which == 'hallway' else ())
+    for rev, tile, grid in runs:
+        say('%s: 5 steps, segments reversed %s, tile %s, wall grid %s' % (which, rev, tile, grid))
         segs = scn.segments[::-1].copy() if rev else scn.segments
         st = Stepper(scn.agents.copy(), segments=segs, nav=scn.nav, cell_size=scn.cell_size,
                      dt_min=scn.dt_min, dt_max=scn.dt_max, seed=scn.seed, bounds=scn.bounds,
-                     tile=tile)
+                     tile=tile, wall_grid=grid)
+        assert (st.wall_cells is None) == (grid is None)
         st.step(5)
         st.check()
         outs.append(st.download(scn.agents.copy()))
