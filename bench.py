@@ -436,7 +436,8 @@ def _gradient_summary(g):
                 v0_max_abs=float(g[4].abs().max()))
 
 
-def adjoint_checks(arr, scn, lanes, windows=(10, 25, 50, 100), hs=(1e-4, 1e-5, 1e-6, 1e-7, 1e-8),
+def adjoint_checks(arr, scn, lanes, windows=(5, 10, 25, 50, 100),
+                   hs=(1e-4, 1e-5, 1e-6, 1e-7, 1e-8),
                    checkpoint=25):
     """On the differentiable workload, for rollouts of each length in
     `windows`: the dot-product test of the adjoint against the forward-mode
@@ -487,11 +488,19 @@ def adjoint_checks(arr, scn, lanes, windows=(10, 25, 50, 100), hs=(1e-4, 1e-5, 1
                 d.fill_diagonal_(float('inf'))
                 gap = min(gap, float(d.min()))
         adj = float(g[0])
-        floor = 100 * np.finfo(np.float64).eps * abs(L) / 1e-6
-        agree = abs(fd['1e-06'] - adj) <= 1e-4 * abs(adj) + floor
+
+        def rel_diff(h):
+            floor = 100 * np.finfo(np.float64).eps * abs(L) / h
+            return abs(fd['%g' % h] - adj) / (abs(adj) + floor + 1e-300)
+        agree = rel_diff(1e-6) <= 1e-4
+        best_h = min(hs, key=rel_diff)
         rec = dict(steps=W, loss=L, dot_product=dict(tangent=lhs, adjoint=rhs,
                                                       rel_diff=dot_rel),
-                   tau_0_adjoint=adj, tau_0_central_fd=fd, fd_agrees_at_h_1e_6=bool(agree),
+                   tau_0_adjoint=adj, tau_0_central_fd=fd,
+                   fd_rel_diff={'%g' % h: rel_diff(h) for h in hs},
+                   fd_agrees_at_h_1e_6=bool(agree),
+                   fd_best=dict(h=best_h, rel_diff=rel_diff(best_h),
+                                converged=bool(rel_diff(best_h) <= 1e-4)),
                    min_gap_m=gap, gradient=_gradient_summary(g))
         log('adjoint window %d steps: dot-product rel diff %.2e, dL/dtau_0 adjoint %.6e, '
             'fd(1e-6) %.6e (%s), min gap %.3f m' % (W, dot_rel, adj, fd['1e-06'],
@@ -560,6 +569,8 @@ def bench_adjoint(n, steps, checkpoint):
     checks = adjoint_checks(arr, scn, lanes)
     good = [c for c in checks if c['fd_agrees_at_h_1e_6']]
     best = max(good, key=lambda c: c['steps']) if good else None
+    conv = [c for c in checks if c['fd_best']['converged']]
+    best_conv = max(conv, key=lambda c: c['steps']) if conv else None
     return dict(agents=n, steps=steps, checkpoint=checkpoint, dt=scn.dt_max, lanes=lanes,
                 lane_ms=lane_ms, loss=float(L.detach()),
                 forward_ms_per_step=1e3 * fwd / steps,
@@ -576,7 +587,13 @@ def bench_adjoint(n, steps, checkpoint):
                 fd_confirmed_window=(dict(steps=best['steps'], gradient=best['gradient'],
                                           loss=best['loss'],
                                           tau_0_central_fd=best['tau_0_central_fd']['1e-06'])
-                                     if best else None))
+                                     if best else None),
+                fd_converged_window=(dict(steps=best_conv['steps'],
+                                          h=best_conv['fd_best']['h'],
+                                          rel_diff=best_conv['fd_best']['rel_diff'],
+                                          gradient=best_conv['gradient'])
+                                     if best_conv else None),
+                dot_product_max_rel_diff=max(c['dot_product']['rel_diff'] for c in checks))
 
 
 def main():

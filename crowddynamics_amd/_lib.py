@@ -143,6 +143,20 @@ SIGNATURES = {
                                 _P(C.c_double), _vp]),
     'cd_kernel_timed': (C.c_int, [_P(CDAgents), _P(CDStepParams), _W, C.c_int32, C.c_int64,
                                   _P(C.c_double), _vp]),
+    'cd_field_work_bytes': (C.c_int64, [C.c_int64, C.c_int64]),
+    'cd_distance_map': (C.c_int, [_vp, _vp, C.c_int64, C.c_int64, C.c_double, _vp, _vp,
+                                  C.c_int64, _vp]),
+    'cd_direction_map': (C.c_int, [_vp, C.c_int64, C.c_int64, _vp, _vp, _vp]),
+    'cd_obstacle_handling': (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_int64, C.c_double,
+                                       C.c_double, _vp]),
+    'cd_fill_missing': (C.c_int, [_vp, _vp, _vp, C.c_int64, C.c_int64, C.c_int32, _vp, _vp,
+                                  _vp]),
+    'cd_inside_domain': (C.c_int, [_vp, C.c_int64, _vp, C.c_int64, _vp, _vp, _vp]),
+    'cd_target_reached': (C.c_int, [_vp, C.c_int64, _vp, C.c_int64, _vp, _vp, _vp]),
+    'cd_nearest_neighbors': (C.c_int, [_P(CDAgents), C.c_double, C.c_int32, _vp, C.c_int64,
+                                       _vp, _W, _vp]),
+    'cd_herding_interaction': (C.c_int, [_vp, _vp, C.c_int64, _vp, _vp, C.c_int32, C.c_double,
+                                         C.c_double, _vp, _vp, _vp]),
     'cd_graph_begin': (C.c_int, [_vp]),
     'cd_graph_end': (C.c_int, [_vp, _P(_vp)]),
     'cd_graph_launch': (C.c_int, [_vp, _vp]),

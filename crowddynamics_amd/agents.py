@@ -141,6 +141,8 @@ SCALAR_FIELDS = ('radius', 'r_t', 'r_s', 'r_ts', 'mass', 'inertia_rot',
                  'orientation', 'angular_velocity', 'target_orientation', 'torque',
                  'torque_prev', 'tau_rot', 'std_rand_torque')
 INT_FIELDS = ('target',)
+#: Boolean fields mirrored on the device as uint8 (state nodes, herding).
+BOOL_FIELDS = ('active', 'target_reached', 'is_leader', 'is_follower')
 #: Fields the step writes (downloaded by ``DeviceAgents.download``).
 MUTABLE_FIELDS = ('position', 'velocity', 'target_direction', 'force', 'force_prev',
                   'position_ls', 'position_rs', 'orientation', 'angular_velocity',

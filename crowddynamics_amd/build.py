@@ -19,7 +19,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'libcdamd.so')
-SOURCES = ['cd_kernels.hip', 'cd_adjoint.hip']
+SOURCES = ['cd_kernels.hip', 'cd_adjoint.hip', 'cd_field.hip', 'cd_state.hip']
 
 HIPCC_FLAGS = [
     '--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC', '-shared',

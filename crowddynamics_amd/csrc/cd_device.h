@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace cd {
 
 constexpr double kSightSoc = 3.0;  // core/interactions.py:45
@@ -277,12 +279,31 @@ struct Owner {
 // pair adds nothing).  The j-side of the reference pair equals this function
 // evaluated with i and j swapped bit for bit (every term is odd in (x, v)
 // and IEEE negation is exact), so owner-computes reproduces both sides.
+//
+// In fp64 the exact distance d (the library hypot) matters only near the
+// sight limit and in contact: h = d - r_tot enters the result only through
+// the tests h < 3 and h < 0 and, in contact, the contact force.  s = x^2 +
+// y^2 is within 2^-52 of d^2 and the hypot within 2^-52 of d, so s below
+// (3 + r_tot)^2 (1 - 2^-40) and above r_tot^2 (1 + 2^-40) decides both tests
+// as d would (h in (0, 3)): there the pair skips d; s above (3 + r_tot)^2
+// (1 + 2^-40) rejects it.
+constexpr double kGate = 0x1p-40;
+
 template <typename T>
 __device__ __forceinline__ bool pair_circular(T x, T y, T vx, T vy, T r_tot, const Owner<T> &o,
                                               T &fx, T &fy) {
-  T d = dhypot(x, y);
-  T h = d - r_tot;
-  if (!(h < T(kSightSoc))) return false;
+  T d = T(0), h = T(0);
+  bool contact_possible = true;
+  if constexpr (std::is_same<T, double>::value) {
+    const double s = x * x + y * y, rs = kSightSoc + r_tot;
+    if (s > (rs * rs) * (1.0 + kGate)) return false;
+    contact_possible = !(s < (rs * rs) * (1.0 - kGate) && s > (r_tot * r_tot) * (1.0 + kGate));
+  }
+  if (contact_possible) {
+    d = dhypot(x, y);
+    h = d - r_tot;
+    if (!(h < T(kSightSoc))) return false;
+  }
   T sx = T(0), sy = T(0);
   T a = vx * vx + vy * vy;
   T b = -(x * vx + y * vy);
@@ -299,7 +320,7 @@ __device__ __forceinline__ bool pair_circular(T x, T y, T vx, T vy, T r_tot, con
       truncate(sx, sy, T(kFSocMax));
     }
   }
-  if (h < T(0)) {
+  if (contact_possible && h < T(0)) {
     T nx = T(0), ny = T(0);
     if (d != T(0)) {
       nx = ddiv(x, d);
@@ -447,6 +468,26 @@ __device__ __forceinline__ bool pair_three_circle(const TCView &A, const TCView 
   const double rB[3] = {B.r_t, B.r_s, B.r_s};
   // distance_three_circles; the parts chosen by the minimum are recorded as
   // they are selected (no dynamic indexing of the part arrays).
+  // The minimum's part pair needs the exact distances (the library hypot)
+  // only of the part pairs that can be the minimum: sqrt(x^2 + y^2) is
+  // within 4 ulp of the hypot, so a part pair whose approximate h exceeds
+  // the smallest approximate h by more than 2^-44 (d + 1) on both sides is
+  // not the minimum and skips its hypot.  A NaN coordinate makes every part
+  // pair a candidate (the reference's comparisons then decide).
+  double happ[9], hlo = __builtin_inf();
+  bool any_nan = false;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const double x = A.px[i] - B.px[j], y = A.py[i] - B.py[j];
+      const double d0 = sqrt(x * x + y * y);
+      const double ha = d0 - (rA[i] + rB[j]);
+      happ[3 * i + j] = ha - 0x1p-44 * (d0 + 1.0);
+      any_nan = any_nan || __builtin_isnan(ha);
+      hlo = fmin(hlo, ha + 0x1p-44 * (d0 + 1.0));
+    }
+  }
   double h_min = __builtin_nan("");
   double minx = 0.0, miny = 0.0, mind = 0.0;  // x, y and d of the minimum
   double selx = 0.0, sely = 0.0, selr = 0.0;  // A.part[i_min], rA[i_min]
@@ -455,6 +496,7 @@ __device__ __forceinline__ bool pair_three_circle(const TCView &A, const TCView 
   for (int i = 0; i < 3; ++i) {
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
+      if (!any_nan && happ[3 * i + j] > hlo) continue;
       double x = A.px[i] - B.px[j], y = A.py[i] - B.py[j];
       double d = hypot_ref(x, y);
       double h = d - (rA[i] + rB[j]);

@@ -14,7 +14,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .agents import (INT_FIELDS, MUTABLE_FIELDS, REAL_FIELDS, SCALAR_FIELDS,
+from .agents import (BOOL_FIELDS, INT_FIELDS, MUTABLE_FIELDS, REAL_FIELDS, SCALAR_FIELDS,
                      VECTOR_FIELDS, model_of)
 from .exceptions import DeviceError, InvalidValue
 
@@ -60,6 +60,9 @@ class DeviceAgents:
                 self.fields[f] = torch.zeros(self.n, dtype=torch.float64, device=self.device)
         for f in INT_FIELDS:
             self.fields[f] = torch.zeros(self.n, dtype=torch.int64, device=self.device)
+        for f in BOOL_FIELDS:
+            if f in names:
+                self.fields[f] = torch.zeros(self.n, dtype=torch.uint8, device=self.device)
         self.upload(array)
         self.struct = _lib.CDAgents()
         s = self.struct
@@ -80,13 +83,19 @@ class DeviceAgents:
             if fields is not None and f not in fields:
                 continue
             src = np.array(array[f], order='C', copy=True)
+            if src.dtype == np.bool_:
+                src = src.astype(np.uint8)
             t.copy_(torch.from_numpy(src).to(t.dtype))
 
-    def download(self, array, fields=MUTABLE_FIELDS):
+    def download(self, array, fields=MUTABLE_FIELDS + ('active', 'target_reached')):
         """Copy the (mutated) fields back into the host record array."""
         for f in fields:
             if f in self.fields and f in array.dtype.names:
-                array[f] = self.fields[f].detach().to('cpu', torch.float64).numpy()
+                t = self.fields[f].detach()
+                if array.dtype[f] == np.bool_:
+                    array[f] = t.to('cpu').numpy().astype(bool)
+                else:
+                    array[f] = t.to('cpu', torch.float64).numpy()
 
     def __getitem__(self, name):
         return self.fields[name]

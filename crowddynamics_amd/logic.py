@@ -152,24 +152,54 @@ class Orientation(LogicNode):
 
 
 class InsideDomain(LogicNode):
-    """Sets `active` from a point-in-polygon test (logic.py:351-357).  No
-    per-step operator reads `active`; the flag is evaluated on the host copy
-    when the records are read."""
+    """Sets agents not inside the domain inactive (logic.py:343-358): on the
+    device, active = position inside the domain polygon (the crossings test
+    of Path.contains_points), and data['inactive'] counts the changes (read
+    from the device when accessed)."""
+
+    def __init__(self, simulation, name=None, **traits):
+        super().__init__(simulation, name, **traits)
+        self._counter = None
+        simulation.data['inactive'] = 0
 
     def update(self):
-        pass
-
-    def evaluate(self, array):
         dom = self.simulation.field.domain
         if dom is None:
             return
-        x, y = array['position'][:, 0], array['position'][:, 1]
-        inside = np.zeros(len(array), dtype=bool)
-        xv, yv = dom[:, 0], dom[:, 1]
-        j = len(dom) - 1
-        for i in range(len(dom)):
-            cond = ((yv[i] > y) != (yv[j] > y)) & \
-                   (x < (xv[j] - xv[i]) * (y - yv[i]) / (yv[j] - yv[i] + 1e-300) + xv[i])
-            inside ^= cond
-            j = i
-        array['active'] = inside
+        import torch
+        from . import state
+        agents = self.device_agents
+        if self._counter is None:
+            self._counter = torch.zeros(1, dtype=torch.int64, device=agents.device)
+            counter = self._counter
+            self.simulation.data.bind_value('inactive', lambda: int(counter.item()))
+        state.inside_domain(agents, dom, self._counter)
+
+
+class TargetReached(LogicNode):
+    """Detects agents that reached the polygon targets (logic.py:361-387):
+    for each polygon target i of field.target_polygons, reached_by |= inside
+    and data['target_<i>'] = the number of agents that reached it (read from
+    the device when accessed).  The reference passes the whole record array
+    to contains_points (logic.py:385); the positions are meant and used."""
+    prefix = 'target_{index}'
+
+    def __init__(self, simulation, name=None, **traits):
+        super().__init__(simulation, name, **traits)
+        self._reached = {}
+        for i in simulation.field.target_polygons:
+            simulation.data[self.prefix.format(index=i)] = 0
+
+    def update(self):
+        import torch
+        from . import state
+        agents = self.device_agents
+        for i, poly in self.simulation.field.target_polygons.items():
+            if i not in self._reached:
+                reached = torch.zeros(agents.n, dtype=torch.uint8, device=agents.device)
+                count = torch.zeros(1, dtype=torch.int64, device=agents.device)
+                self._reached[i] = (reached, count)
+                self.simulation.data.bind_value(self.prefix.format(index=i),
+                                                lambda c=count: int(c.item()))
+            reached, count = self._reached[i]
+            state.target_reached(agents, poly, reached, count)

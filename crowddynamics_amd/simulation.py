@@ -181,6 +181,8 @@ class Field:
         self.obstacles = obstacles
         self.targets = list(range(targets)) if isinstance(targets, int) else list(targets)
         self.domain = None if domain is None else np.asarray(domain, dtype=np.float64)
+        #: polygon targets (vertex arrays [k, 2]) for TargetReached, by target index
+        self.target_polygons = {}
         self.navigation = navigation
         self.nav_origin = tuple(nav_origin)
         self.nav_step = float(nav_step)
@@ -217,16 +219,25 @@ class SimulationData(OrderedDict):
         super().__init__(*a, **kw)
         self._source = None
         self._time_offset = 0.0
+        self._device_values = {}
 
     def bind(self, source):
         """source() -> cd_state of the workspace that integrates."""
         self._source = source
+
+    def bind_value(self, key, fn):
+        """data[key] is read (synchronously) from the device by fn()."""
+        self._device_values[key] = fn
+        if key not in self:
+            super().__setitem__(key, 0)
 
     def __getitem__(self, key):
         if key in ('dt', 'time_tot') and self._source is not None:
             st = self._source()
             if st is not None:
                 return float(st.dt) if key == 'dt' else self._time_offset + float(st.time_tot)
+        if key in self._device_values:
+            return self._device_values[key]()
         return super().__getitem__(key)
 
 
@@ -256,14 +267,12 @@ class MultiAgentSimulation:
         from . import logic as L
         order = list(post_order(self.logic.root))
         plan = dict(order=order, fused_flags=None)
-        for n in order:
-            if isinstance(n, L.InsideDomain) and n.evaluate not in self.agents.on_download:
-                self.agents.on_download.append(n.evaluate)
         if self.fused is False:
             return plan
         fusable = [L.Fluctuation, L.Navigation, L.Orientation, L.Adjusting,
                    L.AgentAgentInteractions, L.AgentObstacleInteractions, L.Integrator, L.Reset]
-        seq = [n for n in order if not isinstance(n, L.InsideDomain)]
+        state = (L.InsideDomain, L.TargetReached)
+        seq = [n for n in order if not isinstance(n, state)]
         kinds = [type(n) for n in seq]
         if any(k not in fusable for k in kinds) or len(set(kinds)) != len(kinds):
             return plan
@@ -284,7 +293,8 @@ class MultiAgentSimulation:
         flags |= _lib.CD_USE_RESET if L.Reset in has else 0
         plan['fused_flags'] = flags
         plan['nodes'] = {type(n): n for n in seq}
-        plan['inside'] = [n for n in order if isinstance(n, L.InsideDomain)]
+        # state nodes read positions only: after the fused step (post-order)
+        plan['inside'] = [n for n in order if isinstance(n, state)]
         return plan
 
     def _fused_stepper(self):
@@ -316,9 +326,9 @@ class MultiAgentSimulation:
                 self.data.bind(lambda: self._stepper.ws.state())
             elif uploaded:
                 self._stepper.prepare(self.data['iterations'])
+            self._stepper.step(1)
             for n in self._plan['inside']:
                 n.update()
-            self._stepper.step(1)
         else:
             for node in self._plan['order']:
                 node.update()

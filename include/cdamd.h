@@ -387,6 +387,84 @@ int cd_kernel_timed(const cd_agents_t *agents, const cd_step_params_t *params,
                     const cd_workspace_t *ws, int32_t kernel, int64_t reps, double *avg_ms,
                     cd_stream_t stream);
 
+/* ---- navigation field construction --------------------------------------- */
+
+/* Bytes of device scratch cd_distance_map needs for a rows x cols node grid
+ * (negative for invalid sizes). */
+int64_t cd_field_work_bytes(int64_t rows, int64_t cols);
+
+/* distance_map (crowddynamics/core/steering/quickest_path.py:54-115), i.e.
+ * scikit-fmm 0.0.9's distance(phi, dx) on the node grid: phi [rows][cols]
+ * fp64 (row = y, col = x; > 0 target region, < 0 elsewhere), mask
+ * [rows][cols] uint8 (non-zero: excluded node, may be NULL), spacing dx.
+ * out [rows][cols]: the signed distance to the zero contour (negative where
+ * phi < 0), NaN at excluded and unreachable nodes; nodes next to the
+ * contour by scikit-fmm's linear interpolation, the rest by the first-order
+ * upwind Eikonal update.  Synchronous (the relaxation passes read a
+ * convergence flag); work is device scratch of cd_field_work_bytes bytes.
+ * CD_ERR_HIP also when the passes do not converge within their bound. */
+int cd_distance_map(const double *phi, const uint8_t *mask, int64_t rows, int64_t cols,
+                    double dx, double *out, void *work, int64_t work_bytes,
+                    cd_stream_t stream);
+
+/* direction_map (quickest_path.py:125-149): the normalised np.gradient of
+ * dmap [rows][cols], as ex (x, along columns) and ey (y, along rows); NaN
+ * where the gradient vanishes or a NaN enters it. */
+int cd_direction_map(const double *dmap, int64_t rows, int64_t cols, double *ex, double *ey,
+                     cd_stream_t stream);
+
+/* obstacle_handling (core/steering/obstacle_handling.py:15-74) over n nodes:
+ * where 0 < -dobs < radius, (tx, ty) <- -p (ox, oy) + (1 - p) (tx, ty) with
+ * p = strength^(-dobs / radius); then every node is normalised.  dobs and
+ * (ox, oy) are the obstacles' distance and direction maps. */
+int cd_obstacle_handling(const double *dobs, const double *ox, const double *oy, double *tx,
+                         double *ty, int64_t n, double radius, double strength,
+                         cd_stream_t stream);
+
+/* fill_missing (quickest_path.py:152-166): (ex, ey) = (sx, sy), except that
+ * nodes with fill[k] != 0 whose direction is NaN take the direction of the
+ * nearest node (index distance, within `reach` nodes; ties to the smaller
+ * row, then column) whose direction is not NaN.  ex, ey must not alias sx,
+ * sy. */
+int cd_fill_missing(const double *sx, const double *sy, const uint8_t *fill, int64_t rows,
+                    int64_t cols, int32_t reach, double *ex, double *ey, cd_stream_t stream);
+
+/* ---- agent state nodes and herding ---------------------------------------- */
+
+/* InsideDomain.update (crowddynamics/simulation/logic.py:351-357): active[i]
+ * (uint8, the agents' `active` flags) = position i inside the polygon
+ * vertices [V][2] (fp64, device; closed from the last vertex back to the
+ * first), by the crossings test of matplotlib's Path.contains_points, which
+ * the reference calls; *inactive (a device uint64) += the number of agents
+ * whose flag changed (data['inactive']). */
+int cd_inside_domain(const double *position, int64_t n, const double *vertices,
+                     int64_t n_vertices, uint8_t *active, uint64_t *inactive,
+                     cd_stream_t stream);
+/* TargetReached.update (logic.py:383-387) for one polygon target:
+ * reached_by[i] |= position i inside the polygon; *count (device uint64) =
+ * the number of agents with reached_by set (data['target_<i>']). */
+int cd_target_reached(const double *position, int64_t n, const double *vertices,
+                      int64_t n_vertices, uint8_t *reached_by, uint64_t *count,
+                      cd_stream_t stream);
+/* find_nearest_neighbors (core/steering/collective_motion.py:69-107, with
+ * the block list at cell size = sight, :250-257): neighbors [n][k] (int64,
+ * device) = the k (1..16) nearest agents strictly within sight whose line of
+ * sight crosses none of the segments ([W][4], core/geom2D.py:38-59
+ * line_intersect), nearest first (ties to the smaller index), -1 where
+ * missing.  Bins the agents into the workspace (cd_add_to_cells). */
+int cd_nearest_neighbors(const cd_agents_t *agents, double sight, int32_t k,
+                         const double *segments, int64_t n_segments, int64_t *neighbors,
+                         const cd_workspace_t *ws, cd_stream_t stream);
+/* herding_interaction (collective_motion.py:110-154): for agents with
+ * is_follower[i] set, the normalised weighted average of the direction to
+ * the mean position and of the mean velocity of the neighbours heading away
+ * (herding_relationship, :26-58, with cos_phi = cos(phi)); direction [n][2]
+ * (zero otherwise) and has_direction [n] (uint8). */
+int cd_herding_interaction(const double *position, const double *velocity, int64_t n,
+                           const uint8_t *is_follower, const int64_t *neighbors, int32_t k,
+                           double weight_position, double cos_phi, double *direction,
+                           uint8_t *has_direction, cd_stream_t stream);
+
 /* ---- HIP graphs (synchronous helpers) ------------------------------------- */
 
 int cd_graph_begin(cd_stream_t stream);

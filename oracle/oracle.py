@@ -51,6 +51,19 @@ Parity pinning of each oracle part (what checks it besides the GPU):
                            bilinear potentials (gradient mode)
   fluctuation              Philox4x32-10 against its published known-answer
                            vectors; sample moments of the truncated normal
+  distance_map (cd_field.c) the exact distances of plane fronts (the
+                           hallway's straight targets), the closed-form
+                           distance to the exits of a convex room within the
+                           first-order scheme's error, the fixed point of the
+                           upwind update checked by plain relaxation in numpy
+  direction_map            np.gradient itself
+  obstacle_handling        a straight wall with a target front along it,
+                           where every map is a plane wave and the blended
+                           direction has a closed form
+  line_intersect,          crossing and parallel segment pairs of known answer;
+  nearest neighbours       the k nearest by brute force over all pairs
+  herding_interaction      one neighbour heading away (closed form) and the
+                           heading-away cone by construction
 """
 from __future__ import annotations
 
@@ -395,3 +408,121 @@ def run(arr, scenario, steps, first_step=0, threads=None):
 
 def max_threads():
     return lib().orc_max_threads()
+
+
+# ---------------------------------------------------------------- fields --
+
+def distance_map(phi, dx, mask=None):
+    """scikit-fmm's distance(phi, dx) restated (cd_field.c): signed distance
+    to the zero contour of phi [rows, cols]; NaN at masked and unreachable
+    nodes."""
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    rows, cols = phi.shape
+    out = np.empty_like(phi)
+    m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+    rc = lib().orc_distance_map(_ptr(phi), None if m is None else m.ctypes.data_as(C.c_void_p),
+                                C.c_int64(rows), C.c_int64(cols), C.c_double(dx), _ptr(out))
+    if rc != 0:
+        raise ValueError('distance_map: no node borders the contour')
+    return out
+
+
+def direction_map(dmap):
+    """(ex, ey): the normalised np.gradient of dmap (cd_field.c)."""
+    d = np.ascontiguousarray(dmap, dtype=np.float64)
+    ex = np.empty_like(d)
+    ey = np.empty_like(d)
+    lib().orc_direction_map(_ptr(d), C.c_int64(d.shape[0]), C.c_int64(d.shape[1]), _ptr(ex),
+                            _ptr(ey))
+    return ex, ey
+
+
+def obstacle_handling(dobs, ox, oy, tx, ty, radius, strength):
+    """Blended and normalised directions (cd_field.c), new arrays."""
+    tx = np.array(tx, dtype=np.float64, order='C')
+    ty = np.array(ty, dtype=np.float64, order='C')
+    args = [np.ascontiguousarray(a, dtype=np.float64) for a in (dobs, ox, oy)]
+    lib().orc_obstacle_handling(*[_ptr(a) for a in args], _ptr(tx), _ptr(ty),
+                                C.c_int64(tx.size), C.c_double(radius), C.c_double(strength))
+    return tx, ty
+
+
+# ------------------------------------------------------- collective motion --
+
+def line_intersect(x0, x1, y0, y1):
+    """line_intersect, crowddynamics/core/geom2D.py:38-59."""
+    ux, uy = x1[0] - x0[0], x1[1] - x0[1]
+    vx, vy = y1[0] - y0[0], y1[1] - y0[1]
+    bx, by = y0[0] - x0[0], y0[1] - x0[1]
+    d = ux * vy - uy * vx
+    if d == 0:
+        return False
+    t0 = bx * vy - by * vx
+    t1 = bx * uy - by * ux
+    return 0 <= t0 / d <= 1 and 0 <= t1 / d <= 1
+
+
+def nearest_neighbors(position, sight, k, segments):
+    """The set computed by find_nearest_neighbors
+    (core/steering/collective_motion.py:69-107), by brute force over all
+    pairs: the k nearest agents strictly within sight whose line of sight
+    crosses no segment ([W, 4]), nearest first, ties to the smaller index,
+    -1 where missing."""
+    n = len(position)
+    out = np.full((n, k), -1, dtype=np.int64)
+    segs = np.asarray(segments, dtype=np.float64).reshape(-1, 4)
+    for i in range(n):
+        cand = []
+        for j in range(n):
+            if j == i:
+                continue
+            l = float(np.hypot(position[i, 0] - position[j, 0], position[i, 1] - position[j, 1]))
+            if not l < sight:
+                continue
+            if any(line_intersect(position[i], position[j], s[0:2], s[2:4]) for s in segs):
+                continue
+            cand.append((l, j))
+        cand.sort()
+        for m, (_, j) in enumerate(cand[:k]):
+            out[i, m] = j
+    return out
+
+
+def _normalize(v):
+    l = float(np.hypot(v[0], v[1]))
+    return v / l if l else v
+
+
+def herding_interaction(is_herding, position, velocity, neighbors, weight_position, phi):
+    """herding_interaction (collective_motion.py:110-154) with
+    herding_relationship (:26-58), over the neighbour lists in their order."""
+    n = len(position)
+    new = np.zeros_like(position)
+    has = np.zeros(n, dtype=bool)
+    cos_phi = np.cos(phi)
+    for i in range(n):
+        if not is_herding[i]:
+            continue
+        mp = np.zeros(2)
+        mv = np.zeros(2)
+        num = 0
+        for j in neighbors[i]:
+            if j < 0:
+                continue
+            v1, v2 = velocity[i], velocity[j]
+            if np.hypot(*v1) == 0 or np.hypot(*v2) == 0:
+                continue
+            e = _normalize(position[j] - position[i])
+            n2 = _normalize(v2)
+            cj = -(e[0] * n2[0] + e[1] * n2[1])
+            if cos_phi < cj < 1.0:
+                continue
+            mp += position[j]
+            mv += velocity[j]
+            num += 1
+        if num > 0:
+            e0 = _normalize(mp / num - position[i])
+            e1 = _normalize(mv)
+            new[i] = _normalize(weight_position * e0 + (1 - weight_position) * e1)
+            has[i] = True
+    return new, has

@@ -62,15 +62,6 @@ def test_agents_type_check():
         Agents(np.zeros(2, dtype=[('x', 'f8')]).dtype)
 
 
-def test_inside_domain_flag():
-    sim = E.Hallway(size=20)
-    node = sim.logic['InsideDomain']
-    arr = sim.agents._array.copy()
-    arr['position'][0] = (-1.0, 2.0)
-    node.evaluate(arr)
-    assert not arr['active'][0] and arr['active'][1:].all()
-
-
 def test_agent_agent_cell_size_default():
     sim = MultiAgentSimulation(agents=Agents('circular', empty_agents(2)))
     assert L.AgentAgentInteractions(sim).cell_size == 3.6     # logic.py:114-116
