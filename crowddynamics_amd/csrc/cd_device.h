@@ -73,6 +73,12 @@ __device__ __forceinline__ double hypot_ref(double x, double y) {
   return hypot_kernel(ax, ay);
 }
 
+// The same, out of line: the pair functions reach it only on rare paths,
+// and a call keeps its registers out of their hot loops.
+__device__ __attribute__((noinline)) double hypot_exact(double x, double y) {
+  return hypot_ref(x, y);
+}
+
 __device__ __forceinline__ double dhypot(double a, double b) { return hypot_ref(a, b); }
 __device__ __forceinline__ float dhypot(float a, float b) { return __builtin_amdgcn_sqrtf(a * a + b * b); }
 __device__ __forceinline__ double dsqrt(double a) { return sqrt(a); }
@@ -149,9 +155,19 @@ __device__ __forceinline__ T magnitude(T tau, T tau_0) {
   return ddiv((ddiv(T(2), tau) + ddiv(T(1), tau_0)) * dexp(ddiv(-tau, tau_0)), tau * tau);
 }
 
-// truncate, core/vector2D.py:167-187
+// Relative margin of the squared-length gates: x^2 + y^2 evaluated in
+// double is within 2^-52 of the exact value, so a gate 2^-40 away from a
+// threshold decides the same way as the exact length would.
+constexpr double kGate = 0x1p-40;
+
+// truncate, core/vector2D.py:167-187.  In double, a vector whose squared
+// length is below l^2 (1 - 2^-40) has hypot(x, y) < l and stays as it is;
+// only the others take the exact length.
 template <typename T>
 __device__ __forceinline__ void truncate(T &x, T &y, T l) {
+  if constexpr (std::is_same<T, double>::value) {
+    if (x * x + y * y < (l * l) * (1.0 - kGate)) return;
+  }
   T vlen = dhypot(x, y);
   if (vlen > l) {
     T s = ddiv(l, vlen);
@@ -280,14 +296,16 @@ struct Owner {
 // evaluated with i and j swapped bit for bit (every term is odd in (x, v)
 // and IEEE negation is exact), so owner-computes reproduces both sides.
 //
-// In fp64 the exact distance d (the library hypot) matters only near the
-// sight limit and in contact: h = d - r_tot enters the result only through
-// the tests h < 3 and h < 0 and, in contact, the contact force.  s = x^2 +
-// y^2 is within 2^-52 of d^2 and the hypot within 2^-52 of d, so s below
-// (3 + r_tot)^2 (1 - 2^-40) and above r_tot^2 (1 + 2^-40) decides both tests
-// as d would (h in (0, 3)): there the pair skips d; s above (3 + r_tot)^2
-// (1 + 2^-40) rejects it.
-constexpr double kGate = 0x1p-40;
+// In fp64 the bits of the distance d (the library hypot) matter only where
+// they decide a branch: h = d - r_tot enters the result through the tests
+// h < 3 and h < 0 and, in contact, the contact force, which is continuous
+// in d.  s = x^2 + y^2 is within 2^-52 of d^2 and the hypot within 2^-52 of
+// d, so s below (3 + r_tot)^2 (1 - 2^-40) and above r_tot^2 (1 + 2^-40)
+// decides both tests as d would (h in (0, 3)) and the pair skips d; s above
+// (3 + r_tot)^2 (1 + 2^-40) rejects it; s below r_tot^2 (1 - 2^-40) is surely
+// in contact, and d = sqrt(s) (correctly rounded, within one ulp of the
+// hypot) feeds the contact force.  Only the pairs within 2^-40 of a
+// threshold take the library hypot.
 
 template <typename T>
 __device__ __forceinline__ bool pair_circular(T x, T y, T vx, T vy, T r_tot, const Owner<T> &o,
@@ -295,11 +313,16 @@ __device__ __forceinline__ bool pair_circular(T x, T y, T vx, T vy, T r_tot, con
   T d = T(0), h = T(0);
   bool contact_possible = true;
   if constexpr (std::is_same<T, double>::value) {
-    const double s = x * x + y * y, rs = kSightSoc + r_tot;
+    const double s = x * x + y * y, rs = kSightSoc + r_tot, r2 = r_tot * r_tot;
     if (s > (rs * rs) * (1.0 + kGate)) return false;
-    contact_possible = !(s < (rs * rs) * (1.0 - kGate) && s > (r_tot * r_tot) * (1.0 + kGate));
-  }
-  if (contact_possible) {
+    const bool clear = s < (rs * rs) * (1.0 - kGate) && s > r2 * (1.0 + kGate);
+    contact_possible = !clear;
+    if (contact_possible) {
+      d = (s < r2 * (1.0 - kGate)) ? sqrt(s) : hypot_exact(x, y);
+      h = d - r_tot;
+      if (!(h < kSightSoc)) return false;
+    }
+  } else {
     d = dhypot(x, y);
     h = d - r_tot;
     if (!(h < T(kSightSoc))) return false;
@@ -469,25 +492,13 @@ __device__ __forceinline__ bool pair_three_circle(const TCView &A, const TCView 
   // distance_three_circles; the parts chosen by the minimum are recorded as
   // they are selected (no dynamic indexing of the part arrays).
   // The minimum's part pair needs the exact distances (the library hypot)
-  // only of the part pairs that can be the minimum: sqrt(x^2 + y^2) is
-  // within 4 ulp of the hypot, so a part pair whose approximate h exceeds
-  // the smallest approximate h by more than 2^-44 (d + 1) on both sides is
-  // not the minimum and skips its hypot.  A NaN coordinate makes every part
-  // pair a candidate (the reference's comparisons then decide).
-  double happ[9], hlo = __builtin_inf();
-  bool any_nan = false;
-#pragma unroll
-  for (int i = 0; i < 3; ++i) {
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      const double x = A.px[i] - B.px[j], y = A.py[i] - B.py[j];
-      const double d0 = sqrt(x * x + y * y);
-      const double ha = d0 - (rA[i] + rB[j]);
-      happ[3 * i + j] = ha - 0x1p-44 * (d0 + 1.0);
-      any_nan = any_nan || __builtin_isnan(ha);
-      hlo = fmin(hlo, ha + 0x1p-44 * (d0 + 1.0));
-    }
-  }
+  // only of the part pairs that can still become the minimum.  The
+  // hardware square root of x^2 + y^2 is within 2^-24 of the distance
+  // (measured: at most 1.7e8 ulp), so a part pair whose approximate h
+  // exceeds the exact minimum so far by more than 2^-20 (d + 1) has a
+  // larger exact h and is not selected (the selection takes a strictly
+  // smaller h); it skips its hypot.  The first part pair, and any NaN,
+  // take the exact path.
   double h_min = __builtin_nan("");
   double minx = 0.0, miny = 0.0, mind = 0.0;  // x, y and d of the minimum
   double selx = 0.0, sely = 0.0, selr = 0.0;  // A.part[i_min], rA[i_min]
@@ -496,8 +507,11 @@ __device__ __forceinline__ bool pair_three_circle(const TCView &A, const TCView 
   for (int i = 0; i < 3; ++i) {
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      if (!any_nan && happ[3 * i + j] > hlo) continue;
       double x = A.px[i] - B.px[j], y = A.py[i] - B.py[j];
+      {
+        const double d0 = __builtin_amdgcn_sqrt(x * x + y * y);
+        if (d0 - (rA[i] + rB[j]) - 0x1p-20 * (d0 + 1.0) > h_min) continue;
+      }
       double d = hypot_ref(x, y);
       double h = d - (rA[i] + rB[j]);
       if (h < h_min || __builtin_isnan(h_min)) {
