@@ -32,17 +32,29 @@ def neutral_cell_sizes(r_max, factors=(1.0, 1.15, 1.35, 1.6), sight=SIGHT_SOC):
     return [float(np.ceil(base * f * 1000.0) / 1000.0) for f in factors]
 
 
-def tune_ensemble(scenarios, precision='f32', cell_sizes=None,
-                  kernels=((0, 0, 4), (0, 0, 8), (0, 0, 4, 10), (0, 0, 8, 10), (1, 1, 4),
-                           (2, 2, 4)), steps=10, warmup=3, flags=None):
+# Default force-kernel candidates.  The LDS-tiled kernel (tile != (0, 0))
+# is not among them: it stages a region's neighbour records through LDS
+# with a block-wide scan, a binary search per staged record and barriers
+# between the phases, and the pair arithmetic it then runs is the same as
+# the agent-parallel kernel's, which already finds its candidates' records
+# in L1 (96 % hits in cell order) and is bound by that arithmetic.  Timed on
+# the 256-scenario ensemble it took 2.3 to 6.3 ms per step against 0.75 ms,
+# so it only costs tuning time; it stays selectable by passing a tile.
+ENSEMBLE_KERNELS = ((0, 0, 4), (0, 0, 8), (0, 0, 4, 10), (0, 0, 8, 10))
+ROOM_KERNELS = ((0, 0, 4), (0, 0, 8), (0, 0, 16))
+
+
+def tune_ensemble(scenarios, precision='f32', cell_sizes=None, kernels=ENSEMBLE_KERNELS,
+                  steps=10, warmup=3, flags=None):
     """Time every (cell size, force kernel) pair over `steps` batched steps
     (after `warmup` steps) and return the fastest.
 
     A force kernel is (tile_x, tile_y, lanes[, order_every]): tile (0, 0)
     selects the agent-parallel kernel with `lanes` lanes per agent, serving
     the agents in index order, or in cell order rebuilt every `order_every`
-    steps; otherwise LDS tiles of tile_x * tile_y cells.  The choice never
-    changes results (pair and wall terms are summed exactly).  Returns
+    steps; otherwise LDS tiles of tile_x * tile_y cells (not a default
+    candidate, see ENSEMBLE_KERNELS).  The choice never changes results (pair
+    and wall terms are summed exactly).  Returns
     dict(cell_size, tile, lanes, order_every, ms_per_step, table) where table
     lists (ms_per_step, cell_size, kernel, per-kernel ms)."""
     if cell_sizes is None:
@@ -64,8 +76,7 @@ def tune_ensemble(scenarios, precision='f32', cell_sizes=None,
                 table=table)
 
 
-def tune_kernel(make_stepper, kernels=((0, 0, 4), (0, 0, 8), (0, 0, 16), (8, 8, 4),
-                                       (16, 16, 4)), steps=20, warmup=5):
+def tune_kernel(make_stepper, kernels=ROOM_KERNELS, steps=20, warmup=5):
     """Fastest force kernel (tile_x, tile_y, lanes) for one simulation:
     make_stepper(tile, lanes) builds a Stepper on a fresh copy of the state.
     Returns (kernel, table)."""

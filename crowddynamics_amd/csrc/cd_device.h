@@ -178,10 +178,12 @@ __device__ __forceinline__ void truncate(T &x, T &y, T l) {
 
 // Exact, order-independent sum of doubles: a 128-bit two's-complement
 // fixed-point number with 64 fractional bits (range +-2^63, resolution
-// 2^-64).  Every term is converted exactly when its magnitude is at least
-// 2^-11 (and truncated to the 2^-64 grid below that); additions are exact
-// integer additions, so the sum does not depend on the order of the terms.
-// Terms that are not finite or of magnitude 2^62 or more make the sum NaN.
+// 2^-64).  Every double term is converted exactly when its magnitude is at
+// least 2^-11 (and truncated to the 2^-64 grid below that); a float term
+// below 2^20 is rounded to the 2^-31 grid.  Each term thus maps to one
+// fixed-point value of its own, and additions are exact integer additions,
+// so the sum does not depend on the order of the terms.  Terms that are not
+// finite or of magnitude 2^62 or more make the sum NaN.
 struct Fix128 {
   long long hi;
   unsigned long long lo;
@@ -203,18 +205,25 @@ struct Fix128 {
     const double t1 = __builtin_trunc(g1);
     add_mag((unsigned)t2, (unsigned)t1, (unsigned)((g1 - t1) * 0x1p32), x < 0.0);
   }
-  // A float term: the same value as add((double)x), by the same steps in
-  // float arithmetic.
+  // A float term (the fp32 mode's pair and wall terms) below 2^20: rounded
+  // to the nearest point of the 2^-31 grid, ties to even, by one double
+  // addition.  x + 1.5 2^21 lies in [2^21, 2^22), where doubles are spaced
+  // 2^-31, so its bits minus those of 1.5 2^21 are the grid index
+  // t = round(x 2^31) in two's complement (|t| < 2^51); t 2^33 joins the
+  // sum.  The grid is far below the float term's own rounding (2^-24 of
+  // its magnitude) and costs a third of the exact conversion.  Larger
+  // terms, inf and NaN take add_bits.
   __host__ __device__ __forceinline__ void add(float x) {
-    const float a = __builtin_fabsf(x);
-    if (!(a < 0x1p32f)) {
+    if (!(__builtin_fabsf(x) < 0x1p20f)) {
       add_bits((double)x);
       return;
     }
-    const float t2 = __builtin_truncf(a);
-    const float g1 = (a - t2) * 0x1p32f;
-    const float t1 = __builtin_truncf(g1);
-    add_mag((unsigned)t2, (unsigned)t1, (unsigned)((g1 - t1) * 0x1p32f), x < 0.0f);
+    const double y = (double)x + 0x1.8p21;
+    const long long t =
+        __builtin_bit_cast(long long, y) - __builtin_bit_cast(long long, 0x1.8p21);
+    const unsigned long long l = (unsigned long long)t << 33;
+    lo += l;
+    hi += (t >> 31) + (lo < l ? 1 : 0);
   }
   // The bit-level conversion, valid for every double: the mantissa shifted
   // into place (|x| = m 2^(e - 1075) for biased exponent e, so |x| 2^64 =

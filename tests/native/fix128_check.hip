@@ -1,8 +1,9 @@
 // Host check of the exact accumulator of the force kernels (Fix128 in
 // crowddynamics_amd/csrc/cd_device.h): exact round trips, independence of
-// the order of the terms, accuracy of sums, NaN for out-of-range terms, and
-// the floating-point conversions (double and float terms below 2^32) adding
-// exactly what the bit-level conversion adds.
+// the order of the terms, accuracy of sums, NaN for out-of-range terms, the
+// floating-point conversion of double terms below 2^32 adding exactly what
+// the bit-level conversion adds, and float terms below 2^20 adding their
+// value rounded to the 2^-31 grid.
 // Prints "ok" and exits 0 on success.
 #include <algorithm>
 #include <cmath>
@@ -83,30 +84,62 @@ int main() {
     if (std::fabs((long double)a.value() - exact) > 1e-15L * scale + 1e-18L && fails++ < 5)
       std::printf("sum error %Lg\n", (long double)a.value() - exact);
   }
-  // a float term (the fp32 mode's path) adds exactly what the same value as
-  // a double adds: random bit patterns over every float exponent, both
-  // signs, subnormals, the 2^32 switch to the double path, inf and NaN
+  // a float term (the fp32 mode's path) below 2^20 adds round(x 2^31) 2^-31,
+  // ties to even (x 2^31 is exact in double, nearbyint rounds it in the
+  // default mode); larger terms, inf and NaN add what add_bits adds: random
+  // bit patterns over every float exponent, both signs, subnormals, the
+  // switch at 2^20, ties, inf and NaN
+  auto float_ref = [](float x, long long &h, unsigned long long &l, bool &bad) {
+    if (!(std::fabs(x) < 0x1p20f)) {
+      cd::Fix128 r;
+      r.add_bits((double)x);
+      h = r.hi;
+      l = r.lo;
+      bad = r.bad;
+      return;
+    }
+    const long long t = (long long)std::nearbyint((double)x * 0x1p31);
+    h = t >> 31;
+    l = (unsigned long long)t << 33;
+    bad = false;
+  };
   for (int k = 0; k < 2000000; ++k) {
     float x;
     if (k < 64) {
-      const float special[] = {0.0f, -0.0f, 1.0f, -1.0f, 0x1p32f, -0x1p32f, 0x1.fffffep31f,
-                               -0x1.fffffep31f, 0x1p-64f, -0x1p-65f, 0x1p-149f, -0x1p-149f,
-                               0x1.fffffep127f, INFINITY, -INFINITY, NAN};
+      const float special[] = {0.0f, -0.0f, 1.0f, -1.0f, 0x1p20f, -0x1p20f, 0x1.fffffep19f,
+                               -0x1.fffffep19f, 0x1p-32f, -0x1p-32f, 0x3p-32f, -0x3p-32f,
+                               0x1p-149f, 0x1.fffffep127f, INFINITY, NAN};
       x = special[k % 16];
     } else {
       const unsigned bits = (unsigned)g();
       std::memcpy(&x, &bits, sizeof(x));
     }
-    cd::Fix128 a, b;
+    cd::Fix128 a;
     a.add(x);
-    b.add_bits((double)x);
-    if ((a.hi != b.hi || a.lo != b.lo || a.bad != b.bad) && fails++ < 5)
+    long long h;
+    unsigned long long l;
+    bool bad;
+    float_ref(x, h, l, bad);
+    if ((a.hi != h || a.lo != l || a.bad != bad) && fails++ < 5)
       std::printf("float term %a: %lld %llu %d vs %lld %llu %d\n", (double)x, a.hi, a.lo,
-                  (int)a.bad, b.hi, b.lo, (int)b.bad);
-    a.add(-x * 0.75f);  // and as the second term of a sum
-    b.add_bits((double)(-x * 0.75f));
-    if ((a.hi != b.hi || a.lo != b.lo || a.bad != b.bad) && fails++ < 5)
-      std::printf("float sum %a\n", (double)x);
+                  (int)a.bad, h, l, (int)bad);
+  }
+  // float sums: independent of the order of the terms, and within half a
+  // grid step per term of the exact sum
+  for (int trial = 0; trial < 20000; ++trial) {
+    const int n = 1 + (int)(g() % 60);
+    std::vector<float> xs(n);
+    for (float &x : xs) x = (float)(u(g) * std::pow(10.0, (int)(g() % 10) - 5));
+    cd::Fix128 a, b;
+    for (float x : xs) a.add(x);
+    std::shuffle(xs.begin(), xs.end(), g);
+    for (float x : xs) b.add(x);
+    if ((a.hi != b.hi || a.lo != b.lo) && fails++ < 5) std::printf("float order dependence\n");
+    long double exact = 0.0L;
+    for (float x : xs) exact += x;
+    if (std::fabs((long double)a.value() - exact) > n * 0x1p-32L + 1e-16L * std::fabs(exact) &&
+        fails++ < 5)
+      std::printf("float sum error %Lg\n", (long double)a.value() - exact);
   }
   cd::Fix128 c;
   c.add(-0.5);
