@@ -27,6 +27,7 @@ NOALLOC_SCRATCH_BYTES = 64 << 20
 CD_STATUS_GRID_OVERFLOW = 1
 CD_STATUS_NONFINITE = 2
 CD_STATUS_SLOTS_FULL = 4
+CD_STATUS_NAV_KEPT = 8
 
 CD_USE_FLUCT = 1
 CD_USE_NAV = 2
@@ -72,11 +73,12 @@ class CDScenario(C.Structure):
 
 class CDAdjoint(C.Structure):
     _fields_ = [('lam_x', _vp), ('lam_v', _vp), ('lam_g', _vp), ('lam_f', _vp),
-                ('lam_vt', _vp), ('grad', _vp)]
+                ('lam_vt', _vp), ('grad', _vp), ('x_next', _vp), ('v_next', _vp)]
 
 
 class CDTangent(C.Structure):
-    _fields_ = [('dx', _vp), ('dv', _vp), ('dg', _vp), ('df', _vp), ('dp', _vp)]
+    _fields_ = [('dx', _vp), ('dv', _vp), ('dg', _vp), ('df', _vp), ('dp', _vp),
+                ('x_next', _vp), ('v_next', _vp)]
 
 
 class CDState(C.Structure):

@@ -2,12 +2,13 @@
 ``torch.autograd.Function`` whose backward pass is the hand-written adjoint
 (cd_adjoint_step), with timestep checkpointing.
 
-Scope: circular agents in fp64 at a fixed dt (dt_min = dt_max), with the
-Reset, Integrator, Adjusting, AgentAgentInteractions and (parameter-free)
-Fluctuation nodes of the standard tree
-(crowddynamics/examples/simulations.py:123-136 without walls and
-navigation).  Differentiable inputs: tau_0, mu, kappa, damping (one value
-for all agents, or one per agent) and the per-agent target_velocity.
+Scope: circular agents in fp64 with every node of the standard tree
+(crowddynamics/examples/simulations.py:123-136): Reset, Integrator at a
+fixed or adaptive dt, Adjusting, Navigation (the direction looked up again
+at each step's positions), AgentAgentInteractions,
+AgentObstacleInteractions and the (parameter-free) Fluctuation.
+Differentiable inputs: tau_0, mu, kappa, damping (one value for all agents,
+or one per agent) and the per-agent target_velocity.
 
 The forward pass stores the state (position, velocity, force_prev) every
 ``checkpoint`` steps; the backward pass re-runs each segment from its
@@ -34,17 +35,30 @@ class Rollout:
 
     def __init__(self, array, steps, dt=0.005, cell_size=3.6, checkpoint=25, seed=1,
                  fluctuation=False, cell_capacity=None, bounds=None, device=None, lanes=0,
-                 order_every=0):
+                 order_every=0, segments=None, nav=None, dt_min=None, dt_max=None):
+        """dt: a fixed step; dt_min < dt_max instead select the adaptive
+        step (core/integrator.py:29-97).  segments ([W, 4]) add the wall
+        contacts, nav (as Stepper) the navigation lookup."""
         if 'position_ls' in array.dtype.names:
             raise InvalidValue('the adjoint covers circular agents')
         if steps < 1 or checkpoint < 1:
             raise InvalidValue('steps and checkpoint must be positive')
         flags = ROLLOUT_NODES | (_lib.CD_USE_FLUCT if fluctuation else 0)
+        if segments is not None and len(segments):
+            flags |= _lib.CD_USE_OBSTACLE
+        if nav is not None:
+            flags |= _lib.CD_USE_NAV
         self.array0 = array.copy()
         self.steps = int(steps)
         self.checkpoint = int(checkpoint)
-        self.dt = float(dt)
-        self.st = Stepper(array.copy(), cell_size=cell_size, dt_min=dt, dt_max=dt, seed=seed,
+        self.dt_min = float(dt if dt_min is None else dt_min)
+        self.dt_max = float(dt if dt_max is None else dt_max)
+        if not 0.0 < self.dt_min <= self.dt_max:
+            raise InvalidValue('need 0 < dt_min <= dt_max')
+        self.dt = self.dt_max
+        self.adaptive = self.dt_min < self.dt_max
+        self.st = Stepper(array.copy(), segments=segments, nav=nav, cell_size=cell_size,
+                          dt_min=self.dt_min, dt_max=self.dt_max, seed=seed,
                           flags=flags, precision='f64', cell_capacity=cell_capacity,
                           bounds=bounds, device=device, lanes=lanes, order_every=order_every)
         self.agents = self.st.agents
@@ -161,22 +175,38 @@ class Rollout:
         L = _lib.lib()
         s_h = stream_handle()
         par = C.byref(self.st.params)
+        # the tangent step of k -> k + 1 runs after the fused step, from a
+        # copy of state k, with the agents' new state as state k + 1
+        xk = torch.empty((n, 2), dtype=f64, device=dev)
+        vk = torch.empty((n, 2), dtype=f64, device=dev)
+        view = _lib.CDAgents.from_buffer_copy(self.agents.struct)
+        view.position = C.c_void_p(xk.data_ptr())
+        view.velocity = C.c_void_p(vk.data_ptr())
+        tan.x_next = self.agents.struct.position
+        tan.v_next = self.agents.struct.velocity
+        f = self.agents.fields
         for k in range(self.steps):
-            _lib.check(L.cd_tangent_step(self.agents.ref(), par, C.byref(tan),
-                                         self.tan_ws.ref(), s_h), 'tangent_step')
+            xk.copy_(f['position'])
+            vk.copy_(f['velocity'])
             _lib.check(L.cd_step(self.agents.ref(), par, self.st.ws.ref(), s_h), 'step')
+            _lib.check(L.cd_tangent_step(C.byref(view), par, C.byref(tan),
+                                         self.tan_ws.ref(), s_h), 'tangent_step')
             dX[k + 1] = T['dx']
         self.st.check()
-        if self.tan_ws.state().status & ~_lib.CD_STATUS_SLOTS_FULL:
+        if self.tan_ws.state().status & ~(_lib.CD_STATUS_SLOTS_FULL | _lib.CD_STATUS_NAV_KEPT):
             raise InvalidValue('tangent binning: status %d' % self.tan_ws.state().status)
         return dX
 
-    def adjoint_step(self, grad_x=None):
+    def adjoint_step(self, grad_x=None, x_next=None, v_next=None):
         """One cd_adjoint_step from the agents' current position and
-        velocity, updating the adjoint buffers ``adj`` and ``grad`` in place."""
+        velocity, updating the adjoint buffers ``adj`` and ``grad`` in place
+        (x_next, v_next: the state after the step, for adaptive dt)."""
         gx = C.c_void_p(grad_x.data_ptr()) if grad_x is not None else C.c_void_p(0)
+        a = _lib.CDAdjoint.from_buffer_copy(self.adj_struct)
+        a.x_next = C.c_void_p(x_next.data_ptr()) if x_next is not None else None
+        a.v_next = C.c_void_p(v_next.data_ptr()) if v_next is not None else None
         _lib.check(_lib.lib().cd_adjoint_step(self.agents.ref(), C.byref(self.st.params), gx,
-                                              C.byref(self.adj_struct), self.st.ws.ref(),
+                                              C.byref(a), self.st.ws.ref(),
                                               stream_handle()), 'adjoint_step')
 
 

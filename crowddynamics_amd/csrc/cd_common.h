@@ -242,6 +242,22 @@ struct Work {
   int order_every;
 };
 
+// Scalars of an adjoint or tangent step with adaptive dt, in the workspace's
+// state region after DevState: the step's dt and the maxima it was formed
+// from (core/integrator.py:29-97), whether dt lies between its clamps (its
+// derivative is then not zero), the lowest agent index attaining each
+// maximum, the exact sum of dL/d(dt) over the agents (adjoint, as a 128-bit
+// fixed-point pair) and the tangent of dt (tangent step).
+struct AdjScal {
+  double dt, vmax, v0max, ddt;
+  int inside, arg_v, arg_v0, bad;
+  unsigned long long ldt_lo;
+  long long ldt_hi;
+};
+constexpr int kAdjScalOffset = 64;
+static_assert(sizeof(DevState) <= kAdjScalOffset && kAdjScalOffset + sizeof(AdjScal) <= 256,
+              "the adjoint scalars share the 256-byte state region");
+
 // Scenario view of a launch: the table (null for a single scenario, which is
 // then `one`), the per-scenario states and the block maps (null: block b of a
 // single scenario is its b-th block).
@@ -289,6 +305,10 @@ Work carve(const cd_workspace_t *w) {
   W.K = (int)w->cell_slots;
   W.ntiles = (int)((w->cell_capacity + kScanTile - 1) / kScanTile);
   return W;
+}
+
+__host__ __device__ __forceinline__ AdjScal *adj_scal(const Work &W) {
+  return (AdjScal *)((char *)W.st + kAdjScalOffset);
 }
 
 inline int single_parts(int64_t n) {
@@ -820,6 +840,119 @@ int check_step(const cd_agents_t *a, const cd_step_params_t *p, const cd_workspa
                         p->wall_nx * p->wall_ny > ((int64_t)1 << 26) || !(p->wall_cell > 0.0)))
     return CD_ERR_ARG;
   return check_ws(a, w);
+}
+
+// ---------------------------------------------------------------------------
+// Shared by the step kernels and the adjoint: the navigation lookup and the
+// fused step's arguments.
+// ---------------------------------------------------------------------------
+
+// Navigation lookup in two halves: nav_fetch checks the cell and issues the
+// map loads, nav_finish turns them into the direction (returns false when
+// e0 is kept), so a kernel can put independent work between the two.
+struct NavFetch {
+  bool ok;
+  int mode;
+  double fx, fy, step;
+  double p00, p01, p10, p11;  // potential (mode 1) or the direction in p00, p01 (mode 0)
+};
+
+__device__ __forceinline__ NavFetch nav_fetch(double px, double py, int64_t t, const double *maps,
+                                              int64_t T_, int64_t rows, int64_t cols,
+                                              double minx, double miny, double step, int mode) {
+  NavFetch n;
+  n.ok = false;
+  n.mode = mode;
+  n.step = step;
+  n.fx = n.fy = n.p00 = n.p01 = n.p10 = n.p11 = 0.0;
+  if (t < 0 || t >= T_) return n;
+  const double qx = (px - minx) / step;
+  const double qy = (py - miny) / step;
+  if (mode == 0) {
+    // indicer truncation + is_inside (quickest_path.py:41-44, navigation.py:59-78)
+    if (!(qx > -1.0 && qx < (double)cols && qy > -1.0 && qy < (double)rows)) return n;
+    const int64_t jj = (int64_t)qx, ii = (int64_t)qy;
+    const double2 m = ((const double2 *)maps)[(t * rows + ii) * cols + jj];
+    n.p00 = m.x;
+    n.p01 = m.y;
+    n.ok = true;
+    return n;
+  }
+  if (!(qx >= 0.0 && qx <= (double)(cols - 1) && qy >= 0.0 && qy <= (double)(rows - 1))) return n;
+  int64_t c0 = (int64_t)qx, r0 = (int64_t)qy;
+  if (c0 > cols - 2) c0 = cols - 2;
+  if (r0 > rows - 2) r0 = rows - 2;
+  if (c0 < 0 || r0 < 0) return n;
+  n.fx = qx - (double)c0;
+  n.fy = qy - (double)r0;
+  const double *P = maps + t * rows * cols;
+  n.p00 = P[r0 * cols + c0];
+  n.p01 = P[r0 * cols + c0 + 1];
+  n.p10 = P[(r0 + 1) * cols + c0];
+  n.p11 = P[(r0 + 1) * cols + c0 + 1];
+  n.ok = true;
+  return n;
+}
+
+__device__ __forceinline__ bool nav_finish(const NavFetch &n, double &ex, double &ey) {
+  if (!n.ok) return false;
+  if (n.mode == 0) {
+    ex = n.p00;
+    ey = n.p01;
+    return true;
+  }
+  const double gx = ((n.p01 - n.p00) * (1.0 - n.fy) + (n.p11 - n.p10) * n.fy) / n.step;
+  const double gy = ((n.p10 - n.p00) * (1.0 - n.fx) + (n.p11 - n.p01) * n.fx) / n.step;
+  const double g = hypot_ref(gx, gy);
+  if (!(g > 0.0)) return false;
+  ex = -gx / g;
+  ey = -gy / g;
+  return true;
+}
+
+__device__ __forceinline__ bool nav_lookup(double px, double py, int64_t t, const double *maps,
+                                           int64_t T_, int64_t rows, int64_t cols, double minx,
+                                           double miny, double step, int mode, double &ex,
+                                           double &ey) {
+  return nav_finish(nav_fetch(px, py, t, maps, T_, rows, cols, minx, miny, step, mode), ex, ey);
+}
+
+struct FusedArgs {
+  int flags, nav_mode;
+  const double *maps;
+  int64_t nav_T, nav_rows, nav_cols;
+  double nav_minx, nav_miny, nav_step;
+  const double4 *segs;
+  double2 *tape_x;  // when set, the step also writes its new positions here ([n])
+  void *tape_v;     // ... and its new velocities
+  // wall grid (cd_wall_grid_build) or null: offsets of the nx * ny cells,
+  // then the segment indices
+  const int *wall_cells;
+  int wall_nx, wall_ny;
+  double wall_minx, wall_miny, wall_cell;
+};
+
+FusedArgs fused_args(const cd_step_params_t *p) {
+  FusedArgs fa;
+  fa.flags = p->flags;
+  fa.nav_mode = p->nav_mode;
+  fa.maps = p->nav_maps;
+  fa.nav_T = p->nav_targets;
+  fa.nav_rows = p->nav_rows;
+  fa.nav_cols = p->nav_cols;
+  fa.nav_minx = p->nav_minx;
+  fa.nav_miny = p->nav_miny;
+  fa.nav_step = p->nav_step;
+  fa.segs = (const double4 *)p->segments;
+  fa.tape_x = nullptr;
+  fa.tape_v = nullptr;
+  fa.wall_cells = p->wall_cells;
+  fa.wall_nx = (int)p->wall_nx;
+  fa.wall_ny = (int)p->wall_ny;
+  fa.wall_minx = p->wall_minx;
+  fa.wall_miny = p->wall_miny;
+  fa.wall_cell = p->wall_cell;
+  return fa;
 }
 
 }  // namespace

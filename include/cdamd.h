@@ -42,7 +42,10 @@ enum { CD_F64 = 0, CD_F32 = 1 };
 enum {
   CD_STATUS_GRID_OVERFLOW = 1, /* grid larger than cell_capacity (or stale box): pairs skipped */
   CD_STATUS_NONFINITE = 2,     /* a non-finite position reached the binning */
-  CD_STATUS_SLOTS_FULL = 4     /* a cell held more than cell_slots agents: slow exact path used */
+  CD_STATUS_SLOTS_FULL = 4,    /* a cell held more than cell_slots agents: slow exact path used */
+  CD_STATUS_NAV_KEPT = 8       /* adjoint or tangent step with navigation: an agent's lookup kept
+                                  its previous e0 (outside the map), which was then taken from the
+                                  agents' e0 and held constant */
 };
 
 /* Device structure-of-arrays view of agent_type_circular /
@@ -308,21 +311,30 @@ int cd_batch_step(const cd_agents_t *agents, const cd_step_params_t *params,
 int cd_batch_read_states(const cd_workspace_t *ws, cd_state_t *out, int32_t count,
                          cd_stream_t stream);
 
-/* Adjoint of one fused step (circular agents, fp64, fixed dt = dt_min =
- * dt_max, flags within FLUCT | ADJUST | AGENT_AGENT | RESET with RESET set,
- * single-scenario workspace).  `agents` holds the state x_k, v_k the step
- * started from (position / velocity may point into a caller's tape).
- * Maps the adjoint (lam_x, lam_v, lam_g) of step k+1 to that of step k in
- * place, adds grad_x (dL/dx_k, [n][2], may be null) to lam_x, and adds the
- * step's parameter gradients to grad ([n][5]: tau_0, mu, kappa, damping,
- * target_velocity of each agent).  Stream-ordered, no allocation.
- * Replaces reverse-mode differentiation through interactions.py:51-70,
- * adjusting.py:16-51 and integrator.py:164-176, which the reference does
- * not provide. */
+/* Adjoint of one fused step (circular agents, fp64, every node of the
+ * standard tree with RESET set: fluctuation, navigation, adjusting, agent
+ * and wall interactions, fixed or adaptive dt; single-scenario workspace).
+ * `agents` holds the state x_k, v_k the step started from (position /
+ * velocity may point into a caller's tape).  Maps the adjoint (lam_x, lam_v,
+ * lam_g) of step k+1 to that of step k in place, adds grad_x (dL/dx_k,
+ * [n][2], may be null) to lam_x, and adds the step's parameter gradients to
+ * grad ([n][5]: tau_0, mu, kappa, damping, target_velocity of each agent).
+ * With navigation, e0 is looked up again at x_k (its derivative in the
+ * bilinear mode; zero at the nearest node); an agent whose lookup keeps its
+ * e0 sets CD_STATUS_NAV_KEPT.  With adaptive dt (dt_min < dt_max), adj->
+ * x_next / v_next give state k + 1, and dt's derivative reaches the agent of
+ * the largest speed and the one of the largest target_velocity while dt lies
+ * between its clamps.  Stream-ordered, no allocation.  Replaces
+ * reverse-mode differentiation through interactions.py:51-70 and 107-141,
+ * adjusting.py:16-51, navigation.py:59-78 and integrator.py:29-97 and
+ * 164-176, which the reference does not provide. */
 typedef struct cd_adjoint {
   double *lam_x, *lam_v, *lam_g; /* [n][2] adjoint state, in/out */
   double *lam_f, *lam_vt;        /* [n][2] scratch of the step */
   double *grad;                  /* [n][5] accumulated parameter gradients */
+  /* [n][2] position and velocity of step k + 1: needed with adaptive dt
+   * (dt_min < dt_max), whose derivative enters through them; else unused */
+  const double *x_next, *v_next;
 } cd_adjoint_t;
 int cd_adjoint_step(const cd_agents_t *agents, const cd_step_params_t *params,
                     const double *grad_x, const cd_adjoint_t *adj, const cd_workspace_t *ws,
@@ -341,6 +353,7 @@ typedef struct cd_tangent {
   double *dx, *dv, *dg; /* [n][2] tangent state, in/out */
   double *df;           /* [n][2] out: tangent of the step's force */
   const double *dp;     /* [n][5] parameter tangent */
+  const double *x_next, *v_next; /* [n][2] state k + 1, as in cd_adjoint_t */
 } cd_tangent_t;
 int cd_tangent_step(const cd_agents_t *agents, const cd_step_params_t *params,
                     const cd_tangent_t *tan, const cd_workspace_t *ws, cd_stream_t stream);
@@ -352,7 +365,8 @@ int cd_tangent_step(const cd_agents_t *agents, const cd_step_params_t *params,
  * state (position, velocity, force_prev) to ck_* [k / checkpoint][n][2].
  * cd_adjoint_segment: restores checkpoint state ck_*[k0] (the caller passes
  * the pointers of that checkpoint), re-runs steps k0 .. k1 - 2 into
- * tape_x / tape_v [k1 - k0][n][2] (bit-identical to the recorded run), then
+ * tape_x / tape_v [k1 - k0][n][2] (bit-identical to the recorded run; with
+ * adaptive dt also step k1 - 1, so the tapes hold k1 - k0 + 1 states), then
  * applies cd_adjoint_step for k = k1 - 1 down to k0 with grad_x + k * n * 2
  * (dL/dx_k of the whole rollout, may be null).  Stream-ordered; no reference
  * counterpart (reverse-mode differentiation of the update). */
