@@ -333,9 +333,9 @@ def bench_corridor(steps, stream):
     flags = ALL_NODES | (_lib.CD_USE_ORIENT_VELOCITY if scn.extra.get('orient') == 'velocity'
                          else 0)
 
-    def stepper(lanes):
+    def stepper(lanes, precision='f64'):
         return Stepper(scn.agents.copy(), segments=scn.segments, cell_size=scn.cell_size,
-                       dt_min=scn.dt_min, dt_max=scn.dt_max, seed=scn.seed, precision='f64',
+                       dt_min=scn.dt_min, dt_max=scn.dt_max, seed=scn.seed, precision=precision,
                        bounds=scn.bounds, lanes=lanes, flags=flags)
 
     with torch.cuda.stream(stream):
@@ -366,7 +366,21 @@ def bench_corridor(steps, stream):
         st.check()
         tally1 = pair_tally_three_circle(st.download(scn.agents.copy()), scn.cell_size,
                                          scn.segments)
-        del graph
+        del graph, st
+        # the same workload in the fp32 mode (float velocity, force and pair
+        # arithmetic; fp64 positions, orientation and torque), graph-replayed
+        st = stepper(lanes, 'f32')
+        st.step(10)
+        graph = st.capture(G)
+        graph.launch()
+        stream.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            graph.launch()
+        stream.synchronize()
+        el32 = time.perf_counter() - t0
+        st.check()
+        del graph, st
     v = scn.n * reps * G / el
     log('corridor: %.3e agent-steps/s, kernels %s' % (v, {k: round(x, 4) for k, x in kms.items()}))
     flops = 0.5 * (tally0['flops'] + tally1['flops'])
@@ -381,6 +395,7 @@ def bench_corridor(steps, stream):
     roof = _force_roofline(scn.n, kms['forces'], fb, flops, cand)
     return dict(agents=scn.n, steps=reps * G, value=v, unit='agent-steps/s',
                 ms_per_step=1e3 * el / (reps * G), precision='f64', lanes=lanes,
+                fp32_value=scn.n * reps * G / el32, fp32_ms_per_step=1e3 * el32 / (reps * G),
                 orientation='angle(velocity)' if flags & _lib.CD_USE_ORIENT_VELOCITY
                 else 'angle(target_direction)',
                 lane_ms=lane_ms, kernel_ms=kms,

@@ -754,7 +754,6 @@ int check_agents(const cd_agents_t *a) {
   if (a->model != CD_CIRCULAR && a->model != CD_THREE_CIRCLE) return CD_ERR_TYPE;
   if (a->real != CD_F64 && a->real != CD_F32) return CD_ERR_ARG;
   if (a->n < 0 || a->n > (int64_t)0x1fffffff) return CD_ERR_ARG;
-  if (a->model == CD_THREE_CIRCLE && a->real != CD_F64) return CD_ERR_UNSUPPORTED;
   if (a->n == 0) return CD_OK;
   if (!a->position || !a->velocity || !a->target_direction || !a->force || !a->force_prev ||
       !a->radius || !a->mass || !a->target_velocity || !a->tau_adj || !a->k_soc || !a->tau_0 ||

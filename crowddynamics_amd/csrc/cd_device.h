@@ -634,6 +634,138 @@ __device__ __forceinline__ bool pair_three_circle(const TCView &A, const TCView 
   return true;
 }
 
+// fp32 mode of the same pair (velocities and forces in float; positions,
+// the moment arm and the torque in double): the part distances from the
+// double differences rounded to float, the hardware square root, and the
+// divisions by a, ds, tau, tau_0 and d as products with one shared
+// reciprocal each, as in pair_circular<float>.  The selections keep the
+// reference's rules and quirks (the first minimum of h, strict <;
+// x0[j_min] for side B's moment arm; the first valid tau kept unless a
+// later one is in (0, tau)); near ties the float distances may pick
+// another part pair than the double ones, which the fp32 mode's error
+// bound covers.
+__device__ __forceinline__ bool pair_three_circle_f32(const TCView &A, const TCView &B, bool side_b,
+                                                      const Owner<float> &o, float &fx, float &fy,
+                                                      double &torque) {
+  const float rA[3] = {(float)A.r_t, (float)A.r_s, (float)A.r_s};
+  const float rB[3] = {(float)B.r_t, (float)B.r_s, (float)B.r_s};
+  float h_min = __builtin_nanf("");
+  float minx = 0.0f, miny = 0.0f, mind = 0.0f;
+  double selx = 0.0, sely = 0.0, bugx = 0.0, bugy = 0.0;
+  float selr = 0.0f, bugr = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const float x = (float)(A.px[i] - B.px[j]), y = (float)(A.py[i] - B.py[j]);
+      const float d = __builtin_amdgcn_sqrtf(x * x + y * y);
+      const float h = d - (rA[i] + rB[j]);
+      if (h < h_min || __builtin_isnan(h_min)) {
+        h_min = h;
+        minx = x;
+        miny = y;
+        mind = d;
+        selx = A.px[i];
+        sely = A.py[i];
+        selr = rA[i];
+        bugx = A.px[j];
+        bugy = A.py[j];
+        bugr = rB[j];
+      }
+    }
+  }
+  float nx = 0.0f, ny = 0.0f;
+  if (mind != 0.0f) {
+    const float rd = frcp_normal(mind);
+    nx = minx * rd;
+    ny = miny * rd;
+  }
+  if (!(h_min < float(kSightSoc))) return false;
+  double rmx, rmy;
+  if (!side_b) {
+    rmx = (selx + (double)(selr * nx)) - A.px[0];
+    rmy = (sely + (double)(selr * ny)) - A.py[0];
+  } else {
+    rmx = (bugx - (double)(bugr * nx)) - B.px[0];
+    rmy = (bugy - (double)(bugr * ny)) - B.py[0];
+  }
+  float sx = 0.0f, sy = 0.0f;
+  const float vx = (float)(A.vx - B.vx), vy = (float)(A.vy - B.vy);
+  const float a = vx * vx + vy * vy;
+  if (a != 0.0f) {
+    const float ra = frcp(a);
+    int ci = 0, cj = 0;
+    float tau = __builtin_nanf(""), b_min = 0.0f, d_min = 0.0f;
+#pragma unroll
+    for (int pi = 0; pi < 3; ++pi) {
+#pragma unroll
+      for (int pj = 0; pj < 3; ++pj) {
+        const float x = (float)(A.px[pi] - B.px[pj]), y = (float)(A.py[pi] - B.py[pj]);
+        const float r_tot = rA[pi] + rB[pj];
+        const float b = -(x * vx + y * vy);
+        const float c = (x * x + y * y) - r_tot * r_tot;
+        const float d = __builtin_amdgcn_sqrtf(b * b - a * c);
+        if (__builtin_isnan(d) || d == 0.0f) continue;
+        const float tau_new = (b - d) * ra;
+        if (__builtin_isnan(tau) || (0.0f < tau_new && tau_new < tau)) {
+          ci = pi;
+          cj = pj;
+          tau = tau_new;
+          b_min = b;
+          d_min = d;
+        }
+      }
+    }
+    if (!(__builtin_isnan(tau) || tau <= 0.0f)) {
+      float roix = 0.0f, roiy = 0.0f, rojx = 0.0f, rojy = 0.0f;
+      if (ci == 1) {
+        roix = (float)A.ox;
+        roiy = (float)A.oy;
+      } else if (ci == 2) {
+        roix = -(float)A.ox;
+        roiy = -(float)A.oy;
+      }
+      if (cj == 1) {
+        rojx = (float)B.ox;
+        rojy = (float)B.oy;
+      } else if (cj == 2) {
+        rojx = -(float)B.ox;
+        rojy = -(float)B.oy;
+      }
+      const float x = (float)(A.px[0] - B.px[0]), y = (float)(A.py[0] - B.py[0]);
+      const float rox = roix - rojx, roy = roiy - rojy;
+      const float rds = frcp_normal(d_min);
+      const float gx = (vx - (a * (x + 2.0f * rox) + b_min * vx) * rds) * ra;
+      const float gy = (vy - (a * (y + 2.0f * roy) + b_min * vy) * rds) * ra;
+      const float rt = frcp_normal(tau), rt0 = frcp_normal(o.tau_0);
+      const float m = ((2.0f * rt + rt0) * __expf(-tau * rt0)) * (rt * rt);
+      if (!side_b) {
+        sx = (o.neg_mk * gx) * m;
+        sy = (o.neg_mk * gy) * m;
+      } else {
+        sx = -((o.neg_mk * gx) * m);
+        sy = -((o.neg_mk * gy) * m);
+      }
+      truncate(sx, sy, float(kFSocMax));
+    }
+  }
+  if (h_min < 0.0f) {
+    float cx, cy;
+    force_contact(h_min, nx, ny, vx, vy, ny, -nx, o.mu, o.kappa, o.damping, cx, cy);
+    if (!side_b) {
+      sx += cx;
+      sy += cy;
+    } else {
+      sx -= cx;
+      sy -= cy;
+    }
+  }
+  fx = sx;
+  fy = sy;
+  torque = rmx * (double)sy - rmy * (double)sx;  // cross(r_moment, f), core/vector2D.py:149
+  return true;
+}
+
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
 __device__ __forceinline__ void philox4x32(uint32_t c[4], uint32_t k0, uint32_t k1) {
 #pragma unroll

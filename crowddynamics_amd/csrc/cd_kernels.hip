@@ -681,6 +681,20 @@ __device__ __forceinline__ void pair_add_tc(int i, int j, const TCView &me, cons
   }
 }
 
+// The same in the fp32 mode (float force terms, double torque).
+__device__ __forceinline__ void pair_add_tc(int i, int j, const TCView &me, const TCView &other,
+                                            const Owner<float> &o, PairSum &ps) {
+  float px, py;
+  double m;
+  const bool hit = (i < j) ? pair_three_circle_f32(me, other, false, o, px, py, m)
+                           : pair_three_circle_f32(other, me, true, o, px, py, m);
+  if (hit) {
+    ps.x.add(px);
+    ps.y.add(py);
+    ps.m.add(m);
+  }
+}
+
 // Own (per-agent) terms of the fused step in the reference's post-order:
 // fluctuation, navigation (e0), orientation (phi0), adjusting; force
 // (torque) += fluctuation + adjusting.  own_load issues the agent's loads,
@@ -1294,7 +1308,7 @@ __device__ __forceinline__ void agent_forces(const FusedArgs &fa, const ScnDesc 
       });
     } else {
       rows.each(q, L, i, [&](int j, const TCView &other) {
-        if constexpr (sizeof(T) == 8) pair_add_tc(i, j, me, other, self.o, ps);
+        pair_add_tc(i, j, me, other, self.o, ps);
       });
     }
   }
@@ -1753,12 +1767,10 @@ int launch_forces(const cd_agents_t *a, const Work &W, const Batch &B, const Fus
                   int reset = 0) {
   const Agents<T> A = typed<T>(a);
   if (a->model == CD_THREE_CIRCLE) {
-    if constexpr (sizeof(T) == 8) {
-      if (fused)
-        launch_forces_m<T, 1, true>(A, W, B, fa, L, w, s, reset);
-      else
-        launch_forces_m<T, 1, false>(A, W, B, fa, L, w, s, reset);
-    }
+    if (fused)
+      launch_forces_m<T, 1, true>(A, W, B, fa, L, w, s, reset);
+    else
+      launch_forces_m<T, 1, false>(A, W, B, fa, L, w, s, reset);
   } else {
     if (fused)
       launch_forces_m<T, 0, true>(A, W, B, fa, L, w, s, reset);
@@ -1792,14 +1804,12 @@ void launch_step_begin(const cd_agents_t *a, const cd_step_params_t *p, const Wo
   const bool one = !B.map_f;
   const dim3 grid_s(one ? 4 * L.blocks_f : L.blocks_f), block_s(one ? 128 : kBS);
   if (a->model == CD_THREE_CIRCLE) {
-    if constexpr (sizeof(T) == 8) {
-      if (split)
-        hipLaunchKernelGGL((k_step_begin<T, 1, true>), grid_s, block_s, 0, s, A, W, B, fa,
-                           p->cell_size, binning);
-      else
-        hipLaunchKernelGGL((k_step_begin<T, 1, false>), dim3(L.blocks_f), dim3(kBS), 0, s, A, W,
-                           B, fa, p->cell_size, binning);
-    }
+    if (split)
+      hipLaunchKernelGGL((k_step_begin<T, 1, true>), grid_s, block_s, 0, s, A, W, B, fa,
+                         p->cell_size, binning);
+    else
+      hipLaunchKernelGGL((k_step_begin<T, 1, false>), dim3(L.blocks_f), dim3(kBS), 0, s, A, W,
+                         B, fa, p->cell_size, binning);
   } else if (split) {
     hipLaunchKernelGGL((k_step_begin<T, 0, true>), grid_s, block_s, 0, s, A, W, B, fa,
                        p->cell_size, binning);

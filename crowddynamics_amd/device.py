@@ -43,8 +43,6 @@ class DeviceAgents:
         if precision not in ('f64', 'f32'):
             raise InvalidValue("precision must be 'f64' or 'f32'")
         self.model = model_of(array)
-        if self.model == 1 and precision == 'f32':
-            raise InvalidValue('three-circle agents run in fp64 only')
         self.precision = precision
         self.real = torch.float64 if precision == 'f64' else torch.float32
         self.device = torch.device(device) if device is not None else torch.device('cuda')

@@ -31,7 +31,7 @@ enum {
   CD_ERR_ARG = -1,         /* null pointer, bad size or bad parameter */
   CD_ERR_TYPE = -2,        /* not a circular / three-circle agent array (InvalidType) */
   CD_ERR_WORKSPACE = -3,   /* workspace smaller than cd_workspace_bytes() or for other sizes */
-  CD_ERR_UNSUPPORTED = -4, /* combination not built (three-circle in fp32) */
+  CD_ERR_UNSUPPORTED = -4, /* combination not built (e.g. the adjoint of three-circle agents) */
   CD_ERR_HIP = -5          /* a HIP runtime call or launch failed */
 };
 

@@ -63,10 +63,14 @@ def test_argument_validation_without_device():
     a = _lib.CDAgents()
     a.n, a.model, a.real = 10, 7, 0
     assert L.cd_force_adjust_agents(ctypes.byref(a), None) == _lib.CD_ERR_TYPE
-    a.model, a.real = 1, 1
-    assert L.cd_force_adjust_agents(ctypes.byref(a), None) == _lib.CD_ERR_UNSUPPORTED
-    a.model, a.real = 0, 0
+    a.model, a.real = 0, 5
     assert L.cd_force_adjust_agents(ctypes.byref(a), None) == _lib.CD_ERR_ARG
+    for model, real in ((0, 0), (0, 1), (1, 0), (1, 1)):  # every model in fp64 and fp32
+        a.model, a.real = model, real
+        assert L.cd_force_adjust_agents(ctypes.byref(a), None) == _lib.CD_ERR_ARG
+    # the adjoint covers circular agents in fp64 only
+    a.model, a.real, a.n = 1, 0, 0
+    assert L.cd_adjoint_step(ctypes.byref(a), None, None, None, None, None) == _lib.CD_ERR_ARG
 
 
 def _fake_circular(n):

@@ -114,7 +114,8 @@ def test_ensemble_graph_replay_and_tiles(cell_size):
         assert st == res[0][1]
 
 
-def test_ensemble_three_circle():
+@pytest.mark.parametrize('precision', ['f64', 'f32'])
+def test_ensemble_three_circle(precision):
     scns = [E.corridor(n=n, seed=k, width=25.0 + 5 * k, height=12.0, n_obstacles=4)
             for k, n in enumerate((120, 40, 260))]
     for k, s in enumerate(scns):
@@ -122,14 +123,14 @@ def test_ensemble_three_circle():
     steps = 25
     stream = torch.cuda.Stream()
     with torch.cuda.stream(stream):
-        ens = Ensemble(scns, precision='f64')
+        ens = Ensemble(scns, precision=precision)
         ens.step(steps)
         ens.check()
         outs = ens.download_scenarios()
         states = ens.states()
     refs = []
     for k, s in enumerate(scns):
-        out, state = _standalone(s, steps, 'f64', ens.cell_size, stream)
+        out, state = _standalone(s, steps, precision, ens.cell_size, stream)
         refs.append(out)
         assert states[k].time_tot == state.time_tot
     _compare(outs, refs, TC_FIELDS)

@@ -495,20 +495,51 @@ def test_step_processing_order(room, corridor_small, hallway):
 def test_step_wide_lane_groups(hallway, corridor_small):
     """2, 32 and 64 lanes per agent (32 agents per wave, one agent per half
     wave or per wave) give the 4-lane step bit for bit: hallway (circular)
-    and corridor (three-circle), 5 fused steps, fp64."""
-    for scn in (hallway, corridor_small):
+    and corridor (three-circle, fp64 and fp32), 5 fused steps."""
+    for scn, prec in ((hallway, 'f64'), (corridor_small, 'f64'), (corridor_small, 'f32')):
         outs = []
         for lanes in (4, 2, 32, 64):
-            say('%s 5 steps, %d lanes per agent' % (scn.name, lanes))
+            say('%s %s 5 steps, %d lanes per agent' % (scn.name, prec, lanes))
             st = Stepper(scn.agents.copy(), segments=scn.segments, nav=scn.nav,
                          cell_size=scn.cell_size, dt_min=scn.dt_min, dt_max=scn.dt_max,
-                         seed=scn.seed, bounds=scn.bounds, lanes=lanes)
+                         seed=scn.seed, bounds=scn.bounds, lanes=lanes, precision=prec)
             st.step(5)
             st.check()
             outs.append(st.download(scn.agents.copy()))
         for out in outs[1:]:
             for f in ('position', 'velocity', 'force_prev'):
-                assert np.array_equal(out[f], outs[0][f]), (scn.name, f)
+                assert np.array_equal(out[f], outs[0][f]), (scn.name, prec, f)
+
+
+def test_step_corridor_fp32(corridor_small):
+    """The fp32 mode of three-circle agents against the fp64 oracle, one
+    step at a time along the GPU trajectory (steps 0, 4 and 8 of the
+    3000-agent corridor): velocity, force and torque in fp32 arithmetic,
+    position, orientation and torque accumulation in fp64."""
+    st = _stepper(corridor_small, precision='f32')
+    done = 0
+    for k in (0, 4, 8):
+        st.step(k - done)
+        done = k + 1
+        state = st.download(corridor_small.agents.copy())
+        ref = state.copy()
+        O.run(ref, _oracle_scenario(corridor_small), 1, first_step=k)
+        st.step(1)
+        out = st.download(corridor_small.agents.copy())
+        for f in ('position', 'velocity', 'force_prev'):
+            scale = np.hypot(*ref[f].T) + 1.0
+            rel = np.hypot(*(out[f] - ref[f]).T) / scale
+            say('  fp32 corridor step %d %s: frac within 1e-4 %.5f, within 1e-6 %.5f, max %.3e'
+                % (k, f, np.mean(rel <= 1e-4), np.mean(rel <= 1e-6), rel.max()))
+            assert np.mean(rel <= 1e-4) >= 0.999
+            assert rel.max() <= (5e-2 if f == 'force_prev' else 1e-3)
+        for f in ('torque_prev', 'angular_velocity', 'orientation'):
+            scale = np.abs(ref[f]) + 1.0
+            rel = np.abs(out[f] - ref[f]) / scale
+            say('  fp32 corridor step %d %s: frac within 1e-4 %.5f, max %.3e'
+                % (k, f, np.mean(rel <= 1e-4), rel.max()))
+            assert np.mean(rel <= 1e-4) >= 0.99
+    st.check()
 
 
 def _pairs_off_stepper(scn, tile=None):
