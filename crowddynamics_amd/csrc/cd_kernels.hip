@@ -1096,10 +1096,14 @@ struct MemRow {
           // agent's own record (centre cell k = 4, its slot) is skipped by
           // its candidate index
           const int sidx = self_slot >= 0 ? pre[4] + self_slot : -1;
-          for (int b = b0; b < b1; b += 4) {
-            int s[4];
+          // batches of kRecBatch candidates whose loads are issued together:
+          // four fp32 records, two fp64 ones (a batch of four fp64 records
+          // holds 40 registers, which made the fp64 kernel spill)
+          constexpr int kRecBatch = sizeof(T) == 8 ? 2 : 4;
+          for (int b = b0; b < b1; b += kRecBatch) {
+            int s[kRecBatch];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < kRecBatch; ++u) {
               const int idx = b + u;
               s[u] = -1;
               if (idx < b1 && idx != sidx) {
@@ -1110,21 +1114,23 @@ struct MemRow {
                 s[u] = o + idx;
               }
             }
-            NbrRec<T> rr[4];
+            NbrRec<T> rr[kRecBatch];
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
+            for (int u = 0; u < kRecBatch; ++u)
               if (s[u] >= 0) rr[u] = rec[s[u]];
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
+            for (int u = 0; u < kRecBatch; ++u)
               if (s[u] >= 0) fn(-1, rr[u].pos, rr[u].vel, rr[u].r);
           }
           return;
         }
       }
-      for (int b = b0; b < b1; b += 4) {
-        int j[4];
+      // batches as for the records: four candidates, two fp64 circular ones
+      constexpr int kIdBatch = (MODEL == 0 && sizeof(T) == 8) ? 2 : 4;
+      for (int b = b0; b < b1; b += kIdBatch) {
+        int j[kIdBatch];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kIdBatch; ++u) {
           const int idx = b + u;
           j[u] = -1;
           if (idx < b1) {
@@ -1137,11 +1143,11 @@ struct MemRow {
           }
         }
         if constexpr (MODEL == 0) {
-          double2 xj[4];
-          V2<T> vj[4];
-          double rj[4];
+          double2 xj[kIdBatch];
+          V2<T> vj[kIdBatch];
+          double rj[kIdBatch];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
+          for (int u = 0; u < kIdBatch; ++u) {
             if (j[u] >= 0) {
               xj[u] = N.pos[j[u]];
               vj[u] = N.vel[j[u]];
@@ -1149,11 +1155,11 @@ struct MemRow {
             }
           }
 #pragma unroll
-          for (int u = 0; u < 4; ++u)
+          for (int u = 0; u < kIdBatch; ++u)
             if (j[u] >= 0 && j[u] != self) fn(j[u], xj[u], vj[u], rj[u]);
         } else {
 #pragma unroll
-          for (int u = 0; u < 4; ++u)
+          for (int u = 0; u < kIdBatch; ++u)
             if (j[u] >= 0) visit(self, j[u], fn);
         }
       }
