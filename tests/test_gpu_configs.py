@@ -98,6 +98,36 @@ def test_corridor50k_three_circle_fp64(corridor50k, orient):
     say('corridor50k: max |dx| after 4 steps %.3e m' % err)
 
 
+def test_corridor50k_three_circle_fp32(corridor50k):
+    """The corridor in the fp32 mode (float velocity, force and pair
+    arithmetic; double position, orientation and torque), one step against
+    the fp64 oracle from the same state: positions within 1e-6 and forces within
+    1e-4 of their scale for at least 99.9 % of the agents, and orientation
+    and torque within 1e-4 for at least 99 %."""
+    scn = corridor50k
+    flags = ALL_NODES | _lib.CD_USE_ORIENT_VELOCITY
+    say('corridor50k fp32: one fused step against the fp64 oracle')
+    ref = scn.agents.copy()
+    dts = O.run(ref, _oracle_scenario(scn, orient_velocity=True), 1)
+    st = Stepper(scn.agents.copy(), segments=scn.segments, cell_size=scn.cell_size,
+                 dt_min=scn.dt_min, dt_max=scn.dt_max, seed=scn.seed, precision='f32',
+                 bounds=scn.bounds, flags=flags)
+    st.step(1)
+    s = st.check()
+    say('  dt gpu %.17g oracle %.17g' % (s.dt, dts[0]))
+    out = st.download(scn.agents.copy())
+    for f in ('position', 'velocity', 'force_prev'):
+        scale = np.hypot(*ref[f].T) + 1.0
+        rel = np.hypot(*(out[f] - ref[f]).T) / scale
+        tol = 1e-4 if f == 'force_prev' else 1e-6
+        say('  %s: frac within %.0e %.5f, max %.3e' % (f, tol, np.mean(rel <= tol), rel.max()))
+        assert np.mean(rel <= tol) >= 0.999, f
+    for f in ('orientation', 'torque_prev', 'angular_velocity'):
+        rel = np.abs(out[f] - ref[f]) / (np.abs(ref[f]) + 1.0)
+        say('  %s: frac within 1e-4 %.5f, max %.3e' % (f, np.mean(rel <= 1e-4), rel.max()))
+        assert np.mean(rel <= 1e-4) >= 0.99, f
+
+
 # ------------------------------------------------------------- ensemble256 --
 
 @pytest.fixture(scope='module')
