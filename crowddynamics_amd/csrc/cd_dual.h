@@ -1,9 +1,12 @@
 // Forward-mode dual numbers for the adjoint: a value and K directional
-// derivatives carried exactly through +, -, *, /, sqrt, exp and hypot.
+// derivatives carried through +, -, *, /, sqrt, exp and hypot.
 // The physics functions of cd_device.h are templates on their scalar type,
 // so evaluating one of them on Dual<K> yields the term and its Jacobian
 // along K input directions with the very branches, cut-offs and
-// truncations of the forward step.  Comparisons act on the value.
+// truncations of the forward step.  Comparisons act on the value, which is
+// computed by the forward's own operations (an IEEE quotient, the library
+// hypot); the derivative parts of a quotient or hypot multiply by one
+// reciprocal of the divisor rather than dividing K times.
 #pragma once
 
 #include "cd_device.h"
@@ -79,8 +82,9 @@ template <int K>
 __device__ __forceinline__ Dual<K> operator/(const Dual<K> &a, const Dual<K> &b) {
   Dual<K> r;
   r.v = a.v / b.v;
+  const double inv = 1.0 / b.v;
 #pragma unroll
-  for (int k = 0; k < K; ++k) r.d[k] = (a.d[k] - r.v * b.d[k]) / b.v;
+  for (int k = 0; k < K; ++k) r.d[k] = (a.d[k] - r.v * b.d[k]) * inv;
   return r;
 }
 
@@ -126,8 +130,9 @@ template <int K>
 __device__ __forceinline__ Dual<K> dhypot(const Dual<K> &a, const Dual<K> &b) {
   Dual<K> r;
   r.v = hypot_ref(a.v, b.v);
+  const double inv = 1.0 / r.v;
 #pragma unroll
-  for (int k = 0; k < K; ++k) r.d[k] = (a.v * a.d[k] + b.v * b.d[k]) / r.v;
+  for (int k = 0; k < K; ++k) r.d[k] = (a.v * a.d[k] + b.v * b.d[k]) * inv;
   return r;
 }
 
