@@ -218,9 +218,16 @@ struct Fix128 {
       add_bits((double)x);
       return;
     }
+    add_grid31(grid31(x));
+  }
+  // round(x 2^31) for |x| < 2^20 (the grid index above)
+  __host__ __device__ static __forceinline__ long long grid31(float x) {
     const double y = (double)x + 0x1.8p21;
-    const long long t =
-        __builtin_bit_cast(long long, y) - __builtin_bit_cast(long long, 0x1.8p21);
+    return __builtin_bit_cast(long long, y) - __builtin_bit_cast(long long, 0x1.8p21);
+  }
+  // Adds t 2^-31 for any 64-bit t: t 2^33 on the 2^-64 grid, whose high
+  // word is t >> 31 (an arithmetic shift) and low word t << 33.
+  __host__ __device__ __forceinline__ void add_grid31(long long t) {
     const unsigned long long l = (unsigned long long)t << 33;
     lo += l;
     hi += (t >> 31) + (lo < l ? 1 : 0);
@@ -376,30 +383,32 @@ __device__ __forceinline__ bool pair_circular(T x, T y, T vx, T vy, T r_tot, con
 // range (|v| below 2^-32 m/s), so only it takes the scaled frcp; ds and d are
 // square roots of nonzero floats (at least 2^-75), tau_0 is a positive
 // parameter, and a tau below 2^-64 makes 1/tau^2 overflow fp32 however the
-// quotient is formed.
+// quotient is formed.  Sums of products are fused multiply-adds written
+// out (one rounding instead of two, the same in every kernel that inlines
+// the pair); each keeps its parity in (x, v), so the pair stays odd.
 template <>
 __device__ __forceinline__ bool pair_circular<float>(float x, float y, float vx, float vy,
                                                      float r_tot, const Owner<float> &o,
                                                      float &fx, float &fy) {
-  const float d2 = x * x + y * y;
+  const float d2 = __builtin_fmaf(x, x, y * y);
   const float d = __builtin_amdgcn_sqrtf(d2);
   const float h = d - r_tot;
   if (!(h < float(kSightSoc))) return false;
   float sx = 0.0f, sy = 0.0f;
-  const float a = vx * vx + vy * vy;
-  const float b = -(x * vx + y * vy);
-  const float c = d2 - r_tot * r_tot;
-  const float ds = __builtin_amdgcn_sqrtf(b * b - a * c);
+  const float a = __builtin_fmaf(vx, vx, vy * vy);
+  const float b = -__builtin_fmaf(x, vx, y * vy);
+  const float c = __builtin_fmaf(-r_tot, r_tot, d2);
+  const float ds = __builtin_amdgcn_sqrtf(__builtin_fmaf(b, b, -(a * c)));
   if (!(is_nan(ds) || ds == 0.0f || a == 0.0f)) {
     const float ra = frcp(a);
     const float tau = (b - ds) * ra;
     if (!(tau <= 0.0f || tau > float(kTauMax))) {
       const float rds = frcp_normal(ds);
-      const float gx = (vx - (vx * b + x * a) * rds) * ra;
-      const float gy = (vy - (vy * b + y * a) * rds) * ra;
+      const float gx = __builtin_fmaf(-__builtin_fmaf(vx, b, x * a), rds, vx) * ra;
+      const float gy = __builtin_fmaf(-__builtin_fmaf(vy, b, y * a), rds, vy) * ra;
       const float rt = frcp_normal(tau), rt0 = frcp_normal(o.tau_0);
       // magnitude(tau, tau_0) = (2/tau + 1/tau_0) exp(-tau/tau_0) / tau^2
-      const float m = ((2.0f * rt + rt0) * __expf(-tau * rt0)) * (rt * rt);
+      const float m = (__builtin_fmaf(2.0f, rt, rt0) * __expf(-tau * rt0)) * (rt * rt);
       sx = (o.neg_mk * gx) * m;
       sy = (o.neg_mk * gy) * m;
       truncate(sx, sy, float(kFSocMax));

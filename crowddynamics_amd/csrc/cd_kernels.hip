@@ -651,9 +651,37 @@ __device__ __forceinline__ TCView tc_lds(const TileLds<T, MODEL> &S, int e) {
   return t;
 }
 
-// Pair sums of one lane.
+// Pair sums of one lane.  A float term pair (the fp32 mode) whose parts
+// are both below 2^20 gathers its grid indices round(x 2^31) (Fix128::add
+// of a float) in two 64-bit integers, folded into the exact sums after
+// kFold such pairs and at the end (fold): an index is below 2^51 in
+// magnitude, so kFold of them cannot overflow.  The sums equal those of
+// adding every term to the Fix128 pair directly.
 struct PairSum {
+  static constexpr int kFold = 2048;
   Fix128 x, y, m;
+  long long gx = 0, gy = 0;
+  int n = 0;
+  __device__ __forceinline__ void add(double px, double py) {
+    x.add(px);
+    y.add(py);
+  }
+  __device__ __forceinline__ void add(float px, float py) {
+    if (__builtin_fabsf(px) < 0x1p20f && __builtin_fabsf(py) < 0x1p20f) {
+      gx += Fix128::grid31(px);
+      gy += Fix128::grid31(py);
+      if (++n == kFold) fold();
+    } else {
+      x.add(px);
+      y.add(py);
+    }
+  }
+  __device__ __forceinline__ void fold() {
+    x.add_grid31(gx);
+    y.add_grid31(gy);
+    gx = gy = 0;
+    n = 0;
+  }
 };
 
 // Circular: agent i's side of its pair with a neighbour at xj, vj, rj.
@@ -663,8 +691,7 @@ __device__ __forceinline__ void pair_add_c(double2 xi, V2<T> vi, double ri, cons
   T px, py;
   if (pair_circular<T>((T)(xi.x - xj.x), (T)(xi.y - xj.y), vi.x - vj.x, vi.y - vj.y,
                        (T)(ri + rj), o, px, py)) {
-    ps.x.add(px);
-    ps.y.add(py);
+    ps.add(px, py);
   }
 }
 
@@ -689,8 +716,7 @@ __device__ __forceinline__ void pair_add_tc(int i, int j, const TCView &me, cons
   const bool hit = (i < j) ? pair_three_circle_f32(me, other, false, o, px, py, m)
                            : pair_three_circle_f32(other, me, true, o, px, py, m);
   if (hit) {
-    ps.x.add(px);
-    ps.y.add(py);
+    ps.add(px, py);
     ps.m.add(m);
   }
 }
@@ -1321,8 +1347,7 @@ __device__ __forceinline__ void agent_forces(const FusedArgs &fa, const ScnDesc 
   const bool walls = FUSED && (fa.flags & CD_USE_OBSTACLE);
   if (walls) {  // wall contacts join the same exact sums
     auto add = [&](T cx, T cy, double mm) {
-      ps.x.add(cx);
-      ps.y.add(cy);
+      ps.add(cx, cy);
       if (MODEL) ps.m.add(mm);
     };
     if (fa.wall_cells) {
@@ -1362,6 +1387,7 @@ __device__ __forceinline__ void agent_forces(const FusedArgs &fa, const ScnDesc 
       }
     }
   }
+  if constexpr (sizeof(T) == 4) ps.fold();
   // exact sums over the L lanes (butterfly: every lane ends with the totals)
   for (int mask = 1; mask < L; mask <<= 1) {
     ps.x.add(shfl_xor(ps.x, mask));
