@@ -162,11 +162,16 @@ constexpr double kGate = 0x1p-40;
 
 // truncate, core/vector2D.py:167-187.  In double, a vector whose squared
 // length is below l^2 (1 - 2^-40) has hypot(x, y) < l and stays as it is;
-// only the others take the exact length.
+// only the others take the exact length.  In float likewise below
+// l^2 (1 - 2^-16): the float length (the hardware square root of the
+// float squared length, within 2^-22 of |(x, y)|) is then below l, so the
+// gate skips the square root without changing a result.
 template <typename T>
 __device__ __forceinline__ void truncate(T &x, T &y, T l) {
   if constexpr (std::is_same<T, double>::value) {
     if (x * x + y * y < (l * l) * (1.0 - kGate)) return;
+  } else if constexpr (std::is_same<T, float>::value) {
+    if (x * x + y * y < (l * l) * (1.0f - 0x1p-16f)) return;
   }
   T vlen = dhypot(x, y);
   if (vlen > l) {
@@ -302,6 +307,7 @@ template <typename T>
 struct Owner {
   T neg_mk;  // (-mass) * k_soc, power_law.py:250
   T tau_0, mu, kappa, damping;
+  T rtau_0;  // fp32 mode: frcp_normal(tau_0), once per agent (else unused)
 };
 
 // interaction_agent_agent_circular (core/interactions.py:51-70) seen from
@@ -406,7 +412,7 @@ __device__ __forceinline__ bool pair_circular<float>(float x, float y, float vx,
       const float rds = frcp_normal(ds);
       const float gx = __builtin_fmaf(-__builtin_fmaf(vx, b, x * a), rds, vx) * ra;
       const float gy = __builtin_fmaf(-__builtin_fmaf(vy, b, y * a), rds, vy) * ra;
-      const float rt = frcp_normal(tau), rt0 = frcp_normal(o.tau_0);
+      const float rt = frcp_normal(tau), rt0 = o.rtau_0;
       // magnitude(tau, tau_0) = (2/tau + 1/tau_0) exp(-tau/tau_0) / tau^2
       const float m = (__builtin_fmaf(2.0f, rt, rt0) * __expf(-tau * rt0)) * (rt * rt);
       sx = (o.neg_mk * gx) * m;
@@ -746,7 +752,7 @@ __device__ __forceinline__ bool pair_three_circle_f32(const TCView &A, const TCV
       const float rds = frcp_normal(d_min);
       const float gx = (vx - (a * (x + 2.0f * rox) + b_min * vx) * rds) * ra;
       const float gy = (vy - (a * (y + 2.0f * roy) + b_min * vy) * rds) * ra;
-      const float rt = frcp_normal(tau), rt0 = frcp_normal(o.tau_0);
+      const float rt = frcp_normal(tau), rt0 = o.rtau_0;
       const float m = ((2.0f * rt + rt0) * __expf(-tau * rt0)) * (rt * rt);
       if (!side_b) {
         sx = (o.neg_mk * gx) * m;

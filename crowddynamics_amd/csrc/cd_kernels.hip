@@ -537,6 +537,10 @@ __device__ __forceinline__ Owner<T> owner_of(const Agents<T> &A, int i) {
   o.mu = (T)A.mu[i];
   o.kappa = (T)A.kappa[i];
   o.damping = (T)A.damping[i];
+  if constexpr (sizeof(T) == 4)
+    o.rtau_0 = frcp_normal(o.tau_0);
+  else
+    o.rtau_0 = T(0);
   return o;
 }
 
