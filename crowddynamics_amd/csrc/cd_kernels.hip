@@ -699,12 +699,31 @@ __device__ __forceinline__ void pair_add_c(double2 xi, V2<T> vi, double ri, cons
   }
 }
 
+// Selects a or b field by field (one inlined pair function for both
+// argument orders below).
+__device__ __forceinline__ TCView tc_select(bool take_b, const TCView &a, const TCView &b) {
+  TCView t;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    t.px[k] = take_b ? b.px[k] : a.px[k];
+    t.py[k] = take_b ? b.py[k] : a.py[k];
+  }
+  t.vx = take_b ? b.vx : a.vx;
+  t.vy = take_b ? b.vy : a.vy;
+  t.r_t = take_b ? b.r_t : a.r_t;
+  t.r_s = take_b ? b.r_s : a.r_s;
+  t.ox = take_b ? b.ox : a.ox;
+  t.oy = take_b ? b.oy : a.oy;
+  return t;
+}
+
 // Three-circle: the reference evaluates the pair as interaction(min, max).
 __device__ __forceinline__ void pair_add_tc(int i, int j, const TCView &me, const TCView &other,
                                             const Owner<double> &o, PairSum &ps) {
   double px, py, m;
-  const bool hit = (i < j) ? pair_three_circle(me, other, false, o, px, py, m)
-                           : pair_three_circle(other, me, true, o, px, py, m);
+  const bool sb = !(i < j);
+  const bool hit = pair_three_circle(tc_select(sb, me, other), tc_select(sb, other, me), sb, o,
+                                     px, py, m);
   if (hit) {
     ps.x.add(px);
     ps.y.add(py);
@@ -717,8 +736,9 @@ __device__ __forceinline__ void pair_add_tc(int i, int j, const TCView &me, cons
                                             const Owner<float> &o, PairSum &ps) {
   float px, py;
   double m;
-  const bool hit = (i < j) ? pair_three_circle_f32(me, other, false, o, px, py, m)
-                           : pair_three_circle_f32(other, me, true, o, px, py, m);
+  const bool sb = !(i < j);
+  const bool hit = pair_three_circle_f32(tc_select(sb, me, other), tc_select(sb, other, me), sb,
+                                         o, px, py, m);
   if (hit) {
     ps.add(px, py);
     ps.m.add(m);
@@ -1188,9 +1208,16 @@ struct MemRow {
           for (int u = 0; u < kIdBatch; ++u)
             if (j[u] >= 0 && j[u] != self) fn(j[u], xj[u], vj[u], rj[u]);
         } else {
+          // one copy of the three-circle pair code for the batch (the id
+          // picked by selects, not by indexing): an unrolled batch held
+          // eight inlined copies, more code than the instruction cache
+#pragma unroll 1
+          for (int u = 0; u < kIdBatch; ++u) {
+            int ju = j[0];
 #pragma unroll
-          for (int u = 0; u < kIdBatch; ++u)
-            if (j[u] >= 0) visit(self, j[u], fn);
+            for (int w = 1; w < kIdBatch; ++w) ju = u == w ? j[w] : ju;
+            if (ju >= 0) visit(self, ju, fn);
+          }
         }
       }
       return;
