@@ -79,7 +79,7 @@ struct ScnState {
 };
 
 constexpr int kBS = 256;              // threads per block of per-agent kernels
-constexpr int kAccSub = 256;          // sub-slots of a scenario's reduction slot
+constexpr int kAccSub = 128;          // sub-slots of a scenario's reduction slot
 constexpr int kMaxPartsPerScn = 128;  // blocks of the partial-producing kernels per scenario
 constexpr int kSingleParts = 512;     // the same for a single (unbatched) scenario
 constexpr int kAgentsPerForceBlock = 64;  // default agents per force block (256 threads)

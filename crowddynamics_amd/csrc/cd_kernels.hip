@@ -1498,7 +1498,10 @@ __global__ void __launch_bounds__(256, MODEL == 0 ? 3 : 1) k_forces_direct(Agent
   ScnState *ss = &B.sst[sk.x];
   const int p = FUSED ? ss->par : 0;
   const bool pairs_on = FUSED ? ((fa.flags & CD_USE_AGENT_AGENT) != 0) : true;
-  const bool ok = pairs_on && ss->valid && !ss->oob[p];
+  // both parities' flags are read with the rest of the state, so that no
+  // load waits for the parity
+  const int oob0 = ss->oob[0], oob1 = ss->oob[1];
+  const bool ok = pairs_on && ss->valid && !(p ? oob1 : oob0);
   if (!FUSED && !ok) return;
   const double dt = FUSED ? ss->dt : 0.0;
   stage_walls(fa, d, sseg, FUSED);
@@ -1519,16 +1522,20 @@ __global__ void __launch_bounds__(256, MODEL == 0 ? 3 : 1) k_forces_direct(Agent
   TCView me;
   Self<T> self;
   Tail<T> tail;
+  // the agent's loads do not wait for the binning's validity: the cell and
+  // slot are read whatever it is and used only when it holds
   if (have) {
     pos = A.pos[i];
     vel = A.vel[i];
-    if (ok) {
-      c = W.cell_of[i];
-      if (MODEL == 0 && FUSED) self_slot = W.rank[i];
-    }
+    c = W.cell_of[i];
+    if (MODEL == 0 && FUSED) self_slot = W.rank[i];
     if constexpr (MODEL == 1) me = tc_agent(A, i);
     self = self_of<T, MODEL>(A, i, pos);
     if (FUSED && lead) tail = tail_of(A, i);
+  }
+  if (!ok) {
+    c = d.c0;
+    self_slot = -1;
   }
   const int nx = ss->nx, ny = ss->ny;
   const int lc = c - d.c0, lx = ny > 0 ? lc / ny : 0, ly = lc - lx * ny;
