@@ -511,6 +511,41 @@ def test_step_wide_lane_groups(hallway, corridor_small):
                 assert np.array_equal(out[f], outs[0][f]), (scn.name, prec, f)
 
 
+@pytest.mark.parametrize('slots', [None, 1024])
+def test_step_fp32_pair_sums_fold(slots):
+    """fp32 pair terms gather in 64-bit integers per lane and join the
+    128-bit sums every 2048 pairs.  5000 overlapping agents in a 3 x 3 m
+    square on 1 m cells: an agent of the centre cell has all 5000 in its
+    stencil and within sight, so at 2 lanes per agent its lanes add about
+    2500 pairs each (one intermediate fold); at 64 lanes about 80 (none).
+    The fused step must agree bit for bit, through the cells' slot records
+    (1024 slots, about 560 agents a cell) and through the overfull-cell path
+    (the default 16 slots)."""
+    import warnings
+    rng = np.random.default_rng(5)
+    n = 5000
+    arr = E._attributes(rng, n)
+    arr['position'] = rng.uniform(0.0, 3.0, (n, 2))
+    arr['velocity'] = rng.normal(0.0, 0.5, (n, 2))
+    arr['target_direction'] = (1.0, 0.0)
+    arr['target'] = 0
+    outs = []
+    for lanes in (2, 64):
+        say('dense crowd fp32, slots %s, %d lanes per agent' % (slots, lanes))
+        st = Stepper(arr.copy(), cell_size=1.0, dt_min=0.001, dt_max=0.01, seed=3,
+                     bounds=(0.0, 0.0, 3.0, 3.0), lanes=lanes, precision='f32',
+                     cell_slots=slots)
+        st.step(1)
+        with warnings.catch_warnings(record=True) as caught:
+            warnings.simplefilter('always')
+            st.check()
+        assert (len(caught) > 0) == (slots is None)  # the overfull-cell path ran
+        outs.append(st.download(arr.copy()))
+    for f in ('position', 'velocity', 'force_prev'):
+        assert np.all(np.isfinite(outs[0][f])), f
+        assert np.array_equal(outs[1][f], outs[0][f]), f
+
+
 def test_step_corridor_fp32(corridor_small):
     """The fp32 mode of three-circle agents against the fp64 oracle, one
     step at a time along the GPU trajectory (steps 0, 4 and 8 of the
