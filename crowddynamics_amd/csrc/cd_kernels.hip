@@ -1171,6 +1171,12 @@ struct MemRow {
 #pragma unroll
             for (int u = 0; u < kRecBatch; ++u)
               if (s[u] >= 0) fn(-1, rr[u].pos, rr[u].vel, rr[u].r);
+            // The batch's loads have all been consumed; waiting for them
+            // here says so to the compiler, which otherwise holds each
+            // record load of the next batch until the previous one arrives
+            // (their registers might, for all it knows, still be filling).
+            // s_waitcnt vmcnt(0), expcnt and lgkmcnt left at their maxima.
+            __builtin_amdgcn_s_waitcnt(0x0F70);
           }
           return;
         }
