@@ -1213,6 +1213,7 @@ struct MemRow {
 #pragma unroll
           for (int u = 0; u < kIdBatch; ++u)
             if (j[u] >= 0 && j[u] != self) fn(j[u], xj[u], vj[u], rj[u]);
+          __builtin_amdgcn_s_waitcnt(0x0F70);  // as for the records
         } else {
           // one copy of the three-circle pair code for the batch (the id
           // picked by selects, not by indexing): an unrolled batch held
@@ -1224,6 +1225,7 @@ struct MemRow {
             for (int w = 1; w < kIdBatch; ++w) ju = u == w ? j[w] : ju;
             if (ju >= 0) visit(self, ju, fn);
           }
+          __builtin_amdgcn_s_waitcnt(0x0F70);  // as for the records
         }
       }
       return;
