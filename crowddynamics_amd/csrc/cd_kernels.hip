@@ -1503,15 +1503,8 @@ __global__ void __launch_bounds__(256, MODEL == 0 ? 3 : 1) k_forces_direct(Agent
   __shared__ double4 sseg[3 * kWallTile];
   const int2 sk = block_e(B);
   const ScnDesc d = desc_of(B, sk.x);
-  ScnState *ss = &B.sst[sk.x];
-  const int p = FUSED ? ss->par : 0;
-  const bool pairs_on = FUSED ? ((fa.flags & CD_USE_AGENT_AGENT) != 0) : true;
-  // both parities' flags are read with the rest of the state, so that no
-  // load waits for the parity
-  const int oob0 = ss->oob[0], oob1 = ss->oob[1];
-  const bool ok = pairs_on && ss->valid && !(p ? oob1 : oob0);
-  if (!FUSED && !ok) return;
-  const double dt = FUSED ? ss->dt : 0.0;
+  // the walls' staging first: its wave waits for the segments, and its
+  // loads would otherwise queue behind the agent's
   stage_walls(fa, d, sseg, FUSED);
   const int per = (int)blockDim.x >> __builtin_ctz(L);  // L is a power of two
   // the agent: the next in index order, or in the processing order (an
@@ -1530,8 +1523,10 @@ __global__ void __launch_bounds__(256, MODEL == 0 ? 3 : 1) k_forces_direct(Agent
   TCView me;
   Self<T> self;
   Tail<T> tail;
-  // the agent's loads do not wait for the binning's validity: the cell and
-  // slot are read whatever it is and used only when it holds
+  // The agent's loads come first: they need only its index, not the
+  // scenario's state, whose loads would otherwise each wait for the one
+  // before.  The cell and slot are read whatever the binning's validity
+  // and used only when it holds.
   if (have) {
     pos = A.pos[i];
     vel = A.vel[i];
@@ -1541,11 +1536,21 @@ __global__ void __launch_bounds__(256, MODEL == 0 ? 3 : 1) k_forces_direct(Agent
     self = self_of<T, MODEL>(A, i, pos);
     if (FUSED && lead) tail = tail_of(A, i);
   }
+  // the scenario's state, every field read unconditionally (a short-circuit
+  // test would hold each load until the previous one arrived)
+  const ScnState *ss = &B.sst[sk.x];
+  const int par = ss->par, valid = ss->valid, oob0 = ss->oob[0], oob1 = ss->oob[1];
+  const int nx = ss->nx, ny = ss->ny;
+  const double dt_s = ss->dt;
+  const int p = FUSED ? par : 0;
+  const bool pairs_on = FUSED ? ((fa.flags & CD_USE_AGENT_AGENT) != 0) : true;
+  const bool ok = pairs_on & (valid != 0) & ((p ? oob1 : oob0) == 0);
+  if (!FUSED && !ok) return;
+  const double dt = FUSED ? dt_s : 0.0;
   if (!ok) {
     c = d.c0;
     self_slot = -1;
   }
-  const int nx = ss->nx, ny = ss->ny;
   const int lc = c - d.c0, lx = ny > 0 ? lc / ny : 0, ly = lc - lx * ny;
   const MemRow<T, MODEL> rows{A, W, W.count + (size_t)p * W.cap,
                               FUSED ? nbr_snapshot<T>(W) : nbr_agents(A),
