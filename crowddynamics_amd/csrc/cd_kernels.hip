@@ -846,11 +846,11 @@ __device__ __forceinline__ void own_apply(const Agents<T> &A, const FusedArgs &f
 // two parts share no data, so the halves run side by side, and a lone wave
 // per SIMD (one scenario: 128-thread blocks) no longer runs both dependent
 // chains one after the other.
-template <typename T, int MODEL, bool SPLIT>
+template <typename T, int MODEL, bool SPLIT, bool ONE>
 __global__ void __launch_bounds__(256) k_step_begin(Agents<T> A, Work W, Batch B, FusedArgs fa,
                                                     double cell, int binning) {
-  const int2 sk = block_f(B);
-  const ScnDesc d = desc_of(B, sk.x);
+  const int2 sk = block_f_t<ONE>(B);
+  const ScnDesc d = desc_t<ONE>(B, sk.x);
   ScnState *ss = &B.sst[sk.x];
   const uint64_t step = W.st->step;
   const int p = (int)(step & 1ull);
@@ -858,7 +858,7 @@ __global__ void __launch_bounds__(256) k_step_begin(Agents<T> A, Work W, Batch B
   // blocks of the scenario: its block map's, or the whole grid for a single
   // scenario (launched as 64-thread blocks, one wave each, spread over all
   // CUs; 128-thread blocks with SPLIT)
-  const int nblk = B.map_f ? d.nf : (int)gridDim.x;
+  const int nblk = (!ONE && B.map_f) ? d.nf : (int)gridDim.x;
   const int per = SPLIT ? (int)blockDim.x / 2 : (int)blockDim.x;  // agents of a block at once
   const bool own_wave = SPLIT && (int)threadIdx.x >= per;
   const bool bin_wave = !own_wave;
@@ -1496,13 +1496,13 @@ __device__ __forceinline__ void step_end(const Work &W, int scn, int p, Red r, i
 // the standard logic tree, then the integrator and Reset of each agent.
 // Neighbour state comes from the step's snapshot, because agents move as
 // they are integrated; without a valid binning only the walls are added.
-template <typename T, int MODEL, bool FUSED>
+template <typename T, int MODEL, bool FUSED, bool ONE>
 __global__ void __launch_bounds__(256, MODEL == 0 ? 3 : 1) k_forces_direct(Agents<T> A, Work W, Batch B,
                                                        FusedArgs fa, int L, int reset,
                                                        int ordered) {
   __shared__ double4 sseg[3 * kWallTile];
-  const int2 sk = block_e(B);
-  const ScnDesc d = desc_of(B, sk.x);
+  const int2 sk = block_e_t<ONE>(B);
+  const ScnDesc d = desc_t<ONE>(B, sk.x);
   // the walls' staging first: its wave waits for the segments, and its
   // loads would otherwise queue behind the agent's
   stage_walls(fa, d, sseg, FUSED);
@@ -1835,9 +1835,14 @@ void launch_forces_m(const Agents<T> &A, const Work &W, const Batch &B, const Fu
     ta.ty = (int)w->tile_y;
     hipLaunchKernelGGL((k_forces_tiled<T, MODEL, FUSED>), dim3(L.blocks_t), dim3(kTileThreads), 0,
                        s, A, W, B, fa, ta, reset);
+  } else if (!B.desc && !B.map_e) {  // one scenario
+    hipLaunchKernelGGL((k_forces_direct<T, MODEL, FUSED, true>), dim3(L.blocks_e),
+                       dim3(L.threads_e), 0, s, A, W, B, fa, L.lanes, reset,
+                       (FUSED && W.order_every > 0) ? 1 : 0);
   } else {
-    hipLaunchKernelGGL((k_forces_direct<T, MODEL, FUSED>), dim3(L.blocks_e), dim3(L.threads_e), 0,
-                       s, A, W, B, fa, L.lanes, reset, (FUSED && W.order_every > 0) ? 1 : 0);
+    hipLaunchKernelGGL((k_forces_direct<T, MODEL, FUSED, false>), dim3(L.blocks_e),
+                       dim3(L.threads_e), 0, s, A, W, B, fa, L.lanes, reset,
+                       (FUSED && W.order_every > 0) ? 1 : 0);
   }
 }
 
@@ -1873,6 +1878,30 @@ Launch batch_launch(const cd_workspace_t *w) {
 }
 
 
+// K_a of the fused step, for one scenario (ONE) or a batch.
+template <typename T, bool ONE>
+void launch_step_begin_m(const Agents<T> &A, const cd_step_params_t *p, const Work &W,
+                         const Batch &B, const FusedArgs &fa, const Launch &L, hipStream_t s,
+                         int model, int binning, bool split, dim3 grid_s, dim3 block_s) {
+  if (model == CD_THREE_CIRCLE) {
+    if (split)
+      hipLaunchKernelGGL((k_step_begin<T, 1, true, ONE>), grid_s, block_s, 0, s, A, W, B, fa,
+                         p->cell_size, binning);
+    else
+      hipLaunchKernelGGL((k_step_begin<T, 1, false, ONE>), dim3(L.blocks_f), dim3(kBS), 0, s, A,
+                         W, B, fa, p->cell_size, binning);
+  } else if (split) {
+    hipLaunchKernelGGL((k_step_begin<T, 0, true, ONE>), grid_s, block_s, 0, s, A, W, B, fa,
+                       p->cell_size, binning);
+  } else if (B.map_f) {
+    hipLaunchKernelGGL((k_step_begin<T, 0, false, ONE>), dim3(L.blocks_f), dim3(kBS), 0, s, A, W,
+                       B, fa, p->cell_size, binning);
+  } else {  // one scenario: one-wave blocks, so the waves spread over all CUs
+    hipLaunchKernelGGL((k_step_begin<T, 0, false, ONE>), dim3(4 * L.blocks_f), dim3(64), 0, s, A,
+                       W, B, fa, p->cell_size, binning);
+  }
+}
+
 // K_a of the fused step.
 template <typename T>
 void launch_step_begin(const cd_agents_t *a, const cd_step_params_t *p, const Work &W,
@@ -1886,23 +1915,12 @@ void launch_step_begin(const cd_agents_t *a, const cd_step_params_t *p, const Wo
   const bool split = (p->flags & (CD_USE_FLUCT | CD_USE_NAV)) != 0;
   const bool one = !B.map_f;
   const dim3 grid_s(one ? 4 * L.blocks_f : L.blocks_f), block_s(one ? 128 : kBS);
-  if (a->model == CD_THREE_CIRCLE) {
-    if (split)
-      hipLaunchKernelGGL((k_step_begin<T, 1, true>), grid_s, block_s, 0, s, A, W, B, fa,
-                         p->cell_size, binning);
-    else
-      hipLaunchKernelGGL((k_step_begin<T, 1, false>), dim3(L.blocks_f), dim3(kBS), 0, s, A, W,
-                         B, fa, p->cell_size, binning);
-  } else if (split) {
-    hipLaunchKernelGGL((k_step_begin<T, 0, true>), grid_s, block_s, 0, s, A, W, B, fa,
-                       p->cell_size, binning);
-  } else if (!one) {
-    hipLaunchKernelGGL((k_step_begin<T, 0, false>), dim3(L.blocks_f), dim3(kBS), 0, s, A, W, B,
-                       fa, p->cell_size, binning);
-  } else {  // one scenario: one-wave blocks, so the waves spread over all CUs
-    hipLaunchKernelGGL((k_step_begin<T, 0, false>), dim3(4 * L.blocks_f), dim3(64), 0, s, A, W,
-                       B, fa, p->cell_size, binning);
-  }
+  if (one && !B.desc)
+    launch_step_begin_m<T, true>(A, p, W, B, fa, L, s, (int)a->model, binning, split, grid_s,
+                                 block_s);
+  else
+    launch_step_begin_m<T, false>(A, p, W, B, fa, L, s, (int)a->model, binning, split, grid_s,
+                                  block_s);
   if (W.order_every > 0 && binning)
     hipLaunchKernelGGL(k_order_build, dim3(L.scenarios), dim3(kOrderThreads), 0, s, W, B);
 }
