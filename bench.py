@@ -98,7 +98,7 @@ def forces_bytes(n, real_bytes=8):
 
 
 def measured_traffic(kernel, threads):
-    """HBM bytes per launch of `kernel` (e.g. 'k_forces_direct<double, 0, true>')
+    """HBM bytes per launch of `kernel` (e.g. 'k_forces_direct<double, 0, true, true>')
     at a grid of `threads` threads, from the committed summary of the two
     rocprofv3 counter runs (FETCH_SIZE and WRITE_SIZE scaled by a 256 MiB
     copy), or None when it is not listed."""
@@ -726,8 +726,9 @@ def main():
     rb = 8 if main_p == 'f64' else 4
     per_kernel = {}
     tname = 'double' if main_p == 'f64' else 'float'
-    knames = {'step_begin': 'k_step_begin<%s, 0, true>' % tname,
-              'forces': 'k_forces_direct<%s, 0, true>' % tname}
+    # the one-scenario instances (last template argument) of the fused step's kernels
+    knames = {'step_begin': 'k_step_begin<%s, 0, true, true>' % tname,
+              'forces': 'k_forces_direct<%s, 0, true, true>' % tname}
     # grid threads of each launch (the traffic summary is keyed by them): the
     # first kernel of one scenario with fluctuation or navigation runs
     # 4 x blocks_f blocks of 128 threads (binning and own-terms waves)
