@@ -83,7 +83,7 @@ __device__ __forceinline__ double block_sum(double v) {
 // partials k_reduce wrote); one block.  adaptive_timestep,
 // core/integrator.py:29-97.
 __global__ void __launch_bounds__(256) k_adj_dt(Work W, Batch B, int adaptive) {
-  const ScnDesc d = desc_of(B, 0);
+  const ScnDesc d = desc_t<true>(B, 0);  // one scenario (single_batch)
   const Red r = reduce_partials(W, d);
   if (threadIdx.x != 0) return;
   AdjScal *sc = adj_scal(W);
@@ -416,8 +416,8 @@ __device__ __forceinline__ void stencil_vjp(const Agents<double> &A, const Work 
 __global__ void __launch_bounds__(256) k_adjoint(Agents<double> A, Work W, Batch B, AdjView J,
                                                  FusedArgs fa, int L) {
   const int flags = fa.flags;
-  const int2 sk = block_e(B);
-  const ScnDesc d = desc_of(B, sk.x);
+  const int2 sk = block_e_t<true>(B);  // one scenario (single_batch)
+  const ScnDesc d = desc_t<true>(B, sk.x);
   const ScnState *ss = &B.sst[sk.x];
   const int lg = __builtin_ctz(L);  // L is a power of two
   // the agent: the next in index order, or in the forward steps' processing
@@ -538,8 +538,8 @@ __device__ __forceinline__ D1 d1(double v, double t) {
 __global__ void __launch_bounds__(256) k_tangent(Agents<double> A, Work W, Batch B, TanView T,
                                                  FusedArgs fa, int L) {
   const int flags = fa.flags;
-  const int2 sk = block_e(B);
-  const ScnDesc d = desc_of(B, sk.x);
+  const int2 sk = block_e_t<true>(B);  // one scenario (single_batch)
+  const ScnDesc d = desc_t<true>(B, sk.x);
   const ScnState *ss = &B.sst[sk.x];
   const int lg = __builtin_ctz(L);
   int i = d.a0 + sk.y * ((int)blockDim.x >> lg) + ((int)threadIdx.x >> lg);
