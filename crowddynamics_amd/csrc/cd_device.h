@@ -128,7 +128,9 @@ __device__ __forceinline__ double from_ord_bits(unsigned long long b) {
 // wrap_to_pi, core/vector2D.py:8-36.
 __device__ __forceinline__ double wrap_to_pi(double rad) {
   const double two_pi = 2.0 * kPi;
-  double r = fmod(rad, two_pi);
+  // fmod(x, y) is x itself when |x| < |y| (the common case: an angle
+  // already near [-pi, pi]), so the library call is needed only beyond
+  double r = __builtin_fabs(rad) < two_pi ? rad : fmod(rad, two_pi);
   if (r != 0.0) {
     if (r < 0.0) r += two_pi;
   } else {
