@@ -373,30 +373,31 @@ __device__ __forceinline__ int2 block_f(const Batch &B) {
 __device__ __forceinline__ int2 block_e(const Batch &B) {
   return B.map_e ? B.map_e[blockIdx.x] : make_int2(0, (int)blockIdx.x);
 }
-// The same for kernels instantiated for one scenario (ONE): the descriptor
-// comes straight from the kernel arguments and the block needs no map.  A
-// descriptor read that may come from either the arguments or the batch
-// table makes each of its scalar loads wait for the previous one.
+// The same for kernels instantiated for one scenario (ONE: single_batch,
+// whose table and maps are null) or for a batch (batch_of, whose table and
+// maps are all set): the descriptor comes straight from the kernel
+// arguments or straight from the table.  A descriptor read that may come
+// from either makes each of its scalar loads wait for the previous one.
 template <bool ONE>
 __device__ __forceinline__ ScnDesc desc_t(const Batch &B, int s) {
   if constexpr (ONE)
     return B.one;
   else
-    return desc_of(B, s);
+    return B.desc[s];
 }
 template <bool ONE>
 __device__ __forceinline__ int2 block_f_t(const Batch &B) {
   if constexpr (ONE)
     return make_int2(0, (int)blockIdx.x);
   else
-    return block_f(B);
+    return B.map_f[blockIdx.x];
 }
 template <bool ONE>
 __device__ __forceinline__ int2 block_e_t(const Batch &B) {
   if constexpr (ONE)
     return make_int2(0, (int)blockIdx.x);
   else
-    return block_e(B);
+    return B.map_e[blockIdx.x];
 }
 
 constexpr unsigned long long kOrdPosInf = 0xFFF0000000000000ull;  // ord_bits(+inf)

@@ -858,7 +858,7 @@ __global__ void __launch_bounds__(256) k_step_begin(Agents<T> A, Work W, Batch B
   // blocks of the scenario: its block map's, or the whole grid for a single
   // scenario (launched as 64-thread blocks, one wave each, spread over all
   // CUs; 128-thread blocks with SPLIT)
-  const int nblk = (!ONE && B.map_f) ? d.nf : (int)gridDim.x;
+  const int nblk = ONE ? (int)gridDim.x : d.nf;
   const int per = SPLIT ? (int)blockDim.x / 2 : (int)blockDim.x;  // agents of a block at once
   const bool own_wave = SPLIT && (int)threadIdx.x >= per;
   const bool bin_wave = !own_wave;
@@ -1835,7 +1835,7 @@ void launch_forces_m(const Agents<T> &A, const Work &W, const Batch &B, const Fu
     ta.ty = (int)w->tile_y;
     hipLaunchKernelGGL((k_forces_tiled<T, MODEL, FUSED>), dim3(L.blocks_t), dim3(kTileThreads), 0,
                        s, A, W, B, fa, ta, reset);
-  } else if (!B.desc && !B.map_e) {  // one scenario
+  } else if (!B.desc || !B.map_e) {  // one scenario (single_batch: table and maps null)
     hipLaunchKernelGGL((k_forces_direct<T, MODEL, FUSED, true>), dim3(L.blocks_e),
                        dim3(L.threads_e), 0, s, A, W, B, fa, L.lanes, reset,
                        (FUSED && W.order_every > 0) ? 1 : 0);
@@ -1915,7 +1915,7 @@ void launch_step_begin(const cd_agents_t *a, const cd_step_params_t *p, const Wo
   const bool split = (p->flags & (CD_USE_FLUCT | CD_USE_NAV)) != 0;
   const bool one = !B.map_f;
   const dim3 grid_s(one ? 4 * L.blocks_f : L.blocks_f), block_s(one ? 128 : kBS);
-  if (one && !B.desc)
+  if (one || !B.desc)  // single_batch: table and maps null
     launch_step_begin_m<T, true>(A, p, W, B, fa, L, s, (int)a->model, binning, split, grid_s,
                                  block_s);
   else
